@@ -1,0 +1,109 @@
+// Dense-grid template build from the sparse spline coefficients
+// (the reference's InterpTDI / TDI kernel, WaveformBuild.cu:41-189).
+//
+// The reference launches one kernel per binary on its own stream. Here all
+// binaries go in ONE launch: each binary's band [start, start + n_b) is cut
+// into tiles of TILE dense bins; block -> (binary, tile) by a binary search
+// over the per-binary tile offsets (ragged CSR). One lane per dense bin sums
+// the modes:
+//   h_ch(f) = sum_m conj(T_ch,m(f) A_m(f) e^{i phi_m(f)})  if t_start <= t_m(f) <= t_end
+// with every quantity a cubic in (f - f_k) on the segment k = inds[m][i].
+
+#include <vector>
+
+#include "bbhx_hip.h"
+#include "common.h"
+#include "interp_device.h"
+#include "workspace.h"
+
+using namespace bbhx;
+
+namespace {
+
+__global__ __launch_bounds__(INTERP_TILE) void k_interp_tdi(const InterpBin* __restrict__ bins, int numBinAll,
+                                                            const double* __restrict__ dataFreqs,
+                                                            const double* __restrict__ freqs,
+                                                            const double* __restrict__ y,
+                                                            const double* __restrict__ c1,
+                                                            const double* __restrict__ c2,
+                                                            const double* __restrict__ c3, int length, int numModes,
+                                                            long nper)
+{
+    const int b = find_bin(bins, numBinAll, (int)blockIdx.x);
+    const InterpBin& d = bins[b];
+    const int i = ((int)blockIdx.x - d.tile_begin) * INTERP_TILE + threadIdx.x;
+    if (i >= d.len) return;
+    const double f = dataFreqs[d.start + i];
+    cd h[3];
+    interp_point(h, f, i, d, b, freqs, y, c1, c2, c3, length, numModes, nper);
+    cd* out = reinterpret_cast<cd*>(d.tmpl);
+    out[0 * (long)d.len + i] = h[0];
+    out[1 * (long)d.len + i] = h[1];
+    out[2 * (long)d.len + i] = h[2];
+}
+
+}  // namespace
+
+namespace bbhx {
+// Uploads a host table of per-binary descriptors to the table workspace.
+// Synchronises the stream so the host vector may be released on return.
+InterpBin* upload_table(const std::vector<InterpBin>& h, hipStream_t stream)
+{
+    InterpBin* d = static_cast<InterpBin*>(workspace_get(WS_TABLE, sizeof(InterpBin) * (h.size() + 1)));
+    if (!d) return nullptr;
+    if (hipMemcpyAsync(d, h.data(), sizeof(InterpBin) * h.size(), hipMemcpyHostToDevice, stream) != hipSuccess)
+        return nullptr;
+    if (hipStreamSynchronize(stream) != hipSuccess) return nullptr;
+    return d;
+}
+
+// Host table for ragged bands cut in tiles of 'tile' dense bins; returns the
+// number of tiles, or -1 when a band is invalid.
+long build_table(std::vector<InterpBin>& h, const int64_t* tmpl_ptrs, const int64_t* inds_ptrs,
+                 const int* inds_start, const int* ind_lengths, const double* t_start, const double* t_end,
+                 int numBinAll, int tile)
+{
+    h.resize(numBinAll);
+    long tiles = 0;
+    for (int b = 0; b < numBinAll; b++) {
+        if (ind_lengths[b] < 0 || inds_start[b] < 0) return -1;
+        h[b].tmpl = reinterpret_cast<double*>(tmpl_ptrs ? tmpl_ptrs[b] : 0);
+        h[b].inds = reinterpret_cast<const int*>(inds_ptrs ? inds_ptrs[b] : 0);
+        h[b].start = inds_start[b];
+        h[b].len = ind_lengths[b];
+        h[b].t_start = t_start ? t_start[b] : 0.0;
+        h[b].t_end = t_end ? t_end[b] : 0.0;
+        h[b].tile_begin = (int)tiles;
+        h[b].pad = 0;
+        tiles += (ind_lengths[b] + tile - 1) / tile;
+        if (tiles > 2147483647L) return -1;
+    }
+    return tiles;
+}
+}  // namespace bbhx
+
+extern "C" int bbhx_hip_InterpTDI(const int64_t* templateChannels_ptrs, const double* dataFreqs, const double* freqs,
+                                  const double* propArrays, const double* c1, const double* c2, const double* c3,
+                                  const double* t_start, const double* t_end, int length, int data_length,
+                                  int numBinAll, int numModes, const int64_t* inds_ptrs, const int* inds_start,
+                                  const int* ind_lengths, void* stream)
+{
+    if (length < 2 || data_length < 0 || numBinAll < 0 || numModes < 1 || numModes > MAX_MODES) return ERR_ARGS;
+    if (numBinAll == 0) return OK;
+    if (!templateChannels_ptrs || !dataFreqs || !freqs || !propArrays || !c1 || !c2 || !c3 || !t_start || !t_end ||
+        !inds_ptrs || !inds_start || !ind_lengths)
+        return ERR_ARGS;
+    for (int b = 0; b < numBinAll; b++)
+        if ((long)inds_start[b] + ind_lengths[b] > data_length) return ERR_ARGS;
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    std::vector<InterpBin> h;
+    long tiles = build_table(h, templateChannels_ptrs, inds_ptrs, inds_start, ind_lengths, t_start, t_end, numBinAll,
+                             INTERP_TILE);
+    if (tiles < 0) return ERR_ARGS;
+    if (tiles == 0) return OK;
+    InterpBin* bins = upload_table(h, st);
+    if (!bins) return ERR_ALLOC;
+    hipLaunchKernelGGL(k_interp_tdi, dim3((unsigned)tiles), dim3(INTERP_TILE), 0, st, bins, numBinAll, dataFreqs, freqs,
+                       propArrays, c1, c2, c3, length, numModes, (long)numBinAll * numModes * length);
+    return hipGetLastError() == hipSuccess ? OK : ERR_LAUNCH;
+}

@@ -1,0 +1,825 @@
+// PhenomD / PhenomHM amplitude, phase and t(f) on gfx950.
+//
+// The work is split in two so that the per-(binary, mode) coefficient setup
+// (about 3k FP64 flops, ~40 transcendentals) is done once by one lane instead
+// of once per frequency:
+//   phm_setup(...)  fills a PhmRec for one (binary, mode)      [one lane]
+//   phm_eval(...)   evaluates amp, phase, t(f) at one frequency [one lane per point]
+//
+// Physics and calibrated coefficients follow the reference's
+// cutils/PhenomHMWaveform.cu (LALSimIMRPhenomD/HM; arXiv:1508.07250,
+// 1508.07253, 1708.00404, QNM fits arXiv:1810.03550); the numbers of the fits
+// are physics tables and are necessarily identical.
+#pragma once
+
+#include "common.h"
+
+namespace bbhx {
+namespace phm {
+
+constexpr double AMP_JOIN_INS = 0.014;  // PhenomHMWaveform.hh:62
+constexpr double PHI_JOIN_INS = 0.018;  // PhenomHMWaveform.hh:68
+
+// ---------------------------------------------------------------- fits
+// Calibrated fits a0 + a1 eta + (b0 + b1 eta + b2 eta^2) xi + (c..) xi^2 + (d..) xi^3
+// with xi = chi_PN - 1 (arXiv:1508.07253 Table 5; PhenomHMWaveform.cu:390-1132).
+enum FitId {
+    F_RHO1, F_RHO2, F_RHO3, F_GAMMA1, F_GAMMA2, F_GAMMA3, F_V2,
+    F_ALPHA1, F_ALPHA2, F_ALPHA3, F_ALPHA4, F_ALPHA5, F_BETA1, F_BETA2, F_BETA3,
+    F_SIGMA1, F_SIGMA2, F_SIGMA3, F_SIGMA4, F_COUNT
+};
+
+__device__ const double FITS[F_COUNT][11] = {
+    {3931.8979897196696, -17395.758706812805, 3132.375545898835, 343965.86092361377, -1.2162565819981997e6,
+     -70698.00600428853, 1.383907177859705e6, -3.9662761890979446e6, -60017.52423652596, 803515.1181825735,
+     -2.091710365941658e6},
+    {-40105.47653771657, 112253.0169706701, 23561.696065836168, -3.476180699403351e6, 1.137593670849482e7,
+     754313.1127166454, -1.308476044625268e7, 3.6444584853928134e7, 596226.612472288, -7.4277901143564405e6,
+     1.8928977514040343e7},
+    {83208.35471266537, -191237.7264145924, -210916.2454782992, 8.71797508352568e6, -2.6914942420669552e7,
+     -1.9889806527362722e6, 3.0888029960154563e7, -8.390870279256162e7, -1.4535031953446497e6,
+     1.7063528990822166e7, -4.2748659731120914e7},
+    {0.006927402739328343, 0.03020474290328911, 0.006308024337706171, -0.12074130661131138, 0.26271598905781324,
+     0.0034151773647198794, -0.10779338611188374, 0.27098966966891747, 0.0007374185938559283,
+     -0.02749621038376281, 0.0733150789135702},
+    {1.010344404799477, 0.0008993122007234548, 0.283949116804459, -4.049752962958005, 13.207828172665366,
+     0.10396278486805426, -7.025059158961947, 24.784892370130475, 0.03093202475605892, -2.6924023896851663,
+     9.609374464684983},
+    {1.3081615607036106, -0.005537729694807678, -0.06782917938621007, -0.6689834970767117, 3.403147966134083,
+     -0.05296577374411866, -0.9923793203111362, 4.820681208409587, -0.006134139870393713, -0.38429253308696365,
+     1.7561754421985984},
+    {0.8149838730507785, 2.5747553517454658, 1.1610198035496786, -2.3627771785551537, 6.771038707057573,
+     0.7570782938606834, -2.7256896890432474, 7.1140380397149965, 0.1766934149293479, -0.7978690983168183,
+     2.1162391502005153},
+    {43.31514709695348, 638.6332679188081, -32.85768747216059, 2415.8938269370315, -5766.875169379177,
+     -61.85459307173841, 2953.967762459948, -8986.29057591497, -21.571435779762044, 981.2158224673428,
+     -3239.5664895930286},
+    {-0.07020209449091723, -0.16269798450687084, -0.1872514685185499, 1.138313650449945, -2.8334196304430046,
+     -0.17137955686840617, 1.7197549338119527, -4.539717148261272, -0.049983437357548705, 0.6062072055948309,
+     -1.682769616644546},
+    {9.5988072383479, -397.05438595557433, 16.202126189517813, -1574.8286986717037, 3600.3410843831093,
+     27.092429659075467, -1786.482357315139, 5152.919378666511, 11.175710130033895, -577.7999423177481,
+     1808.730762932043},
+    {-0.02989487384493607, 1.4022106448583738, -0.07356049468633846, 0.8337006542278661, 0.2240008282397391,
+     -0.055202870001177226, 0.5667186343606578, 0.7186931973380503, -0.015507437354325743, 0.15750322779277187,
+     0.21076815715176228},
+    {0.9974408278363099, -0.007884449714907203, -0.059046901195591035, 1.3958712396764088, -4.516631601676276,
+     -0.05585343136869692, 1.7516580039343603, -5.990208965347804, -0.017945336522161195, 0.5965097794825992,
+     -2.0608879367971804},
+    {97.89747327985583, -42.659730877489224, 153.48421037904913, -1417.0620760768954, 2752.8614143665027,
+     138.7406469558649, -1433.6585075135881, 2857.7418952430758, 41.025109467376126, -423.680737974639,
+     850.3594335657173},
+    {-3.282701958759534, -9.051384468245866, -12.415449742258042, 55.4716447709787, -106.05109938966335,
+     -11.953044553690658, 76.80704618365418, -155.33172948098394, -3.4129261592393263, 25.572377569952536,
+     -54.408036707740465},
+    {-0.000025156429818799565, 0.000019750256942201327, -0.000018370671469295915, 0.000021886317041311973,
+     0.00008250240316860033, 7.157371250566708e-6, -0.000055780000112270685, 0.00019142082884072178,
+     5.447166261464217e-6, -0.00003220610095021982, 0.00007974016714984341},
+    {2096.551999295543, 1463.7493168261553, 1312.5493286098522, 18307.330017082117, -43534.1440746107,
+     -833.2889543511114, 32047.31997183187, -108609.45037520859, 452.25136398112204, 8353.439546391714,
+     -44531.3250037322},
+    {-10114.056472621156, -44631.01109458185, -6541.308761668722, -266959.23419307504, 686328.3229317984,
+     3405.6372187679685, -437507.7208209015, 1.6318171307344697e6, -7462.648563007646, -114585.25177153319,
+     674402.4689098676},
+    {22933.658273436497, 230960.00814979506, 14961.083974183695, 1.1940181342318142e6, -3.1042239693052764e6,
+     -3038.166617199259, 1.8720322849093592e6, -7.309145012085539e6, 42738.22871475411, 467502.018616601,
+     -3.064853498512499e6},
+    {-14621.71522218357, -377812.8579387104, -9608.682631509726, -1.7108925257214056e6, 4.332924601416521e6,
+     -22366.683262266528, -2.5019716386377467e6, 1.0274495902259542e7, -85360.30079034246, -570025.3441737515,
+     4.396844346849777e6},
+};
+
+__device__ inline double fit(int id, double eta, double chi)
+{
+    const double* k = FITS[id];
+    double xi = -1 + chi;
+    double xi2 = xi * xi, xi3 = xi2 * xi, eta2 = eta * eta;
+    return k[0] + k[1] * eta + (k[2] + k[3] * eta + k[4] * eta2) * xi + (k[5] + k[6] * eta + k[7] * eta2) * xi2 +
+           (k[8] + k[9] * eta + k[10] * eta2) * xi3;
+}
+
+// Powers of x built from x^(1/6) (PhenomHMWaveform.cu:354-368).
+struct P6 {
+    double sixth, third, two_thirds, four_thirds, five_thirds, two, seven_thirds, eight_thirds;
+};
+
+__device__ inline P6 pow6(double x)
+{
+    P6 p;
+    p.sixth = pow(x, 1 / 6.0);
+    p.third = p.sixth * p.sixth;
+    p.two_thirds = x / p.third;
+    p.four_thirds = x * p.third;
+    p.five_thirds = p.four_thirds * p.third;
+    p.two = x * x;
+    p.seven_thirds = p.third * p.two;
+    p.eight_thirds = p.two_thirds * p.two;
+    return p;
+}
+
+// ------------------------------------------------------------- record
+
+// Everything point evaluation needs for one (binary, mode). ~100 doubles.
+struct PhmRec {
+    // binary level
+    double M_sec, amp_scale, t0, phi0, Mf_ref;
+    // leading-order PN amplitude model (IMRPhenomHMOnePointFiveSpinPN inputs)
+    double pn_M, pn_eta, pn_delta, pn_Xs, pn_Xa, pn_pref;
+    // PhenomD amplitude (mode independent)
+    double a_fRD, a_fDM, a_g1, a_g2, a_g3, a_fmax;
+    double a_rho1, a_rho2, a_rho3, a_d0, a_d1, a_d2, a_d3, a_d4;
+    double a_pf23, a_pf1, a_pf43, a_pf53, a_pf2, a_amp0;
+    // PhenomD phase with this mode's ringdown ratios
+    double p_eta, p_etaInv, p_fRD, p_fDM, p_Rholm, p_Taulm;
+    double beta1, beta2, beta3, alpha1, alpha2, alpha3, alpha4, alpha5;
+    double sigma1, sigma2, sigma3, sigma4;
+    double C1Int, C2Int, C1MRD, C2MRD, fMRDJoin;
+    double pn_v0, pn_v2, pn_v3, pn_v4, pn_v5, pn_v6, pn_v7, pn_vl5, pn_vl6;
+    double q_init, q_23, q_13, q_13log, q_logv, q_m13, q_m23, q_m1, q_m53, q_1, q_43, q_53, q_2;
+    // HM frequency maps: phase (a*, b*, fi, fr) and amplitude (amp_*)
+    double ai, bi, am, bm, ar, br, fi, fr, PhDBconst, PhDCconst, PhDBAterm, phaseC;
+    double amp_fi, amp_fr, amp_Ai, amp_Am, amp_Bm, amp_Br;
+    double cshift;
+    int ell, mm, phenomd, pad;
+};
+
+// ------------------------------------------------------------ amplitude
+
+__device__ inline double amp_ins(const PhmRec& r, double Mf, const P6& p)
+{
+    double Mf2 = p.two, Mf3 = Mf * Mf2;
+    return 1 + p.two_thirds * r.a_pf23 + Mf * r.a_pf1 + p.four_thirds * r.a_pf43 + p.five_thirds * r.a_pf53 +
+           Mf2 * r.a_pf2 + p.seven_thirds * r.a_rho1 + p.eight_thirds * r.a_rho2 + Mf3 * r.a_rho3;
+}
+
+__device__ inline double amp_mrd(const PhmRec& r, double f)
+{
+    double fDMg3 = r.a_fDM * r.a_g3;
+    double d = f - r.a_fRD;
+    return exp(-(d)*r.a_g2 / (fDMg3)) * (fDMg3 * r.a_g1) / (d * d + fDMg3 * fDMg3);
+}
+
+__device__ inline double amp_mrd_deriv(const PhmRec& r, double f)
+{
+    double fDMg3 = r.a_fDM * r.a_g3;
+    double d = f - r.a_fRD;
+    double e = exp((d * r.a_g2) / (fDMg3));
+    double s = d * d + fDMg3 * fDMg3;
+    return (-2 * r.a_fDM * (d)*r.a_g3 * r.a_g1) / (e * (s * s)) - (r.a_g2 * r.a_g1) / (e * (s));
+}
+
+// IMRPhenDAmplitude without the distance factor (PhenomHMWaveform.cu:1508-1537)
+__device__ inline double amp_imr(const PhmRec& r, double f, const P6& p)
+{
+    double pre = r.a_amp0 / (f * p.sixth);
+    if (!(f >= AMP_JOIN_INS)) return pre * amp_ins(r, f, p);
+    if (f >= r.a_fmax) return pre * amp_mrd(r, f);
+    double f2 = f * f, f3 = f * f2, f4 = f * f3;
+    return pre * (r.a_d0 + r.a_d1 * f + r.a_d2 * f2 + r.a_d3 * f3 + r.a_d4 * f4);
+}
+
+// Leading-order PN amplitude of the (l, m) multipole, IMRPhenomHMOnePointFiveSpinPN
+// (PhenomHMWaveform.cu:2375-2455), with the mass/spin combinations precomputed.
+__device__ inline double pn_amp(const PhmRec& r, double fM, int l, int m, bool spinless22)
+{
+    double v = pow(r.pn_M * 2.0 * PI * fM / m, 1.0 / 3.0);
+    double v2 = v * v, v3 = v * v2;
+    double habs;
+    const double eta = r.pn_eta, delta = r.pn_delta;
+    if (spinless22 || (l == 2 && m == 2)) {
+        habs = 1.0;
+    } else if (l == 2 && m == 1) {
+        double v4 = v * v3;
+        const double Xs = r.pn_Xs, Xa = r.pn_Xa;
+        double re = v * delta - v2 * 1.5 * (Xa + delta * Xs) + v3 * delta * ((335.0 / 672.0) + (eta * 117.0 / 56.0)) +
+                    v4 * (Xa * (3427.0 / 1344 - eta * 2101.0 / 336) + delta * Xs * (3427.0 / 1344 - eta * 965 / 336) +
+                          delta * (-PI));
+        double im = v4 * (delta * (-0.5 - 2.0 * 0.69314718056));
+        const double pre = (sqrt(2.0) / 3.0);
+        habs = hypot(pre * re, pre * im);
+    } else if (l == 3 && m == 3) {
+        habs = fabs(0.75 * sqrt(5.0 / 7.0) * (v * delta));
+    } else if (l == 3 && m == 2) {
+        habs = fabs((1.0 / 3.0) * sqrt(5.0 / 7.0) * (v2 * (1.0 - 3.0 * eta)));
+    } else if (l == 4 && m == 4) {
+        habs = fabs((4.0 / 9.0) * sqrt(10.0 / 7.0) * v2 * (1.0 - 3.0 * eta));
+    } else if (l == 4 && m == 3) {
+        habs = fabs(0.75 * sqrt(3.0 / 35.0) * v3 * delta * (1.0 - 2.0 * eta));
+    } else {
+        habs = 0.0;
+    }
+    return r.pn_pref * pow(v, -3.5) * habs;
+}
+
+// ---------------------------------------------------------------- phase
+
+__device__ inline double phi_ins(const PhmRec& r, double Mf, const P6& p, double logv)
+{
+    double ph = r.q_init;
+    ph += r.q_23 * p.two_thirds;
+    ph += r.q_13 * p.third;
+    ph += r.q_13log * logv * p.third;
+    ph += r.q_logv * logv;
+    ph += r.q_m13 / p.third;
+    ph += r.q_m23 / p.two_thirds;
+    ph += r.q_m1 / Mf;
+    ph += r.q_m53 / p.five_thirds;
+    ph += (r.q_1 * Mf + r.q_43 * p.four_thirds + r.q_53 * p.five_thirds + r.q_2 * p.two) / r.p_eta;
+    return ph;
+}
+
+// d phi_ins / dMf with v = (pi Mf)^(1/3) given
+__device__ inline double dphi_ins(const PhmRec& r, double v, double logv)
+{
+    const double v2 = v * v, v3 = v * v2, v4 = v * v3, v5 = v * v4, v6 = v * v5, v7 = v * v6, v8 = v * v7;
+    double D = 0.0;
+    D += +2.0 * r.pn_v7 * v7;
+    D += (r.pn_v6 + r.pn_vl6 * (1.0 + logv)) * v6;
+    D += r.pn_vl5 * v5;
+    D += -1.0 * r.pn_v4 * v4;
+    D += -2.0 * r.pn_v3 * v3;
+    D += -3.0 * r.pn_v2 * v2;
+    D += -5.0 * r.pn_v0;
+    D /= v8 * 3.0 / PI;
+    const double pi13 = 1.4645918875615231, pi23 = 2.1450293971110255;  // pi^(1/3), pi^(2/3)
+    D += (r.sigma1 + r.sigma2 * v / pi13 + r.sigma3 * v2 / pi23 + (r.sigma4 / PI) * v3) / r.p_eta;
+    return D;
+}
+
+__device__ inline double phi_int(const PhmRec& r, double Mf)
+{
+    return r.beta1 * Mf - r.beta3 / (3. * (Mf * Mf * Mf)) + r.beta2 * log(Mf);
+}
+
+__device__ inline double dphi_int(const PhmRec& r, double Mf)
+{
+    double Mf2 = Mf * Mf;
+    return (r.beta1 + r.beta3 / (Mf2 * Mf2) + r.beta2 / Mf) / r.p_eta;
+}
+
+__device__ inline double phi_mrd(const PhmRec& r, double f)
+{
+    double sf = sqrt(f);
+    double f075 = sqrt(f * sf);
+    return -(r.alpha2 / f) + (4.0 / 3.0) * (r.alpha3 * f075) + r.alpha1 * f +
+           r.alpha4 * r.p_Rholm * atan((f - r.alpha5 * r.p_fRD) / (r.p_Rholm * r.p_fDM * r.p_Taulm));
+}
+
+__device__ inline double dphi_mrd(const PhmRec& r, double f)
+{
+    double num = f - r.alpha5 * r.p_fRD;
+    double den = r.p_fDM * r.p_Taulm * r.p_Rholm;
+    return (r.alpha1 + r.alpha2 / (f * f) + r.alpha3 / pow(f, 0.25) +
+            r.alpha4 / (r.p_fDM * r.p_Taulm * (1 + (num * num) / (den * den)))) *
+           r.p_etaInv;
+}
+
+// IMRPhenDPhase and its derivative at the same Mf, sharing the powers.
+__device__ inline void phase_imr(const PhmRec& r, double f, double& ph, double* dph)
+{
+    if (!(f >= PHI_JOIN_INS)) {
+        P6 p = pow6(f);
+        const double pi13 = 1.4645918875615231;
+        double v = p.third * pi13;
+        double logv = log(v);
+        ph = phi_ins(r, f, p, logv);
+        if (dph) *dph = dphi_ins(r, v, logv);
+        return;
+    }
+    if (f >= r.fMRDJoin) {
+        ph = r.p_etaInv * phi_mrd(r, f) + r.C1MRD + r.C2MRD * f;
+        if (dph) *dph = dphi_mrd(r, f) + r.C2MRD;
+        return;
+    }
+    ph = r.p_etaInv * phi_int(r, f) + r.C1Int + r.C2Int * f;
+    if (dph) *dph = dphi_int(r, f) + r.C2Int;
+}
+
+__device__ inline double phase_only(const PhmRec& r, double f)
+{
+    double ph;
+    phase_imr(r, f, ph, nullptr);
+    return ph;
+}
+
+__device__ inline double dphase_only(const PhmRec& r, double f)
+{
+    if (!(f >= PHI_JOIN_INS)) {
+        double v = cbrt(PI * f);
+        return dphi_ins(r, v, log(v));
+    }
+    if (f >= r.fMRDJoin) return dphi_mrd(r, f) + r.C2MRD;
+    return dphi_int(r, f) + r.C2Int;
+}
+
+// -------------------------------------------------------- final state / QNM
+
+__device__ inline void enforce_primary(double& m1, double& m2, double& c1, double& c2)
+{
+    if (!(m1 > m2)) {
+        double t = m1; m1 = m2; m2 = t;
+        t = c1; c1 = c2; c2 = t;
+    }
+}
+
+// IMRPhenomDFinalMass / FinalSpin (PhenomHMWaveform.cu:1679-1814)
+__device__ inline void final_state(double m1, double m2, double chi1, double chi2, double& finmass, double& finspin)
+{
+    enforce_primary(m1, m2, chi1, chi2);
+    double M = m1 + m2;
+    double eta = m1 * m2 / (M * M);
+    double Seta = sqrt(1.0 - 4.0 * eta);
+    double q1 = 0.5 * (1.0 + Seta), q2 = 0.5 * (1.0 - Seta);
+    double q1s = q1 * q1, q2s = q2 * q2;
+    double eta2 = eta * eta, eta3 = eta2 * eta, eta4 = eta3 * eta;
+    double s = (q1s * chi1 + q2s * chi2) / (q1s + q2s);
+    double erad = ((0.055974469826360077 * eta + 0.5809510763115132 * eta2 - 0.9606726679372312 * eta3 +
+                    3.352411249771192 * eta4) *
+                   (1. + (-0.0030302335878845507 - 2.0066110851351073 * eta + 7.7050567802399215 * eta2) * s)) /
+                  (1. + (-0.6714403054720589 - 1.4756929437702908 * eta + 7.304676214885011 * eta2) * s);
+    finmass = 1.0 - erad;
+    double sf = (q1s * chi1 + q2s * chi2);
+    double sf2 = sf * sf, sf3 = sf2 * sf, sf4 = sf3 * sf;
+    finspin = 3.4641016151377544 * eta - 4.399247300629289 * eta2 + 9.397292189321194 * eta3 -
+              13.180949901606242 * eta4 + (1 - 0.0850917821418767 * eta - 5.837029316602263 * eta2) * sf +
+              (0.1014665242971878 * eta - 2.0967746996832157 * eta2) * sf2 +
+              (-1.3546806617824356 * eta + 4.108962025369336 * eta2) * sf3 +
+              (-0.8676969352555539 * eta + 2.064046835273906 * eta2) * sf4;
+}
+
+// Complex QNM frequency fit CW07102016 (n = 0) of the (l, m) mode in units of 1/M_final;
+// returns false for a mode without a fit. (PhenomHMWaveform.cu:56-211)
+__device__ inline bool qnm_cw(double kappa, int l, int m_in, cd& out)
+{
+    int m = m_in < 0 ? -m_in : m_in;
+    // amplitude / phase pairs per power of kappa; 'base' is the kappa^0 real offset for
+    // zero-damped modes, whose series starts at kappa^1
+    double A[7], P[7];
+    int n;
+    double base;
+    bool zd;
+    if (l == 2 && m == 2) {
+        const double a[5] = {1.557847, 1.95097051, 2.09971716, 1.41094660, 0.41063923};
+        const double p[5] = {2.903124, 5.920970, 2.760585, 5.914340, 2.795235};
+        n = 5; base = 1.0; zd = true;
+        for (int i = 0; i < 5; i++) { A[i] = a[i]; P[i] = p[i]; }
+    } else if (l == 3 && m == 2) {
+        const double a[7] = {1.022464, 0.24731213, 1.70468239, 0.94604882, 1.53189884, 2.28052668, 0.92150314};
+        const double p[7] = {0.004870, 0.665292, 3.138283, 0.163247, 5.703573, 2.685231, 5.841704};
+        n = 7; base = 0.0; zd = false;
+        for (int i = 0; i < 7; i++) { A[i] = a[i]; P[i] = p[i]; }
+    } else if (l == 4 && m == 4) {
+        const double a[5] = {2.658908, 2.97825567, 3.21842350, 2.12764967, 0.60338186};
+        const double p[5] = {3.002787, 6.050955, 2.877514, 5.989669, 2.830031};
+        n = 5; base = 2.0; zd = true;
+        for (int i = 0; i < 5; i++) { A[i] = a[i]; P[i] = p[i]; }
+    } else if (l == 2 && m == 1) {
+        const double a[7] = {0.589113, 0.18896353, 1.15012965, 6.04585476, 11.12627777, 9.34711461, 3.03838318};
+        const double p[7] = {0.043525, 2.289868, 5.810057, 2.741967, 5.844130, 2.669372, 5.791518};
+        n = 7; base = 0.0; zd = false;
+        for (int i = 0; i < 7; i++) { A[i] = a[i]; P[i] = p[i]; }
+    } else if (l == 3 && m == 3) {
+        const double a[5] = {2.095657, 2.46964352, 2.66552551, 1.75836443, 0.49905688};
+        const double p[5] = {2.964973, 5.996734, 2.817591, 5.932693, 2.781658};
+        n = 5; base = 1.5; zd = true;
+        for (int i = 0; i < 5; i++) { A[i] = a[i]; P[i] = p[i]; }
+    } else if (l == 4 && m == 3) {
+        const double a[5] = {0.205046, 3.10333396, 4.23612166, 3.02890198, 0.90843949};
+        const double p[5] = {0.595328, 3.016200, 6.038842, 2.826239, 5.915164};
+        n = 5; base = 1.5; zd = true;
+        for (int i = 0; i < 5; i++) { A[i] = a[i]; P[i] = p[i]; }
+    } else {
+        out = mk(0.0, 0.0);
+        return false;
+    }
+    cd sum = mk(0.0, 0.0);
+    double kp = 1.0;
+    for (int i = 0; i < n; i++) {
+        double s, c;
+        sincos(P[i], &s, &c);
+        sum += (A[i] * kp) * mk(c, s);
+        kp *= kappa;
+    }
+    cd ans = zd ? mk(base, 0.0) + kappa * sum : sum;
+    if (m_in < 0) ans = mk(-ans.re, ans.im);
+    out = ans;
+    return true;
+}
+
+__device__ inline bool hm_ringdown(int ell, int mm, double finmass, double finspin, double& frd, double& fdm)
+{
+    int am = mm < 0 ? -mm : mm;
+    double kappa = pow(log(2.0 - finspin) / log(3.0), 1.0 / (2.0 + ell - am));
+    cd z;
+    bool ok = qnm_cw(kappa, ell, mm, z);
+    const double inv2pi = 0.5 / PI;
+    frd = inv2pi * z.re / finmass;
+    fdm = inv2pi * z.im / finmass;
+    return ok;
+}
+
+// ---------------------------------------------------------------- setup
+
+// TaylorF2 + PhenomD phase coefficients + C1 joins; amplitude coefficients when
+// 'with_amp' (PhenomHMWaveform.cu:253-335, :446-874, :1261-1339, :1587-1644).
+// The primary is enforced on the copies again, as the reference does.
+__device__ inline void setup_phenomd(PhmRec& r, double m1, double m2, double chi1, double chi2, double Rholm,
+                                     double Taulm, double fring, double fdamp, bool with_amp)
+{
+    enforce_primary(m1, m2, chi1, chi2);
+    const double Mtot = m1 + m2;
+    const double eta = m1 * m2 / (Mtot * Mtot);
+    const double Seta = sqrt(1.0 - 4.0 * eta);
+    const double chi = (chi1 + chi2) / 2.0 * (1.0 - eta * 76.0 / 113.0) + Seta * ((chi1 - chi2) / 2.0);
+
+    r.p_eta = eta;
+    r.p_etaInv = 1.0 / eta;
+    r.p_fRD = fring;
+    r.p_fDM = fdamp;
+    r.p_Rholm = Rholm;
+    r.p_Taulm = Taulm;
+    r.sigma1 = fit(F_SIGMA1, eta, chi);
+    r.sigma2 = fit(F_SIGMA2, eta, chi);
+    r.sigma3 = fit(F_SIGMA3, eta, chi);
+    r.sigma4 = fit(F_SIGMA4, eta, chi);
+    r.beta1 = fit(F_BETA1, eta, chi);
+    r.beta2 = fit(F_BETA2, eta, chi);
+    r.beta3 = fit(F_BETA3, eta, chi);
+    r.alpha1 = fit(F_ALPHA1, eta, chi);
+    r.alpha2 = fit(F_ALPHA2, eta, chi);
+    r.alpha3 = fit(F_ALPHA3, eta, chi);
+    r.alpha4 = fit(F_ALPHA4, eta, chi);
+    r.alpha5 = fit(F_ALPHA5, eta, chi);
+
+    // TaylorF2 aligned-spin phasing minus the 3PN spin-spin term
+    {
+        const double d = (m1 - m2) / (m1 + m2);
+        const double eta_pn = m1 * m2 / Mtot / Mtot;
+        const double m1M = m1 / Mtot, m2M = m2 / Mtot;
+        const double SL = m1M * m1M * chi1 + m2M * m2M * chi2;
+        const double dSigmaL = d * (m2M * chi2 - m1M * chi1);
+        const double pfaN = 3. / (128. * eta_pn);
+        const double e = eta_pn;
+        double v0 = 1. * pfaN;
+        double v2 = pfaN * (5. * (743. / 84. + 11. * e) / 9.);
+        double v3 = pfaN * (-16. * PI);
+        double v4 = pfaN * (5. * (3058.673 / 7.056 + 5429. / 7. * e + 617. * e * e) / 72.);
+        double v5 = pfaN * (5. / 9. * (7729. / 84. - 13. * e) * PI);
+        double vl5 = pfaN * (5. / 3. * (7729. / 84. - 13. * e) * PI);
+        double v6 = pfaN * ((11583.231236531 / 4.694215680 - 640. / 3. * PI * PI - 6848. / 21. * GAMMA_E) +
+                            e * (-15737.765635 / 3.048192 + 2255. / 12. * PI * PI) + e * e * 76055. / 1728. -
+                            e * e * e * 127825. / 1296.);
+        v6 += pfaN * (-6848. / 21.) * 1.3862943611198906;  // log(4)
+        double vl6 = pfaN * (-6848. / 21.);
+        double v7 = pfaN * (PI * (77096675. / 254016. + 378515. / 1512. * e - 74045. / 756. * e * e));
+        double sig = e * (721. / 48. * chi1 * chi2 - 247. / 48. * (chi1 * chi2));
+        sig += (720. - 1.) / 96.0 * m1M * m1M * chi1 * chi1;
+        sig += (720. - 1.) / 96.0 * m2M * m2M * chi2 * chi2;
+        sig -= (240. - 7.) / 96.0 * m1M * m1M * (chi1 * chi1);
+        sig -= (240. - 7.) / 96.0 * m2M * m2M * (chi2 * chi2);
+        double ss1 = (4703.5 / 8.4 + 2935. / 6. * m1M - 120. * m1M * m1M) + (-4108.25 / 6.72 - 108.5 / 1.2 * m1M + 125.5 / 3.6 * m1M * m1M);
+        double ss2 = (4703.5 / 8.4 + 2935. / 6. * m2M - 120. * m2M * m2M) + (-4108.25 / 6.72 - 108.5 / 1.2 * m2M + 125.5 / 3.6 * m2M * m2M);
+        double ss3 = (326.75 / 1.12 + 557.5 / 1.8 * e) * e * chi1 * chi2;
+        ss3 += ss1 * m1M * m1M * (chi1 * chi1);
+        ss3 += ss2 * m2M * m2M * (chi2 * chi2);
+        double gam = (554345. / 1134. + 110. * e / 9.) * SL + (13915. / 84. - 10. * e / 3.) * dSigmaL;
+        v7 += pfaN * ((-8980424995. / 762048. + 6586595. * e / 756. - 305. * e * e / 36.) * SL -
+                      (170978035. / 48384. - 2876425. * e / 672. - 4735. * e * e / 144.) * dSigmaL);
+        v6 += pfaN * (PI * (3760. * SL + 1490. * dSigmaL) / 3. + ss3);
+        v5 += pfaN * (-1. * gam);
+        vl5 += pfaN * (-3. * gam);
+        v4 += pfaN * (-10. * sig);
+        v3 += pfaN * (188. * SL / 3. + 25. * dSigmaL);
+        v6 -= ss3 * v0;
+        r.pn_v0 = v0; r.pn_v2 = v2; r.pn_v3 = v3; r.pn_v4 = v4; r.pn_v5 = v5;
+        r.pn_v6 = v6; r.pn_v7 = v7; r.pn_vl5 = vl5; r.pn_vl6 = vl6;
+    }
+
+    const P6 pp = pow6(PI);
+    r.q_init = r.pn_v5 - PI_4;
+    r.q_23 = r.pn_v7 * pp.two_thirds;
+    r.q_13 = r.pn_v6 * pp.third;
+    r.q_13log = r.pn_vl6 * pp.third;
+    r.q_logv = r.pn_vl5;
+    r.q_m13 = r.pn_v4 / pp.third;
+    r.q_m23 = r.pn_v3 / pp.two_thirds;
+    r.q_m1 = r.pn_v2 / PI;
+    r.q_m53 = r.pn_v0 / pp.five_thirds;
+    r.q_1 = r.sigma1;
+    r.q_43 = r.sigma2 * 3.0 / 4.0;
+    r.q_53 = r.sigma3 * 3.0 / 5.0;
+    r.q_2 = r.sigma4 / 2.0;
+
+    // C^1 joins (PhenomHMWaveform.cu:1303-1339)
+    r.fMRDJoin = 0.5 * r.p_fRD;
+    r.C1Int = 0.0;
+    r.C2Int = 0.0;
+    {
+        double vj = cbrt(PI * PHI_JOIN_INS);
+        double dins = dphi_ins(r, vj, log(vj));
+        r.C2Int = dins - dphi_int(r, PHI_JOIN_INS);
+        P6 pj = pow6(PHI_JOIN_INS);
+        double logvj = log(pj.third * 1.4645918875615231);
+        r.C1Int = phi_ins(r, PHI_JOIN_INS, pj, logvj) - r.p_etaInv * phi_int(r, PHI_JOIN_INS) - r.C2Int * PHI_JOIN_INS;
+        const double fj = r.fMRDJoin;
+        double phi_int_j = r.p_etaInv * phi_int(r, fj) + r.C1Int + r.C2Int * fj;
+        double fj2 = fj * fj;
+        double dphi_int_j = r.C2Int + (r.beta1 + r.beta3 / (fj2 * fj2) + r.beta2 / fj) / r.p_eta;
+        r.C2MRD = dphi_int_j - dphi_mrd(r, fj);
+        r.C1MRD = phi_int_j - r.p_etaInv * phi_mrd(r, fj) - r.C2MRD * fj;
+    }
+
+    if (!with_amp) return;
+
+    // amplitude coefficients (PhenomHMWaveform.cu:446-874)
+    r.a_fRD = fring;
+    r.a_fDM = fdamp;
+    r.a_g1 = fit(F_GAMMA1, eta, chi);
+    r.a_g2 = fit(F_GAMMA2, eta, chi);
+    r.a_g3 = fit(F_GAMMA3, eta, chi);
+    if (r.a_g2 <= 1)
+        r.a_fmax = fabs(r.a_fRD + (r.a_fDM * (-1 + sqrt(1 - r.a_g2 * r.a_g2)) * r.a_g3) / r.a_g2);
+    else
+        r.a_fmax = fabs(r.a_fRD + (r.a_fDM * (-1) * r.a_g3) / r.a_g2);
+    r.a_rho1 = fit(F_RHO1, eta, chi);
+    r.a_rho2 = fit(F_RHO2, eta, chi);
+    r.a_rho3 = fit(F_RHO3, eta, chi);
+
+    const double c12 = chi1 * chi1, c22 = chi2 * chi2, eta2 = eta * eta, eta3 = eta * eta2;
+    const double Pi = PI, Pi2 = pp.two;
+    r.a_amp0 = (sqrt(2.0 / 3.0) * sqrt(eta)) / pow(PI, 1. / 6.);
+    r.a_pf23 = ((-969 + 1804 * eta) * pp.two_thirds) / 672.;
+    r.a_pf1 = ((chi1 * (81 * (1 + Seta) - 44 * eta) + chi2 * (81 - 81 * Seta - 44 * eta)) * Pi) / 48.;
+    const double k43 = (-27312085.0 - 10287648 * c22 - 10287648 * c12 * (1 + Seta) + 10287648 * c22 * Seta +
+                        24 * (-1975055 + 857304 * c12 - 994896 * chi1 * chi2 + 857304 * c22) * eta + 35371056 * eta2);
+    r.a_pf43 = (k43 * pp.four_thirds) / 8.128512e6;
+    const double k53 = (chi2 * (-285197 * (-1 + Seta) + 4 * (-91902 + 1579 * Seta) * eta - 35632 * eta2) +
+                        chi1 * (285197 * (1 + Seta) - 4 * (91902 + 1579 * Seta) * eta - 35632 * eta2) +
+                        42840 * (-1.0 + 4 * eta) * Pi);
+    r.a_pf53 = (pp.five_thirds * k53) / 32256.;
+    const double k2 = (-336 * (-3248849057.0 + 2943675504 * c12 - 3339284256 * chi1 * chi2 + 2943675504 * c22) * eta2 -
+                       324322727232 * eta3 -
+                       7 * (-177520268561 + 107414046432 * c22 + 107414046432 * c12 * (1 + Seta) - 107414046432 * c22 * Seta +
+                            11087290368 * (chi1 + chi2 + chi1 * Seta - chi2 * Seta) * Pi) +
+                       12 * eta * (-545384828789 - 176491177632 * chi1 * chi2 + 202603761360 * c22 +
+                                   77616 * c12 * (2610335 + 995766 * Seta) - 77287373856 * c22 * Seta +
+                                   5841690624 * (chi1 + chi2) * Pi + 21384760320 * Pi2));
+    r.a_pf2 = -(Pi2 * k2) / 6.0085960704e10;
+
+    // intermediate amplitude by collocation (PhenomHMWaveform.cu:656-842)
+    const double f1 = AMP_JOIN_INS, f3 = r.a_fmax;
+    const double f2 = f1 + (f3 - f1) / 2.0;
+    const P6 p1 = pow6(f1);
+    const double v1 = amp_ins(r, f1, p1);
+    // d/dMf of the inspiral ansatz at f1 (PhenomHMWaveform.cu:487-506)
+    const double d1 = ((-969 + 1804 * eta) * pow(Pi, 2.0 / 3.0)) / (1008. * pow(f1, 1.0 / 3.0)) +
+                      ((chi1 * (81 * (1 + Seta) - 44 * eta) + chi2 * (81 - 81 * Seta - 44 * eta)) * Pi) / 48. +
+                      (k43 * pow(f1, 1.0 / 3.0) * pow(Pi, 4.0 / 3.0)) / 6.096384e6 +
+                      (5 * pow(f1, 2.0 / 3.0) * pow(Pi, 5.0 / 3.0) * k53) / 96768. -
+                      (f1 * (Pi * Pi) * k2) / 3.0042980352e10 + (7.0 / 3.0) * pow(f1, 4.0 / 3.0) * r.a_rho1 +
+                      (8.0 / 3.0) * pow(f1, 5.0 / 3.0) * r.a_rho2 + 3 * (f1 * f1) * r.a_rho3;
+    const double v3 = amp_mrd(r, f3);
+    const double d2 = amp_mrd_deriv(r, f3);
+    const double v2 = fit(F_V2, eta, chi);
+    const double f12 = f1 * f1, f13 = f1 * f12, f14 = f1 * f13, f15 = f1 * f14;
+    const double f22 = f2 * f2, f23 = f2 * f22, f24 = f2 * f23;
+    const double f32 = f3 * f3, f33 = f3 * f32, f34 = f3 * f33, f35 = f3 * f34;
+    const double den = ((f1 - f2) * (f1 - f2)) * ((f1 - f3) * (f1 - f3) * (f1 - f3)) * ((f3 - f2) * (f3 - f2));
+    r.a_d0 = -((d2 * f15 * f22 * f3 - 2 * d2 * f14 * f23 * f3 + d2 * f13 * f24 * f3 - d2 * f15 * f2 * f32 + d2 * f14 * f22 * f32 -
+                d1 * f13 * f23 * f32 + d2 * f13 * f23 * f32 + d1 * f12 * f24 * f32 - d2 * f12 * f24 * f32 + d2 * f14 * f2 * f33 +
+                2 * d1 * f13 * f22 * f33 - 2 * d2 * f13 * f22 * f33 - d1 * f12 * f23 * f33 + d2 * f12 * f23 * f33 -
+                d1 * f1 * f24 * f33 - d1 * f13 * f2 * f34 - d1 * f12 * f22 * f34 + 2 * d1 * f1 * f23 * f34 + d1 * f12 * f2 * f35 -
+                d1 * f1 * f22 * f35 + 4 * f12 * f23 * f32 * v1 - 3 * f1 * f24 * f32 * v1 - 8 * f12 * f22 * f33 * v1 +
+                4 * f1 * f23 * f33 * v1 + f24 * f33 * v1 + 4 * f12 * f2 * f34 * v1 + f1 * f22 * f34 * v1 - 2 * f23 * f34 * v1 -
+                2 * f1 * f2 * f35 * v1 + f22 * f35 * v1 - f15 * f32 * v2 + 3 * f14 * f33 * v2 - 3 * f13 * f34 * v2 +
+                f12 * f35 * v2 - f15 * f22 * v3 + 2 * f14 * f23 * v3 - f13 * f24 * v3 + 2 * f15 * f2 * f3 * v3 -
+                f14 * f22 * f3 * v3 - 4 * f13 * f23 * f3 * v3 + 3 * f12 * f24 * f3 * v3 - 4 * f14 * f2 * f32 * v3 +
+                8 * f13 * f22 * f32 * v3 - 4 * f12 * f23 * f32 * v3) /
+               den);
+    r.a_d1 = -((-(d2 * f15 * f22) + 2 * d2 * f14 * f23 - d2 * f13 * f24 - d2 * f14 * f22 * f3 + 2 * d1 * f13 * f23 * f3 +
+                2 * d2 * f13 * f23 * f3 - 2 * d1 * f12 * f24 * f3 - d2 * f12 * f24 * f3 + d2 * f15 * f32 -
+                3 * d1 * f13 * f22 * f32 - d2 * f13 * f22 * f32 + 2 * d1 * f12 * f23 * f32 - 2 * d2 * f12 * f23 * f32 +
+                d1 * f1 * f24 * f32 + 2 * d2 * f1 * f24 * f32 - d2 * f14 * f33 + d1 * f12 * f22 * f33 +
+                3 * d2 * f12 * f22 * f33 - 2 * d1 * f1 * f23 * f33 - 2 * d2 * f1 * f23 * f33 + d1 * f24 * f33 +
+                d1 * f13 * f34 + d1 * f1 * f22 * f34 - 2 * d1 * f23 * f34 - d1 * f12 * f35 + d1 * f22 * f35 -
+                8 * f12 * f23 * f3 * v1 + 6 * f1 * f24 * f3 * v1 + 12 * f12 * f22 * f32 * v1 - 8 * f1 * f23 * f32 * v1 -
+                4 * f12 * f34 * v1 + 2 * f1 * f35 * v1 + 2 * f15 * f3 * v2 - 4 * f14 * f32 * v2 + 4 * f12 * f34 * v2 -
+                2 * f1 * f35 * v2 - 2 * f15 * f3 * v3 + 8 * f12 * f23 * f3 * v3 - 6 * f1 * f24 * f3 * v3 +
+                4 * f14 * f32 * v3 - 12 * f12 * f22 * f32 * v3 + 8 * f1 * f23 * f32 * v3) /
+               den);
+    r.a_d2 = -((d2 * f15 * f2 - d1 * f13 * f23 - 3 * d2 * f13 * f23 + d1 * f12 * f24 + 2 * d2 * f12 * f24 - d2 * f15 * f3 +
+                d2 * f14 * f2 * f3 - d1 * f12 * f23 * f3 + d2 * f12 * f23 * f3 + d1 * f1 * f24 * f3 - d2 * f1 * f24 * f3 -
+                d2 * f14 * f32 + 3 * d1 * f13 * f2 * f32 + d2 * f13 * f2 * f32 - d1 * f1 * f23 * f32 + d2 * f1 * f23 * f32 -
+                2 * d1 * f24 * f32 - d2 * f24 * f32 - 2 * d1 * f13 * f33 + 2 * d2 * f13 * f33 - d1 * f12 * f2 * f33 -
+                3 * d2 * f12 * f2 * f33 + 3 * d1 * f23 * f33 + d2 * f23 * f33 + d1 * f12 * f34 - d1 * f1 * f2 * f34 +
+                d1 * f1 * f35 - d1 * f2 * f35 + 4 * f12 * f23 * v1 - 3 * f1 * f24 * v1 + 4 * f1 * f23 * f3 * v1 -
+                3 * f24 * f3 * v1 - 12 * f12 * f2 * f32 * v1 + 4 * f23 * f32 * v1 + 8 * f12 * f33 * v1 - f1 * f34 * v1 -
+                f35 * v1 - f15 * v2 - f14 * f3 * v2 + 8 * f13 * f32 * v2 - 8 * f12 * f33 * v2 + f1 * f34 * v2 + f35 * v2 +
+                f15 * v3 - 4 * f12 * f23 * v3 + 3 * f1 * f24 * v3 + f14 * f3 * v3 - 4 * f1 * f23 * f3 * v3 +
+                3 * f24 * f3 * v3 - 8 * f13 * f32 * v3 + 12 * f12 * f2 * f32 * v3 - 4 * f23 * f32 * v3) /
+               den);
+    r.a_d3 = -((-2 * d2 * f14 * f2 + d1 * f13 * f22 + 3 * d2 * f13 * f22 - d1 * f1 * f24 - d2 * f1 * f24 + 2 * d2 * f14 * f3 -
+                2 * d1 * f13 * f2 * f3 - 2 * d2 * f13 * f2 * f3 + d1 * f12 * f22 * f3 - d2 * f12 * f22 * f3 + d1 * f24 * f3 +
+                d2 * f24 * f3 + d1 * f13 * f32 - d2 * f13 * f32 - 2 * d1 * f12 * f2 * f32 + 2 * d2 * f12 * f2 * f32 +
+                d1 * f1 * f22 * f32 - d2 * f1 * f22 * f32 + d1 * f12 * f33 - d2 * f12 * f33 + 2 * d1 * f1 * f2 * f33 +
+                2 * d2 * f1 * f2 * f33 - 3 * d1 * f22 * f33 - d2 * f22 * f33 - 2 * d1 * f1 * f34 + 2 * d1 * f2 * f34 -
+                4 * f12 * f22 * v1 + 2 * f24 * v1 + 8 * f12 * f2 * f3 * v1 - 4 * f1 * f22 * f3 * v1 - 4 * f12 * f32 * v1 +
+                8 * f1 * f2 * f32 * v1 - 4 * f22 * f32 * v1 - 4 * f1 * f33 * v1 + 2 * f34 * v1 + 2 * f14 * v2 -
+                4 * f13 * f3 * v2 + 4 * f1 * f33 * v2 - 2 * f34 * v2 - 2 * f14 * v3 + 4 * f12 * f22 * v3 - 2 * f24 * v3 +
+                4 * f13 * f3 * v3 - 8 * f12 * f2 * f3 * v3 + 4 * f1 * f22 * f3 * v3 + 4 * f12 * f32 * v3 -
+                8 * f1 * f2 * f32 * v3 + 4 * f22 * f32 * v3) /
+               den);
+    r.a_d4 = -((d2 * f13 * f2 - d1 * f12 * f22 - 2 * d2 * f12 * f22 + d1 * f1 * f23 + d2 * f1 * f23 - d2 * f13 * f3 +
+                2 * d1 * f12 * f2 * f3 + d2 * f12 * f2 * f3 - d1 * f1 * f22 * f3 + d2 * f1 * f22 * f3 - d1 * f23 * f3 -
+                d2 * f23 * f3 - d1 * f12 * f32 + d2 * f12 * f32 - d1 * f1 * f2 * f32 - 2 * d2 * f1 * f2 * f32 +
+                2 * d1 * f22 * f32 + d2 * f22 * f32 + d1 * f1 * f33 - d1 * f2 * f33 + 3 * f1 * f22 * v1 - 2 * f23 * v1 -
+                6 * f1 * f2 * f3 * v1 + 3 * f22 * f3 * v1 + 3 * f1 * f32 * v1 - f33 * v1 - f13 * v2 + 3 * f12 * f3 * v2 -
+                3 * f1 * f32 * v2 + f33 * v2 + f13 * v3 - 3 * f1 * f22 * v3 + 2 * f23 * v3 - 3 * f12 * f3 * v3 +
+                6 * f1 * f2 * f3 * v3 - 3 * f22 * f3 * v3) /
+               den);
+}
+
+// HM frequency-map parameters (PhenomHMWaveform.cu:2107-2286)
+__device__ inline void hm_map(double Rholm, double Mf_RD_lm, double Mf_RD_22, int mm, double flm, bool for_amp,
+                              double& a, double& b, double& fi, double& fr)
+{
+    double Mf_1_22 = for_amp ? AMP_JOIN_INS : PHI_JOIN_INS;
+    fi = Mf_1_22 / Rholm;
+    fr = Mf_RD_lm;
+    double Ai = 2.0 / mm, Bi = 0.0;
+    double Trd = for_amp ? (fr - Mf_RD_lm + Mf_RD_22) : (Rholm * fr);
+    double Ti = 2.0 * fi / mm;
+    double Am = (Trd - Ti) / (fr - fi);
+    double Bm = Ti - fi * Am;
+    double Ar = 1.0, Br = 0.0;
+    if (for_amp)
+        Br = -Mf_RD_lm + Mf_RD_22;
+    else
+        Ar = Rholm;
+    if (flm > fi) {
+        if (flm > fr) { a = Ar; b = Br; }
+        else { a = Am; b = Bm; }
+    } else {
+        a = Ai; b = Bi;
+    }
+}
+
+// Full per-(binary, mode) setup. Masses in solar masses (m1_SI/MSUN), distance in m.
+// Mf_RD_lm/Mf_DM_lm: this mode's QNM frequencies; Mf_RD_D/Mf_DM_D: the PhenomD
+// (2,2) ringdown frequencies from the QNM table (IMRPhenomHMCore, PhenomHMWaveform.cu:2671-2844).
+// Returns false for an unsupported mode.
+__device__ inline bool phm_setup(PhmRec& r, double m1_SI, double m2_SI, double chi1_in, double chi2_in,
+                                 double distance, int ell, int mm, double Mf_RD_lm, double Mf_DM_lm,
+                                 double Mf_RD_D, double Mf_DM_D, bool phenomd)
+{
+    double m1 = m1_SI / MSUN_SI, m2 = m2_SI / MSUN_SI, chi1 = chi1_in, chi2 = chi2_in;
+    enforce_primary(m1, m2, chi1, chi2);
+    double finmass, finspin;
+    final_state(m1, m2, chi1, chi2, finmass, finspin);
+    double Mf_RD_22, Mf_DM_22;
+    bool ok = true;
+    if (!phenomd)
+        ok = hm_ringdown(2, 2, finmass, finspin, Mf_RD_22, Mf_DM_22);
+    else {
+        Mf_RD_22 = Mf_RD_D;
+        Mf_DM_22 = Mf_DM_D;
+    }
+    r.ell = phenomd ? 2 : ell;
+    r.mm = phenomd ? 2 : mm;
+    r.phenomd = phenomd ? 1 : 0;
+    if (!phenomd && !((ell == 2 && mm == 2) || (ell == 2 && mm == 1) || (ell == 3 && mm == 3) ||
+                      (ell == 3 && mm == 2) || (ell == 4 && mm == 4) || (ell == 4 && mm == 3)))
+        ok = false;
+
+    // (2,2) PhenomD: f_ref -> amplitude peak, phi0 and t0 there (PhenomHMWaveform.cu:2752-2784)
+    setup_phenomd(r, m1, m2, chi1, chi2, 1.0, 1.0, Mf_RD_D, Mf_DM_D, true);
+    r.Mf_ref = r.a_fmax;
+    double ph22, dph22;
+    phase_imr(r, r.Mf_ref, ph22, &dph22);
+    r.phi0 = 0.5 * (ph22 + 0.0);
+    r.t0 = dph22;
+    const double Mtot = (m1_SI + m2_SI) / MSUN_SI;
+    r.amp_scale = Mtot * MRSUN_SI * Mtot * MTSUN_SI / distance;
+    r.M_sec = (m1 + m2) * MTSUN_SI;
+    const double cs[7] = {0.0, PI_2, 0.0, -PI_2, PI, PI_2, 0.0};
+    r.cshift = cs[r.mm < 0 ? 0 : (r.mm > 6 ? 0 : r.mm)];
+
+    // PN amplitude-ratio inputs (IMRPhenomHMOnePointFiveSpinPN with the enforced masses/spins)
+    {
+        double Min = m1 + m2;
+        double M1 = m1 / Min, M2 = m2 / Min;
+        double M = M1 + M2;
+        r.pn_M = M;
+        r.pn_eta = M1 * M2 / (M * M);
+        r.pn_delta = sqrt(1.0 - 4 * r.pn_eta);
+        r.pn_Xs = 0.5 * (chi1 + chi2);
+        r.pn_Xa = 0.5 * (chi1 - chi2);
+        r.pn_pref = M * M * PI * sqrt(r.pn_eta * 2.0 / 3);
+    }
+
+    if (phenomd) {
+        r.ai = 1.0; r.bi = 0.0; r.am = 1.0; r.bm = 0.0; r.ar = 1.0; r.br = 0.0;
+        r.fi = 1e300; r.fr = 1e300; r.PhDBconst = 0.0; r.PhDCconst = 0.0; r.PhDBAterm = 0.0; r.phaseC = 0.0;
+        r.amp_fi = 1e300; r.amp_fr = 1e300; r.amp_Ai = 1.0; r.amp_Am = 1.0; r.amp_Bm = 0.0; r.amp_Br = 0.0;
+        return ok;
+    }
+
+    // this mode's ratios and the PhenomD phase re-joined with them
+    const double Rholm = Mf_RD_22 / Mf_RD_lm;
+    const double Taulm = Mf_DM_lm / Mf_DM_22;
+    setup_phenomd(r, m1, m2, chi1, chi2, Rholm, Taulm, Mf_RD_D, Mf_DM_D, false);
+
+    // phase map + constants (IMRPhenomHMPhasePreComp, PhenomHMWaveform.cu:2317-2369)
+    const double shift = 0.0001;
+    double fi, fr;
+    hm_map(Rholm, Mf_RD_lm, Mf_RD_22, mm, shift, false, r.ai, r.bi, fi, fr);
+    hm_map(Rholm, Mf_RD_lm, Mf_RD_22, mm, fi + shift, false, r.am, r.bm, fi, fr);
+    hm_map(Rholm, Mf_RD_lm, Mf_RD_22, mm, fr + shift, false, r.ar, r.br, fi, fr);
+    r.fi = fi;
+    r.fr = fr;
+    r.PhDBconst = phase_only(r, r.am * fi + r.bm) / r.am;
+    r.PhDCconst = phase_only(r, r.ar * fr + r.br) / r.ar;
+    r.PhDBAterm = phase_only(r, r.ai * fi + r.bi) / r.ai;
+    r.phaseC = phase_only(r, r.am * r.fr + r.bm) / r.am - r.PhDBconst + r.PhDBAterm;
+
+    // amplitude map (region boundaries and linear pieces)
+    {
+        double a, b, afi, afr;
+        hm_map(Rholm, Mf_RD_lm, Mf_RD_22, mm, 0.0, true, a, b, afi, afr);
+        r.amp_fi = afi;
+        r.amp_fr = afr;
+        r.amp_Ai = 2.0 / mm;
+        double Trd = afr - Mf_RD_lm + Mf_RD_22;
+        double Ti = 2.0 * afi / mm;
+        r.amp_Am = (Trd - Ti) / (afr - afi);
+        r.amp_Bm = Ti - afi * r.amp_Am;
+        r.amp_Br = -Mf_RD_lm + Mf_RD_22;
+    }
+    return ok;
+}
+
+// Amplitude, phase and t(f) of one (binary, mode) at frequency f in Hz
+// (calculate_modes / calculate_modes_phenomd, PhenomHMWaveform.cu:2461-2662).
+__device__ inline void phm_eval(const PhmRec& r, double f, double& amp, double& phase, double& tf)
+{
+    const double Mf_wf = f * r.M_sec;
+    if (r.phenomd) {
+        P6 p = pow6(Mf_wf);
+        amp = amp_imr(r, Mf_wf, p) * r.amp_scale;
+        double ph, dph;
+        phase_imr(r, Mf_wf, ph, &dph);
+        double phi = r.cshift + ph;
+        phase = -r.t0 * (Mf_wf - r.Mf_ref) + (phi - (2 * r.phi0));
+        double dphidf = r.M_sec * dph / (2 * PI);
+        tf = dphidf - (r.t0 / (2. * PI) * r.M_sec);
+        return;
+    }
+
+    // amplitude
+    double a, b;
+    if (Mf_wf > r.amp_fi) {
+        if (Mf_wf > r.amp_fr) { a = 1.0; b = r.amp_Br; }
+        else { a = r.amp_Am; b = r.amp_Bm; }
+    } else {
+        a = r.amp_Ai; b = 0.0;
+    }
+    const double Mf_amp = a * Mf_wf + b;
+    P6 p = pow6(Mf_amp);
+    double A = amp_imr(r, Mf_amp, p);
+    double beta1 = pn_amp(r, Mf_wf, r.ell, r.mm, false);
+    double beta = 0.0, hm1 = 1.0, hm2 = 1.0;
+    if (beta1 != 0.) {
+        double beta2 = pn_amp(r, 2.0 * Mf_wf / r.mm, r.ell, r.mm, false);
+        beta = beta1 / beta2;
+        hm1 = pn_amp(r, Mf_amp, r.ell, r.mm, false);
+        hm2 = pn_amp(r, Mf_amp, 2, 2, true);
+    }
+    A *= beta * hm1 / hm2;
+    amp = A * r.amp_scale;
+
+    // phase and t(f)
+    double phi = r.cshift;
+    double dph = 0.0;
+    if (!(Mf_wf > r.fi)) {
+        double Mf = r.ai * Mf_wf + r.bi;
+        double ph;
+        phase_imr(r, Mf, ph, &dph);
+        phi += ph / r.ai;
+    } else if (!(Mf_wf > r.fr)) {
+        double Mf = r.am * Mf_wf + r.bm;
+        double ph;
+        phase_imr(r, Mf, ph, &dph);
+        phi += ph / r.am - r.PhDBconst + r.PhDBAterm;
+    } else {
+        double Mf = r.ar * Mf_wf + r.br;
+        double ph;
+        phase_imr(r, Mf, ph, &dph);
+        phi += ph / r.ar - r.PhDCconst + r.phaseC;
+    }
+    phase = -r.t0 * (Mf_wf - r.Mf_ref) + (phi - (r.mm * r.phi0));
+    double tf_i = dph / (2. * PI);
+    tf = (r.M_sec * tf_i) - (r.t0 / (2. * PI) * r.M_sec);
+}
+
+}  // namespace phm
+}  // namespace bbhx

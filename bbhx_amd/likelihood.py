@@ -1,0 +1,269 @@
+"""Direct and heterodyned log-likelihoods on the GPU.
+
+Host-side mirror of the reference's ``likelihood.py`` (``Likelihood`` :26-225,
+``HeterodynedLikelihood`` :228-648): same constructor and ``get_ll``
+signatures and semantics. ``HeterodynedLikelihood.get_ll`` runs the fused
+single-pass kernel by default (``fused=True``); ``fused=False`` runs the
+kernel-by-kernel path of the reference (waveform -> response -> direct sum ->
+r = h/h0 -> hdyn).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from .utils.arrays import as_np1d, to_dev, to_host
+from .utils.parallelbase import BBHxParallelModule
+from .utils.sensitivity import AET1SensitivityMatrix
+
+
+class Likelihood(BBHxParallelModule):
+    """log L = -1/2 (<d|d> + <h|h> - 2 <d|h>) summed over each template's band
+    (likelihood.py:26-225). ``data_channels`` and ``psd`` are (3, N)."""
+
+    def __init__(self, template_gen, data_freqs, data_channels, psd, force_backend=None):
+        super().__init__(force_backend=force_backend)
+        torch = self.xp
+        data_freqs_cpu = np.asarray(to_host(data_freqs), dtype=np.float64)
+        self.data_freqs = to_dev(data_freqs_cpu, self.device)
+        psd = np.asarray(to_host(psd), dtype=np.float64)
+        self.psd = psd
+        self.delta_f = np.zeros_like(self.psd)
+        self.delta_f[:, 1:] = np.diff(data_freqs_cpu)
+        self.delta_f[:, 0] = self.delta_f[:, 1]
+        self.noise_factors = to_dev(np.sqrt(1.0 / psd * self.delta_f).flatten(), self.device)
+        dc = to_dev(data_channels, self.device, dtype=torch.complex128).flatten()
+        self.data_channels = (self.noise_factors * dc).contiguous()
+        self.waveform_gen = template_gen
+        self.data_stream_length = len(data_freqs_cpu)
+        self.d_d = (4 * torch.sum(self.data_channels.conj() * self.data_channels).real).item()
+
+    @property
+    def like_gen(self):
+        return self.backend.direct_like_wrap
+
+    def get_ll(self, params, return_extracted_snr=False, phase_marginalize=False, **waveform_kwargs):
+        """params: (12,) or (12, num_bin_all) rows m1, m2, chi1z, chi2z, distance,
+        phi_ref, f_ref, inc, lam, beta, psi, t_ref (likelihood.py:140-225)."""
+        torch = self.xp
+        self.phase_marginalize = phase_marginalize
+        self.return_extracted_snr = return_extracted_snr
+        waveform_kwargs["freqs"] = self.data_freqs
+        waveform_kwargs["fill"] = False
+        waveform_kwargs["direct"] = False
+        templateChannels, inds_start, ind_lengths = self.waveform_gen(*params, **waveform_kwargs)
+        templateChannels = [tc.flatten() for tc in templateChannels]
+        ptrs = np.asarray([tc.data_ptr() for tc in templateChannels], dtype=np.int64)
+        B = self.waveform_gen.num_bin_all
+        self.d_h = torch.zeros(B, dtype=torch.complex128, device=self.device)
+        self.h_h = torch.zeros(B, dtype=torch.complex128, device=self.device)
+        self.like_gen(self.d_h, self.h_h, self.data_channels, self.noise_factors, ptrs, inds_start, ind_lengths,
+                      self.data_stream_length, B)
+        d_h_temp = self.d_h if not self.phase_marginalize else torch.abs(self.d_h)
+        out = (-1 / 2 * (self.d_d + self.h_h - 2 * d_h_temp).real).cpu().numpy()
+        if self.return_extracted_snr:
+            snr = (d_h_temp.real / torch.sqrt(self.h_h.real)).cpu().numpy()
+            return np.array([out, snr]).T
+        return out
+
+
+class HeterodynedLikelihood(BBHxParallelModule):
+    """Heterodyned (relative-binning) log-likelihood (likelihood.py:228-648;
+    arXiv:1806.08792).
+
+    Extra keyword (no reference counterpart):
+        hdyn_last_bin (str): ``"gpu"`` (default) uses all K-1 sparse segments like
+            the reference's GPU kernel; ``"cpu"`` drops the last one like the
+            reference's CPU loop (Likelihood.cu:150).
+    """
+
+    def __init__(self, template_gen, data_freqs, data_channels, reference_template_params, length_f_het,
+                 template_gen_kwargs=None, reference_gen_kwargs=None, sens_mat=None, force_backend=None,
+                 hdyn_last_bin="gpu"):
+        template_gen_kwargs = {} if template_gen_kwargs is None else template_gen_kwargs
+        reference_gen_kwargs = {} if reference_gen_kwargs is None else reference_gen_kwargs
+        super().__init__(force_backend=force_backend)
+        self.template_gen = template_gen
+        self.f_dense = to_dev(data_freqs, self.device)
+        self.d = to_dev(data_channels, self.device, dtype=self.xp.complex128)
+        self.length_f_het = length_f_het
+        if hdyn_last_bin not in ("gpu", "cpu"):
+            raise ValueError("hdyn_last_bin must be 'gpu' or 'cpu'")
+        self.include_last_segment = hdyn_last_bin == "gpu"
+        self.sens_mat = sens_mat
+        self.init_heterodyne_info(reference_template_params, template_gen_kwargs=template_gen_kwargs,
+                                  reference_gen_kwargs=reference_gen_kwargs)
+
+    @property
+    def sens_mat(self):
+        return self._sens_mat
+
+    @sens_mat.setter
+    def sens_mat(self, sens_mat):
+        if sens_mat is None:
+            sens_mat = AET1SensitivityMatrix(np.logspace(-5, -1, 1000))
+        self._sens_mat = sens_mat
+
+    @property
+    def like_gen(self):
+        return self.backend.hdyn_wrap
+
+    def init_heterodyne_info(self, reference_template_params, template_gen_kwargs=None, reference_gen_kwargs=None):
+        """Reference template and heterodyne constants (likelihood.py:380-548)."""
+        torch = self.xp
+        template_gen_kwargs = {} if template_gen_kwargs is None else template_gen_kwargs
+        reference_gen_kwargs = {} if reference_gen_kwargs is None else reference_gen_kwargs
+        template_gen_kwargs["squeeze"] = True
+        template_gen_kwargs["compress"] = True
+        template_gen_kwargs["direct"] = True
+
+        minF = float(self.f_dense.min()) * 0.999999999999
+        maxF = float(self.f_dense.max()) * 1.000000000001
+        if minF == 0.0:
+            minF = 1e-6
+
+        reference_gen_kwargs["squeeze"] = True
+        reference_gen_kwargs["compress"] = True
+        reference_gen_kwargs["fill"] = True
+        if not ("direct" in reference_gen_kwargs and reference_gen_kwargs["direct"] is True):
+            reference_gen_kwargs["direct"] = False
+            if "length" not in reference_gen_kwargs:
+                reference_gen_kwargs["length"] = 8096
+
+        freqs = np.logspace(np.log10(minF), np.log10(maxF), self.length_f_het)
+        self.reference_template_params = np.asarray(reference_template_params, dtype=np.float64)
+        h0 = self.template_gen(*reference_template_params, freqs=self.f_dense, **reference_gen_kwargs)[0]
+        h0_temp = self.template_gen(*reference_template_params, freqs=freqs, **template_gen_kwargs)[0]
+        keep = ~(torch.abs(h0_temp) == 0.0).cpu().numpy()
+        freqs_keep = freqs[keep]
+        freqs = np.logspace(np.log10(freqs_keep[0]), np.log10(freqs_keep[-1]), self.length_f_het)
+        self.h0_sparse = self.template_gen(*reference_template_params, freqs=freqs, **template_gen_kwargs)[None, :, :]
+
+        f_dense_h = self.f_dense.cpu().numpy()
+        inds = (f_dense_h >= freqs[0]) & (f_dense_h <= freqs[-1])
+        inds_d = torch.as_tensor(inds, device=self.device)
+        self.f_dense = self.f_dense[inds_d].contiguous()
+        self.d = self.d[:, inds_d].contiguous()
+        h0 = h0[:, inds_d]
+        f_n_host = f_dense_h[inds]
+
+        freqs_d = to_dev(freqs, self.device)
+        bins = torch.searchsorted(freqs_d, self.f_dense, right=True) - 1
+        f_m = (freqs_d[1:] + freqs_d[:-1]) / 2
+        df = self.f_dense[1] - self.f_dense[0]
+
+        self.sens_mat.update_frequency_arr(f_n_host)
+        S_n = to_dev(np.asarray([self.sens_mat[0], self.sens_mat[1], self.sens_mat[2]]), self.device)
+
+        A0_flat = 4 * (h0.conj() * self.d) / S_n * df
+        A1_flat = 4 * (h0.conj() * self.d) / S_n * df * (self.f_dense - f_m[bins.clamp(max=f_m.numel() - 1)])
+        B0_flat = 4 * (h0.conj() * h0) / S_n * df
+        B1_flat = 4 * (h0.conj() * h0) / S_n * df * (self.f_dense - f_m[bins.clamp(max=f_m.numel() - 1)])
+
+        # Segmented sums into slot bin + 1. The reference loops over unique(bins[:-1]) and
+        # drops the last dense point of the band (likelihood.py:512-522): same as summing
+        # every point but the last.
+        K = self.length_f_het
+        keep_n = self.f_dense.numel() - 1
+        slot = (bins[:keep_n] + 1).to(torch.int64)
+        consts = []
+        for flat in (A0_flat, A1_flat, B0_flat, B1_flat):
+            acc = torch.zeros((3, K + 1), dtype=torch.complex128, device=self.device)
+            acc.index_add_(1, slot, flat[:, :keep_n])
+            consts.append(acc[:, :K])
+        self.data_constants = torch.cat([c.flatten() for c in consts]).contiguous()
+
+        self.reference_d_d = float(torch.sum(4 * (self.d.conj() * self.d) / S_n * df).real)
+        self.reference_h_h = float(torch.sum(B0_flat).real)
+        self.reference_d_h = float(torch.sum(A0_flat).real)
+        self.reference_ll = -1 / 2 * (self.reference_d_d + self.reference_h_h - 2 * self.reference_d_h)
+
+        self.freqs = freqs_d
+        self.f_m = f_m
+        template_gen_kwargs["squeeze"] = False
+        self.template_gen_kwargs = template_gen_kwargs
+
+    def _merge_kwargs(self, waveform_kwargs):
+        all_keys = list(set(list(waveform_kwargs.keys()) + list(self.template_gen_kwargs.keys())))
+        for key in all_keys:
+            if key in ["direct", "compress", "squeeze"]:
+                waveform_kwargs[key] = self.template_gen_kwargs[key]
+            elif key in self.template_gen_kwargs:
+                waveform_kwargs[key] = waveform_kwargs.get(key, self.template_gen_kwargs[key])
+        waveform_kwargs["freqs"] = self.freqs
+        return waveform_kwargs
+
+    def get_ll(self, params, return_extracted_snr=False, phase_marginalize=False, fused=True, **waveform_kwargs):
+        """log-likelihood of each parameter column (likelihood.py:550-648).
+        ``params``: (12,) or (12, num_bin_all); returns numpy (num_bin_all,)."""
+        torch = self.xp
+        self.phase_marginalize = phase_marginalize
+        self.return_extracted_snr = return_extracted_snr
+        waveform_kwargs = self._merge_kwargs(dict(waveform_kwargs))
+        if fused:
+            self.hdyn_d_h, self.hdyn_h_h = self._fused_inner_products(params, waveform_kwargs)
+        else:
+            self.h_sparse = self.template_gen(*params, **waveform_kwargs)
+            r = self.h_sparse / self.h0_sparse
+            B = self.template_gen.num_bin_all
+            self.hdyn_d_h = torch.zeros(B, dtype=torch.complex128, device=self.device)
+            self.hdyn_h_h = torch.zeros(B, dtype=torch.complex128, device=self.device)
+            residuals_in = r.permute(2, 1, 0).contiguous().flatten()
+            self.like_gen(self.hdyn_d_h, self.hdyn_h_h, residuals_in, self.data_constants, self.freqs, B,
+                          int(self.freqs.numel()), 3, self.include_last_segment)
+        d_h_temp = self.hdyn_d_h if not self.phase_marginalize else torch.abs(self.hdyn_d_h)
+        out = (-1 / 2.0 * (self.reference_d_d + self.hdyn_h_h - 2 * d_h_temp).real).cpu().numpy()
+        if self.return_extracted_snr:
+            snr = (d_h_temp.real / torch.sqrt(self.hdyn_h_h.real)).cpu().numpy()
+            return np.array([out, snr]).T
+        return out
+
+    # ---------------------------------------------------------------- fused path
+
+    def fused_inputs(self, params, waveform_kwargs=None):
+        """Device inputs of the fused kernel: (11, B) parameters (m1 >= m2
+        enforced as phenomhm.py:175-199 does), t_start, t_end."""
+        waveform_kwargs = {} if waveform_kwargs is None else waveform_kwargs
+        p = np.atleast_2d(np.asarray(to_host(params), dtype=np.float64))
+        if p.shape[0] != 12:
+            p = p.T
+        if p.shape[0] != 12:
+            raise ValueError("params must have 12 rows (m1, m2, chi1z, chi2z, distance, phi_ref, f_ref, inc, lam, "
+                             "beta, psi, t_ref)")
+        m1, m2, c1, c2 = p[0].copy(), p[1].copy(), p[2].copy(), p[3].copy()
+        self.template_gen.amp_phase_gen._sanity_check_params(m1, m2, c1, c2)
+        rows = np.stack([m1, m2, c1, c2, p[4], p[5], p[7], p[8], p[9], p[10], p[11]])
+        t_start, t_end = self.template_gen.time_window(p[8], p[9], waveform_kwargs.get("t_obs_start", 0.0),
+                                                       waveform_kwargs.get("t_obs_end", 1.0))
+        return (to_dev(rows, self.device), to_dev(t_start, self.device), to_dev(t_end, self.device))
+
+    def get_ll_device(self, rows, t_start, t_end, precision=0, modes=None):
+        """Fused log-likelihood of device-resident inputs (from ``fused_inputs``);
+        returns a device tensor. No host transfer."""
+        d_h, h_h = self._fused_core(rows, t_start, t_end, modes, precision)
+        self.hdyn_d_h, self.hdyn_h_h = d_h, h_h
+        return -1 / 2.0 * (self.reference_d_d + h_h - 2 * d_h).real
+
+    def _fused_core(self, rows, t_start, t_end, modes, precision):
+        torch = self.xp
+        gen = self.template_gen
+        if modes is None:
+            modes = self.template_gen_kwargs.get("modes") or gen.amp_phase_gen.allowable_modes
+        ells = np.asarray([l for l, m in modes], dtype=np.int32)
+        mms = np.asarray([m for l, m in modes], dtype=np.int32)
+        B = rows.shape[1]
+        d_h = torch.empty(B, dtype=torch.complex128, device=self.device)
+        h_h = torch.empty(B, dtype=torch.complex128, device=self.device)
+        resp = gen.response_gen
+        if not hasattr(self, "_h0_flat"):
+            self._h0_flat = self.h0_sparse.reshape(3, -1).contiguous()
+        self.backend.hdyn_fused(d_h, h_h, rows, t_start, t_end, ells, mms, self.freqs, self._h0_flat,
+                                self.data_constants, gen.amp_phase_gen.qnm_packed, gen.amp_phase_gen.dspin,
+                                gen.amp_phase_gen.initial_t_val, resp.cpp_response, resp.TDItag_int, resp.rescaled,
+                                resp.tdi2, self.include_last_segment, precision)
+        gen.num_bin_all = B
+        return d_h, h_h
+
+    def _fused_inner_products(self, params, waveform_kwargs, precision=0):
+        rows, t_start, t_end = self.fused_inputs(params, waveform_kwargs)
+        return self._fused_core(rows, t_start, t_end, waveform_kwargs.get("modes"), precision)

@@ -1,0 +1,198 @@
+"""Heterodyned log-likelihood throughput on one MI355X.
+
+Workload (the heterodyned-likelihood configuration): one reference MBHB (the
+first draw of the seeded parameter distribution), 4096 templates with every
+sampled parameter offset by N(0, 1e-3 * prior range), PhenomHM modes
+(2,2),(2,1),(3,3),(3,2),(4,4),(4,3), TDI-2 A/E/T response, K = 128
+log-spaced sparse nodes, dense data of 2^20 bins at df = 1/yr (injection +
+Gaussian noise from the analytic A/E/T PSD), complex128 data, FP64 PSD.
+
+A step = one heterodyned log-likelihood evaluation of all templates with the
+parameters already resident in HBM (fused kernel: waveform + response + mode
+sum + r = h/h0 + heterodyned sums). Prints ONE JSON line.
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--batch 4096] [--nodes 128]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def build_problem(batch, nodes, dense_bits, seed=1000):
+    import torch
+
+    from bbhx_amd.likelihood import HeterodynedLikelihood
+    from bbhx_amd.utils.constants import YRSID_SI
+    from bbhx_amd.utils.orbits import EqualArmlengthOrbits
+    from bbhx_amd.utils.sensitivity import AET1SensitivityMatrix
+    from bbhx_amd.waveformbuild import BBHWaveformFD
+    from tests.params import MODES, draw, perturb
+
+    ref = draw(16, seed)[:, 0]
+    templates = perturb(ref, batch, seed + 1, scale=1e-3)
+    f = np.arange(1, 2**dense_bits + 1) / YRSID_SI
+    orbits = EqualArmlengthOrbits(dt=1e4, t_max=3.0 * YRSID_SI)
+    wave = BBHWaveformFD(response_kwargs=dict(orbits=orbits, tdi2=True))
+    sens = AET1SensitivityMatrix(f)
+    psd = np.array([sens[0], sens[1], sens[2]])
+    inj = wave(*ref, freqs=f, length=1024, modes=MODES, direct=False, fill=True)[0].cpu().numpy()
+    rng = np.random.default_rng(seed + 1)
+    df = f[1] - f[0]
+    noise = np.sqrt(psd / (4 * df)) * (rng.standard_normal(psd.shape) + 1j * rng.standard_normal(psd.shape))
+    data = inj + noise
+    het = HeterodynedLikelihood(wave, f, data, ref, nodes, sens_mat=sens, template_gen_kwargs=dict(modes=MODES),
+                                reference_gen_kwargs=dict(length=8096, modes=MODES))
+    torch.cuda.synchronize()
+    return dict(ref=ref, templates=templates, f=f, data=data, psd=psd, het=het, wave=wave, orbits=orbits)
+
+
+def cpu_baseline(prob, sample, nodes):
+    """The FP64 CPU oracle (OpenMP over templates on the host cores) on a bounded sample."""
+    from bbhx_amd.utils.sensitivity import AET1SensitivityMatrix
+    from oracle.pipeline import OracleHeterodyne, OracleWaveform
+    from tests.params import MODES
+
+    owave = OracleWaveform(prob["orbits"].as_host_dict(), tdi2=True)
+    psd_fn = lambda x: [AET1SensitivityMatrix(x)[i] for i in range(3)]  # noqa: E731
+    ohet = OracleHeterodyne(owave, prob["f"], prob["data"], prob["ref"], nodes, psd_fn, reference_length=8096,
+                            modes=MODES)
+    p = prob["templates"][:, :sample]
+    ohet.get_ll(p[:, :8])  # warm-up
+    t0 = time.perf_counter()
+    reps = 0
+    ll = None
+    while True:
+        ll = ohet.get_ll(p)[0]
+        reps += 1
+        if time.perf_counter() - t0 > 10.0 or reps >= 5:
+            break
+    dt = (time.perf_counter() - t0) / reps
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return dict(value=sample / dt, ll=ll, cores=cores, seconds_per_batch=dt)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--batch", type=int, default=4096)
+    ap.add_argument("--nodes", type=int, default=128)
+    ap.add_argument("--dense-bits", type=int, default=20)
+    ap.add_argument("--cpu-sample", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the template batch shards by template "
+              "with no inter-GPU exchange, see DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from bbhx_amd.cutils import timing_enable, timing_read
+
+    log("building problem")
+    prob = build_problem(args.batch, args.nodes, args.dense_bits)
+    het = prob["het"]
+    rows, t_start, t_end = het.fused_inputs(prob["templates"], {})
+    B = rows.shape[1]
+    torch.cuda.synchronize()
+
+    def step():
+        return het.get_ll_device(rows, t_start, t_end)
+
+    log("warmup")
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+
+    timing_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ll_dev = step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_setup, ms_eval, ncalls = timing_read()
+    timing_enable(False)
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    value = B / ((t1 - t0) / args.steps)
+    ll_fused = ll_dev.cpu().numpy()
+    log(f"fused FP64: {ms_per_step:.3f} ms/step, {value:.4g} evals/s; setup {ms_setup / max(ncalls, 1):.3f} ms, "
+        f"eval {ms_eval / max(ncalls, 1):.3f} ms per call")
+
+    # the kernel-by-kernel FP64 path (the reference's structure) on the same templates
+    t0 = time.perf_counter()
+    n_plain = 3
+    for _ in range(n_plain):
+        ll_plain = het.get_ll(prob["templates"], fused=False)
+    torch.cuda.synchronize()
+    plain_rate = B / ((time.perf_counter() - t0) / n_plain)
+    log(f"plain FP64 path: {plain_rate:.4g} evals/s; max |ll_fused - ll_plain| = "
+        f"{np.max(np.abs(ll_fused - ll_plain)):.3e}")
+
+    # algorithmic FP64 flops per launch of the fused evaluation kernel
+    flop_model = json.loads((ROOT / "bbhx_amd" / "flop_model.json").read_text())
+    M = 6
+    flops = flop_model["fused_eval_flops_per_mode_node"] * B * M * args.nodes
+    eval_s = (ms_eval / max(ncalls, 1)) * 1e-3
+    achieved = flops / eval_s / 1e12 if eval_s > 0 else None
+    roofline = {"bound": "mfma", "compute": "FP64 VALU (no dense contraction; no MFMA)", "achieved": achieved,
+                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": (achieved / FP64_PEAK_TFLOPS) if achieved else None, "traffic": None,
+                "kernel": "k_fused_eval", "kernel_ms": ms_eval / max(ncalls, 1),
+                "flops_per_launch": flops}
+
+    cpu = None
+    dll = None
+    if not args.no_cpu:
+        log("timing the CPU oracle")
+        c = cpu_baseline(prob, min(args.cpu_sample, B), args.nodes)
+        dll = float(np.max(np.abs(ll_fused[:len(c["ll"])] - c["ll"])))
+        cpu = {"value": c["value"], "unit": "logL evals/s", "cores": c["cores"], "kind": "port",
+               "sample": f"FP64 CPU oracle (OpenMP over templates), first {len(c['ll'])} templates of the batch, "
+                         f"{c['seconds_per_batch']:.2f} s per pass"}
+        log(f"cpu oracle: {c['value']:.4g} evals/s on {c['cores']} threads; max |dlogL| vs GPU = {dll:.3e}")
+
+    out = {
+        "metric": "heterodyned logL evals/sec + grad evals/sec at 1 GPU, |dlogL|<1e-3 vs FP64",
+        "value": value,
+        "unit": "logL evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded injection + Gaussian noise, analytic A/E/T PSD)",
+        "config": {"workload": "heterodyned logL, fused FP64", "templates": B, "sparse_nodes": args.nodes,
+                   "dense_bins": 2**args.dense_bits, "modes": 6, "tdi": "AET tdi2",
+                   "plain_fp64_path_evals_per_s": plain_rate, "max_abs_dlogL_vs_cpu_oracle": dll,
+                   "grad_evals_per_s": None},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

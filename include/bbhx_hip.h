@@ -1,0 +1,187 @@
+/*
+ * C ABI of the gfx950 (MI355X) backend for the BBHx template-to-likelihood path.
+ *
+ * Every entry point:
+ *   - takes device pointers (unless stated "host") and plain sizes;
+ *   - is asynchronous on 'stream' (a hipStream_t passed as void*; NULL = the
+ *     default stream) and returns as soon as the work is queued, except where a
+ *     host-side table must be uploaded first (noted below);
+ *   - returns 0 on success or a nonzero status: 1 invalid arguments,
+ *     2 launch/runtime error, 3 workspace allocation failure, 4 unsupported mode.
+ *     It never exits the process (the reference calls exit() on errors,
+ *     cutils/global.h:14-22).
+ *
+ * Array layouts are the reference's flat layouts:
+ *   waveform / response buffer   ((p*B + b)*M + m)*L + i,  p = amp, phase, tf,
+ *                                T1re, T1im, T2re, T2im, T3re, T3im
+ *   frequencies                  (b*M + m)*L + i
+ *   ringdown frequencies         b*(M+1) + m, PhenomD value in column M
+ *   complex arrays               interleaved (re, im) doubles
+ *
+ * The library keeps per-device grow-only scratch buffers (coefficient records,
+ * band tables, partial sums). Calls that run concurrently on different streams
+ * of one device share them and must not overlap.
+ *
+ * Reference file:line citations are relative to src/bbhx/ of mikekatz04/BBHx.
+ */
+#ifndef BBHX_HIP_H
+#define BBHX_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- waveform: PhenomHM / PhenomD ------------------------------------------ */
+
+/* Replaces waveform_amp_phase_wrap (cutils/phenomhm.pyx:61) ->
+ * waveform_amp_phase (cutils/PhenomHMWaveform.cu:2998).
+ * Fills the amp | phase | tf slabs of waveformOut (B, M, L each). ells/mms (M,)
+ * int32; masses in kg; distance in m; f_ref is ignored (the reference replaces
+ * it by the (2,2) amplitude peak, PhenomHMWaveform.cu:2767). Mf_RD/DM_lm_all:
+ * (B, M+1) from the two ringdown entry points below, or (B, 1) when
+ * run_phenomd (then numModes must be 1). */
+int bbhx_hip_waveform_amp_phase(double* waveformOut, const int* ells, const int* mms, const double* freqs,
+                                const double* m1_SI, const double* m2_SI, const double* chi1z, const double* chi2z,
+                                const double* distance, const double* f_ref, int numModes, int length,
+                                int numBinAll, const double* Mf_RD_lm_all, const double* Mf_DM_lm_all,
+                                int run_phenomd, void* stream);
+
+/* Replaces get_phenomhm_ringdown_frequencies (cutils/phenomhm.pyx:117 ->
+ * PhenomHMWaveform.cu:1817-1928). Masses in solar masses. Output (B, M). */
+int bbhx_hip_get_phenomhm_ringdown_frequencies(double* fringdown, double* fdamp, const double* m1, const double* m2,
+                                               const double* chi1z, const double* chi2z, const int* ells,
+                                               const int* mms, int numModes, int numBinAll, void* stream);
+
+/* Replaces get_phenomd_ringdown_frequencies (cutils/phenomhm.pyx:155 ->
+ * PhenomHMWaveform.cu:1933-2062). Spline coefficient arrays of the QNM table
+ * have table_len entries each (1001 for the reference's table; the index is
+ * clamped into the table, the reference does not check it). Output (B,). */
+int bbhx_hip_get_phenomd_ringdown_frequencies(double* fringdown, double* fdamp, const double* m1, const double* m2,
+                                              const double* chi1z, const double* chi2z, int numBinAll,
+                                              const double* y_rd, const double* c1_rd, const double* c2_rd,
+                                              const double* c3_rd, const double* y_dm, const double* c1_dm,
+                                              const double* c2_dm, const double* c3_dm, double dspin,
+                                              int table_len, void* stream);
+
+/* ---- LISA TDI response (stateful, like pyFastLISAResponse) ------------------ */
+
+typedef struct bbhx_response bbhx_response;
+
+/* Replaces pyFastLISAResponse.__cinit__ / __dealloc__ (cutils/lisaresponse.pyx:34-40, :117-121). */
+int bbhx_hip_response_create(bbhx_response** out);
+int bbhx_hip_response_destroy(bbhx_response* r);
+
+/* Replaces pyFastLISAResponse.add_orbit_information (cutils/lisaresponse.pyx:41-71):
+ * orbit tables in the lisaanalysistools Orbits layout. n_arr (N, 6, 3),
+ * L_arr (N, 6) and x_arr (N, 3, 3) are device arrays the caller keeps alive;
+ * links, sc_r, sc_e are 6 host ints. Positions/unit vectors are linearly
+ * interpolated in time with step dt (response/fastfdresponse.py:174). */
+int bbhx_hip_response_add_orbits(bbhx_response* r, double dt, int N, const double* n_arr, const double* L_arr,
+                                 const double* x_arr, const int* links, const int* sc_r, const int* sc_e,
+                                 double armlength);
+
+/* Replaces pyFastLISAResponse.LISA_response_wrap (cutils/lisaresponse.pyx:78) ->
+ * FastLISAResponse::LISA_response (cutils/Response.cu:671-746). TDItag 1 = XYZ,
+ * 2 = AET. Writes the 6 transfer-function slabs and adds the orbital Doppler
+ * phase to the phase slab in place. order_fresnel_stencil must be 0. */
+int bbhx_hip_response_compute(bbhx_response* r, double* response_out, const int* ells, const int* mms,
+                              const double* freqs, const double* phi_ref, const double* inc, const double* lam,
+                              const double* beta, const double* psi, int TDItag, int rescaled, int tdi2,
+                              int order_fresnel_stencil, int numModes, int length, int numBinAll, int includesAmps,
+                              void* stream);
+
+/* ---- spline and template build --------------------------------------------- */
+
+/* Replaces interpolate_wrap (cutils/interp.pyx:14) -> interpolate
+ * (cutils/Interpolate.cu:320-359). Not-a-knot cubic splines of the (P, B, M, L)
+ * values over the (B, M, L) abscissae. B receives the knot slopes; upper_diag,
+ * diag, lower_diag receive c1, c2, c3 (as in the reference); the entry at the
+ * last knot of each is 0. length >= 3. */
+int bbhx_hip_interpolate(const double* freqs, const double* propArrays, double* B, double* upper_diag,
+                         double* diag, double* lower_diag, int length, int numInterpParams, int numModes,
+                         int numBinAll, void* stream);
+
+/* Replaces InterpTDI_wrap (cutils/bbhwaveformbuild.pyx:18) -> InterpTDI
+ * (cutils/WaveformBuild.cu:147-189). templateChannels_ptrs, inds_ptrs,
+ * inds_start, ind_lengths, t_start, t_end are HOST arrays of B entries (device
+ * addresses of each binary's (3, n_b) complex template and (M, n_b) int32
+ * segment indices). One launch for all binaries; waits for the upload of the
+ * B-entry band table before returning. */
+int bbhx_hip_InterpTDI(const int64_t* templateChannels_ptrs, const double* dataFreqs, const double* freqs,
+                       const double* propArrays, const double* c1, const double* c2, const double* c3,
+                       const double* t_start, const double* t_end, int length, int data_length, int numBinAll,
+                       int numModes, const int64_t* inds_ptrs, const int* inds_start, const int* ind_lengths,
+                       void* stream);
+
+/* Replaces direct_sum_wrap (cutils/bbhwaveformbuild.pyx:37) -> direct_sum
+ * (cutils/WaveformBuild.cu:194-296). Output (B, 3, L) complex; t_start/t_end (B,) device. */
+int bbhx_hip_direct_sum(double* templateChannels, const double* bbh_buffer, int numBinAll, int data_length,
+                        int nChannels, int numModes, const double* t_start, const double* t_end, void* stream);
+
+/* ---- inner products --------------------------------------------------------- */
+
+/* Replaces hdyn_wrap (cutils/bbhlikelihood.pyx:21) -> hdyn (cutils/Likelihood.cu:21-201).
+ * templateChannels: residuals r = h/h0, (K, 3, B) complex; dataConstants
+ * (4, 3, K) complex (A0, A1, B0, B1); outputs d_h, h_h (B,) complex.
+ * include_last_segment = 1 follows the reference GPU kernel (all K-1
+ * segments), 0 the reference CPU loop (stops one segment early, Likelihood.cu:150). */
+int bbhx_hip_hdyn(double* likeOut1, double* likeOut2, const double* templateChannels, const double* dataConstants,
+                  const double* dataFreqs, int numBinAll, int data_length, int nChannels, int include_last_segment,
+                  void* stream);
+
+/* Replaces direct_like_wrap (cutils/bbhlikelihood.pyx:42) -> direct_like
+ * (cutils/Likelihood.cu:357-471). d_h, h_h: (B,) complex DEVICE arrays,
+ * accumulated with += (the reference accumulates into host arrays).
+ * templateChannels_ptrs, inds_start, ind_lengths: HOST arrays of B entries.
+ * The templates are multiplied by the noise weights in place, as in the
+ * reference. One launch for all binaries; deterministic reduction. */
+int bbhx_hip_direct_like(double* d_h, double* h_h, const double* dataChannels, const double* noise_weight_times_df,
+                         const int64_t* templateChannels_ptrs, const int* inds_start, const int* ind_lengths,
+                         int data_stream_length, int numBinAll, void* stream);
+
+/* ---- fused heterodyned likelihood (no counterpart in the reference) --------- */
+
+/* One pass of HeterodynedLikelihood.get_ll (likelihood.py:550-648) without
+ * materialising the waveform buffer: for each template, PhenomHM amplitude /
+ * phase / t(f) (+ the t_ref shift, phenomhm.py:487-507), the TDI response, the
+ * windowed mode sum (WaveformBuild.cu:48-60), r = h / h0 and the heterodyned
+ * sums (Likelihood.cu:52-111), all at the K sparse nodes.
+ *
+ * params: (11, B) device, rows m1 [Msun], m2 [Msun], chi1z, chi2z,
+ *   distance [m], phi_ref, inc, lam, beta, psi, t_ref [s] (m1 >= m2 already enforced
+ *   by the host, as phenomhm.py:175-199 does).
+ * t_start, t_end: (B,) device, SSB-frame window.
+ * freqs (K,), h0 (3, K) complex, dataConstants (4, 3, K) complex, all device.
+ * qnm: the PhenomD QNM table spline, 8 arrays of table_len entries packed
+ *   contiguously (y_rd, c1_rd, c2_rd, c3_rd, y_dm, c1_dm, c2_dm, c3_dm).
+ * precision: 0 = FP64 throughout; 1 = FP32 bulk / FP64 phase (phases,
+ *   t(f) and the Doppler phase in FP64, transfer functions, amplitudes, the
+ *   residuals and the heterodyne sums in FP32 with compensated summation).
+ * Outputs d_h, h_h: (B,) complex. */
+int bbhx_hip_hdyn_fused(double* d_h, double* h_h, const double* params, const double* t_start,
+                        const double* t_end, int numBinAll, const int* ells_host, const int* mms_host,
+                        int numModes, const double* freqs, const double* h0, const double* dataConstants,
+                        int K, const double* qnm, int table_len, double dspin, double initial_t_val,
+                        const bbhx_response* orbits, int TDItag, int rescaled, int tdi2,
+                        int include_last_segment, int precision, void* stream);
+
+/* Event timing of bbhx_hip_hdyn_fused's two launches (setup, evaluation) on the
+ * caller's stream: enable(1) starts recording (and clears earlier records);
+ * read() waits for the recorded events and returns the summed milliseconds and
+ * the number of calls, then clears. For benchmarks; off by default. */
+int bbhx_hip_timing_enable(int on);
+int bbhx_hip_timing_read(double* ms_setup, double* ms_eval, int* ncalls);
+
+/* Releases the library's scratch buffers on the current device (synchronises it). */
+int bbhx_hip_release_workspace(void);
+
+/* Library version string. */
+const char* bbhx_hip_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BBHX_HIP_H */

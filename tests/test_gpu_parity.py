@@ -1,0 +1,321 @@
+"""Parity of every HIP kernel and of the end-to-end likelihoods with the FP64
+CPU oracle, on identical seeded inputs, through the C ABI.
+
+Tolerances (FP64 both sides; differences come from operation order, FMA
+contraction and libm ulps only):
+  ringdown frequencies, PhenomHM amplitude, t(f), transfer functions: 1e-10 relative
+  PhenomHM phase: 1e-11 relative to each waveform's phase scale (phases reach 1e5-1e6 rad)
+  spline coefficients: 1e-9 relative to the row scale
+  templates / inner products: 1e-9 relative
+  log-likelihoods: |delta logL| < 1e-6 absolute
+"""
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle.pipeline import ALL_MODES, OracleHeterodyne, OracleLikelihood, OracleWaveform
+from tests.params import MODES, YRSID_SI, draw, perturb
+
+pytestmark = pytest.mark.gpu
+
+MSUN_SI = 1.98848e+30
+MTSUN_SI = 4.925491025873693e-06
+
+
+def dev(x, dtype=None):
+    import torch
+
+    dtype = torch.float64 if dtype is None else dtype
+    return torch.as_tensor(np.ascontiguousarray(x), dtype=dtype, device="cuda")
+
+
+def host(t):
+    return t.detach().cpu().numpy()
+
+
+def rel_err(a, b, axis=-1):
+    scale = np.maximum(np.max(np.abs(b), axis=axis, keepdims=True), 1e-300)
+    return np.max(np.abs(a - b) / scale)
+
+
+def ringdown_inputs(p, modes=MODES):
+    m1, m2, c1, c2 = p[0].copy(), p[1].copy(), p[2].copy(), p[3].copy()
+    ells = np.array([l for l, m in modes], np.int32)
+    mms = np.array([m for l, m in modes], np.int32)
+    return m1, m2, c1, c2, ells, mms
+
+
+@pytest.fixture(scope="module")
+def orbits():
+    from bbhx_amd.utils.orbits import EqualArmlengthOrbits
+
+    return EqualArmlengthOrbits(dt=1e4, t_max=3.0 * YRSID_SI)
+
+
+def test_ringdown_frequencies(backend):
+    import torch
+
+    p = draw(64, 11)
+    m1, m2, c1, c2, ells, mms = ringdown_inputs(p)
+    B, M = m1.size, ells.size
+    tab = oracle.qnm_table()
+    fr_o, fd_o = oracle.get_phenomd_ringdown_frequencies(m1, m2, c1, c2)
+    hr_o, hd_o = oracle.get_phenomhm_ringdown_frequencies(m1, m2, c1, c2, ells, mms)
+    fr, fd = torch.zeros(B, dtype=torch.float64, device="cuda"), torch.zeros(B, dtype=torch.float64, device="cuda")
+    args = [dev(np.pad(tab[k], (0, 1001 - len(tab[k])))) for k in
+            ("y_rd", "c1_rd", "c2_rd", "c3_rd", "y_dm", "c1_dm", "c2_dm", "c3_dm")]
+    backend.get_phenomd_ringdown_frequencies(fr, fd, dev(m1), dev(m2), dev(c1), dev(c2), B, *args, tab["dspin"])
+    hr = torch.zeros(B * M, dtype=torch.float64, device="cuda")
+    hd = torch.zeros_like(hr)
+    backend.get_phenomhm_ringdown_frequencies(hr, hd, dev(m1), dev(m2), dev(c1), dev(c2), dev(ells, torch.int32),
+                                              dev(mms, torch.int32), M, B)
+    np.testing.assert_allclose(host(fr), fr_o, rtol=1e-12)
+    np.testing.assert_allclose(host(fd), fd_o, rtol=1e-12)
+    np.testing.assert_allclose(host(hr).reshape(B, M), hr_o, rtol=1e-12)
+    np.testing.assert_allclose(host(hd).reshape(B, M), hd_o, rtol=1e-12)
+
+
+def _phm_case(backend, p, modes, L, run_phenomd=False):
+    import torch
+
+    m1, m2, c1, c2, ells, mms = ringdown_inputs(p, modes)
+    sw = m1 < m2
+    m1[sw], m2[sw] = m2[sw].copy(), m1[sw].copy()
+    c1[sw], c2[sw] = c2[sw].copy(), c1[sw].copy()
+    B, M = m1.size, ells.size
+    frd, fdm = oracle.get_phenomd_ringdown_frequencies(m1, m2, c1, c2)
+    if run_phenomd:
+        Mf_RD, Mf_DM = frd, fdm
+    else:
+        hr, hd = oracle.get_phenomhm_ringdown_frequencies(m1, m2, c1, c2, ells, mms)
+        Mf_RD = np.concatenate([hr, frd[:, None]], 1).ravel()
+        Mf_DM = np.concatenate([hd, fdm[:, None]], 1).ravel()
+    Msec = (m1 + m2) * MTSUN_SI
+    base = np.logspace(-4, np.log10(0.6), L)
+    freqs = (base[None, None, :] / Msec[:, None, None] * mms[None, :, None] / 2).ravel()
+    out_o = np.zeros(3 * B * M * L)
+    oracle.waveform_amp_phase(out_o, ells, mms, freqs, m1 * MSUN_SI, m2 * MSUN_SI, c1, c2, p[4], p[6], M, L, B, Mf_RD,
+                              Mf_DM, run_phenomd)
+    out = torch.zeros(3 * B * M * L, dtype=torch.float64, device="cuda")
+    backend.waveform_amp_phase_wrap(out, dev(ells, torch.int32), dev(mms, torch.int32), dev(freqs),
+                                    dev(m1 * MSUN_SI), dev(m2 * MSUN_SI), dev(c1), dev(c2), dev(p[4]), dev(p[6]), M,
+                                    L, B, dev(Mf_RD), dev(Mf_DM), run_phenomd)
+    return host(out).reshape(3, B, M, L), out_o.reshape(3, B, M, L)
+
+
+@pytest.mark.parametrize("case", ["hm6", "phenomd", "single21", "equal_mass"])
+def test_waveform_amp_phase(backend, case):
+    p = draw(24, 12)
+    modes, run_phenomd = MODES, False
+    if case == "phenomd":
+        modes, run_phenomd = [(2, 2)], True
+    elif case == "single21":
+        modes = [(2, 1)]
+    elif case == "equal_mass":
+        p[1] = p[0]  # m1 == m2 with unequal spins: the odd-m modes vanish
+    g, o = _phm_case(backend, p, modes, 384, run_phenomd)
+    assert np.all(np.isfinite(g))
+    assert rel_err(g[0], o[0]) < 1e-10
+    assert rel_err(g[1], o[1]) < 1e-11  # phase, relative to each waveform's phase scale
+    assert rel_err(g[2], o[2]) < 1e-10
+
+
+def _response_case(backend, orbits, TDItag, rescaled, tdi2, B=12, L=200, modes=MODES):
+    import torch
+
+    from bbhx_amd.response.fastfdresponse import LISATDIResponse
+
+    p = draw(B, 13)
+    g, o = _phm_case(backend, p, modes, L)
+    M = len(modes)
+    nper = B * M * L
+    # shift tf into the orbit span like run_wave does with t_ref
+    buf = np.zeros(9 * nper)
+    buf[:3 * nper] = o.ravel()
+    buf[2 * nper:3 * nper] += np.repeat(p[11], M * L)
+    m1, m2, c1, c2, ells, mms = ringdown_inputs(p, modes)
+    Msec = np.maximum(p[0], p[1]) * MTSUN_SI + np.minimum(p[0], p[1]) * MTSUN_SI
+    base = np.logspace(-4, np.log10(0.6), L)
+    freqs = (base[None, None, :] / Msec[:, None, None] * mms[None, :, None] / 2).ravel()
+    buf_o = buf.copy()
+    oracle.lisa_response(buf_o, ells, mms, freqs, p[5], p[7], p[8], p[9], p[10], 1 if TDItag == "XYZ" else 2,
+                         rescaled, tdi2, M, L, B, 1, orbits.as_host_dict())
+    resp = LISATDIResponse(TDItag=TDItag, orbits=orbits, rescaled=rescaled, tdi2=tdi2, force_backend=backend)
+    buf_g = dev(buf)
+    resp(dev(freqs), p[7], p[8], p[9], p[10], p[5], L, modes=modes, out_buffer=buf_g, direct=True)
+    return host(buf_g).reshape(9, B, M, L), buf_o.reshape(9, B, M, L)
+
+
+@pytest.mark.parametrize("TDItag,rescaled,tdi2", [("AET", False, True), ("AET", False, False), ("AET", True, True),
+                                                  ("XYZ", False, False)])
+def test_response(backend, orbits, TDItag, rescaled, tdi2):
+    g, o = _response_case(backend, orbits, TDItag, rescaled, tdi2)
+    assert np.all(np.isfinite(g))
+    for k in range(3, 9):
+        assert rel_err(g[k], o[k]) < 1e-10, k
+    assert rel_err(g[1], o[1]) < 1e-11
+
+
+def test_spline(backend):
+    import torch
+    from scipy.interpolate import CubicSpline
+
+    from bbhx_amd.utils.interpolate import CubicSplineInterpolant
+
+    rng = np.random.default_rng(3)
+    B, M, P, L = 5, 3, 9, 300
+    x = np.sort(np.exp(rng.uniform(np.log(1e-4), np.log(1e-1), (B, M, L))), axis=-1)
+    y = np.sin(40 * x[None] * (1 + np.arange(P)[:, None, None, None])) + 0.1 * rng.standard_normal((P, B, M, L))
+    spl = CubicSplineInterpolant(dev(x), dev(y), force_backend=backend)
+    Bo, c1o, c2o, c3o = oracle.interpolate(x.ravel(), y.ravel(), L, P, M, B)
+    for g, o in ((spl.c1, c1o), (spl.c2, c2o), (spl.c3, c3o)):
+        g = host(g).reshape(P * B * M, L)[:, :-1]
+        o = o.reshape(P * B * M, L)[:, :-1]
+        assert rel_err(g, o) < 1e-9
+    # scipy not-a-knot on one system
+    cs = CubicSpline(x[2, 1], y[4, 2, 1], bc_type="not-a-knot")
+    np.testing.assert_allclose(host(spl.c1_shaped[4, 2, 1, :-1]), cs.c[2], rtol=1e-8, atol=1e-8 * np.abs(cs.c[2]).max())
+    np.testing.assert_allclose(host(spl.c3_shaped[4, 2, 1, :-1]), cs.c[0], rtol=1e-7, atol=1e-7 * np.abs(cs.c[0]).max())
+
+
+def test_direct_sum_and_hdyn(backend):
+    import torch
+
+    rng = np.random.default_rng(5)
+    B, M, L = 7, 6, 130
+    buf = rng.standard_normal(9 * B * M * L)
+    nper = B * M * L
+    buf[:nper] = np.abs(buf[:nper])
+    buf[nper:2 * nper] *= 1e5
+    buf[2 * nper:3 * nper] = rng.uniform(-1e6, 3e7, nper)
+    t_start = rng.uniform(0, 1e6, B)
+    t_end = np.where(rng.uniform(size=B) > 0.3, rng.uniform(1e7, 3e7, B), 0.0)
+    o = oracle.direct_sum(buf, B, L, M, t_start, t_end)
+    g = torch.zeros(B * 3 * L, dtype=torch.complex128, device="cuda")
+    backend.direct_sum_wrap(g, dev(buf), B, L, 3, M, dev(t_start), dev(t_end))
+    assert rel_err(host(g).reshape(B, 3, L), o) < 1e-10
+
+    K = 77
+    f = np.sort(rng.uniform(1e-4, 1e-2, K))
+    r = (1 + 0.01 * rng.standard_normal((K, 3, B))) * np.exp(1j * 0.1 * rng.standard_normal((K, 3, B)))
+    dc = (rng.standard_normal(4 * 3 * K) + 1j * rng.standard_normal(4 * 3 * K)) * 1e3
+    for last in (True, False):
+        o1, o2 = oracle.hdyn(r.ravel(), dc, f, B, K, include_last_segment=last)
+        g1 = torch.zeros(B, dtype=torch.complex128, device="cuda")
+        g2 = torch.zeros_like(g1)
+        backend.hdyn_wrap(g1, g2, dev(r.ravel(), torch.complex128), dev(dc, torch.complex128), dev(f), B, K, 3, last)
+        np.testing.assert_allclose(host(g1), o1, rtol=1e-10)
+        np.testing.assert_allclose(host(g2), o2, rtol=1e-10)
+
+
+def test_interp_and_direct_like(backend):
+    import torch
+
+    rng = np.random.default_rng(9)
+    B, M, L = 4, 2, 60
+    N = 4000
+    data_freqs = np.arange(1, N + 1) * 1e-5
+    x = np.sort(rng.uniform(2e-4, 3.5e-2, (B, M, L)), axis=-1)
+    y = rng.standard_normal(9 * B * M * L)
+    y[2 * B * M * L:3 * B * M * L] = rng.uniform(0, 1e7, B * M * L)
+    _, c1, c2, c3 = oracle.interpolate(x.ravel(), y, L, 9, M, B)
+    st = np.searchsorted(data_freqs, x.min(axis=(1, 2)), side="right")
+    et = np.searchsorted(data_freqs, x.max(axis=(1, 2)), side="right")
+    inds = [np.concatenate([np.searchsorted(x[i, j], data_freqs[st[i]:et[i]], side="right") - 1
+                            for j in range(M)]).astype(np.int32) for i in range(B)]
+    n = (et - st).astype(np.int32)
+    t_start = np.zeros(B)
+    t_end = np.full(B, 5e6)
+    o = oracle.interp_tdi(data_freqs, x.ravel(), y, c1, c2, c3, t_start, t_end, L, B, M, inds, st.astype(np.int32), n)
+    tm = [torch.zeros(3 * int(k), dtype=torch.complex128, device="cuda") for k in n]
+    ind_d = [dev(a, torch.int32) for a in inds]
+    backend.InterpTDI_wrap(np.array([t.data_ptr() for t in tm]), dev(data_freqs), dev(x.ravel()), dev(y), dev(c1),
+                           dev(c2), dev(c3), t_start, t_end, L, N, B, M, np.array([a.data_ptr() for a in ind_d]),
+                           st.astype(np.int32), n)
+    for gt, ot in zip(tm, o):
+        assert rel_err(host(gt).reshape(3, -1), ot) < 1e-9
+
+    data = rng.standard_normal((3, N)) + 1j * rng.standard_normal((3, N))
+    w = rng.uniform(0.5, 2.0, 3 * N)
+    d_o, h_o = oracle.direct_like(data.ravel(), w, [t.copy() for t in o], st.astype(np.int32), n, N)
+    d_g = torch.zeros(B, dtype=torch.complex128, device="cuda")
+    h_g = torch.zeros_like(d_g)
+    backend.direct_like_wrap(d_g, h_g, dev(data.ravel(), torch.complex128), dev(w),
+                             np.array([t.data_ptr() for t in tm]), st.astype(np.int32), n, N, B)
+    np.testing.assert_allclose(host(d_g), d_o, rtol=1e-9)
+    np.testing.assert_allclose(host(h_g), h_o, rtol=1e-9)
+
+
+# ------------------------------------------------------------------ end to end
+
+
+@pytest.fixture(scope="module")
+def small_problem(orbits):
+    """Injection + Gaussian noise on a 2^15-bin grid with 4/yr spacing."""
+    from bbhx_amd.utils.sensitivity import AET1SensitivityMatrix
+
+    ref = draw(1, 1001)[:, 0]
+    ref[0], ref[1] = 1.3e6, 4.1e5
+    ref[11] = 0.62 * YRSID_SI
+    f = np.arange(1, 2**15 + 1) * (4.0 / YRSID_SI)
+    owave = OracleWaveform(orbits.as_host_dict())
+    inj = owave(*ref, freqs=f, length=1024, modes=MODES, direct=False, fill=True, squeeze=True)[0]
+    sens = AET1SensitivityMatrix(f)
+    psd = np.array([sens[0], sens[1], sens[2]])
+    rng = np.random.default_rng(1001)
+    df = f[1] - f[0]
+    noise = np.sqrt(psd / (4 * df)) * (rng.standard_normal(psd.shape) + 1j * rng.standard_normal(psd.shape))
+    return dict(ref=ref, f=f, data=inj + noise, psd=psd, owave=owave)
+
+
+def test_full_waveform(backend, orbits, small_problem):
+    from bbhx_amd.waveformbuild import BBHWaveformFD
+
+    sp = small_problem
+    p = perturb(sp["ref"], 5, 21, scale=1e-4)
+    wave = BBHWaveformFD(response_kwargs=dict(orbits=orbits), force_backend=backend)
+    g = host(wave(*p, freqs=sp["f"], length=1024, modes=MODES, direct=False, fill=True))
+    o = sp["owave"](*p, freqs=sp["f"], length=1024, modes=MODES, direct=False, fill=True)
+    assert np.all(np.isfinite(g))
+    assert rel_err(g, o, axis=None) < 1e-8
+
+
+def test_direct_likelihood(backend, orbits, small_problem):
+    from bbhx_amd.likelihood import Likelihood
+    from bbhx_amd.waveformbuild import BBHWaveformFD
+
+    sp = small_problem
+    p = perturb(sp["ref"], 6, 22, scale=1e-5)
+    wave = BBHWaveformFD(response_kwargs=dict(orbits=orbits), force_backend=backend)
+    like = Likelihood(wave, sp["f"], sp["data"], sp["psd"], force_backend=backend)
+    ll = like.get_ll(p, length=1024, modes=MODES)
+    olike = OracleLikelihood(sp["owave"], sp["f"], sp["data"], sp["psd"])
+    ll_o = olike.get_ll(p, length=1024, modes=MODES)[0]
+    assert np.all(np.isfinite(ll))
+    assert np.max(np.abs(ll - ll_o)) < 1e-6
+
+
+@pytest.mark.parametrize("K,last", [(128, "gpu"), (100, "cpu"), (257, "gpu")])
+def test_heterodyned_likelihood(backend, orbits, small_problem, K, last):
+    from bbhx_amd.likelihood import HeterodynedLikelihood
+    from bbhx_amd.utils.sensitivity import AET1SensitivityMatrix
+    from bbhx_amd.waveformbuild import BBHWaveformFD
+
+    sp = small_problem
+    wave = BBHWaveformFD(response_kwargs=dict(orbits=orbits), force_backend=backend)
+    sens = AET1SensitivityMatrix(sp["f"])
+    het = HeterodynedLikelihood(wave, sp["f"], sp["data"], sp["ref"], K, sens_mat=sens,
+                                reference_gen_kwargs=dict(length=1024), force_backend=backend, hdyn_last_bin=last)
+    psd_fn = lambda x: [AET1SensitivityMatrix(x)[i] for i in range(3)]  # noqa: E731
+    ohet = OracleHeterodyne(sp["owave"], sp["f"], sp["data"], sp["ref"], K, psd_fn, reference_length=1024,
+                            include_last_segment=(last == "gpu"))
+    assert abs(het.reference_ll - ohet.reference_ll) < 1e-6
+    np.testing.assert_allclose(host(het.freqs), ohet.freqs, rtol=1e-14)
+    p = perturb(sp["ref"], 9, 23, scale=1e-4)
+    ll_o = ohet.get_ll(p)[0]
+    ll_f = het.get_ll(p, fused=True)
+    ll_p = het.get_ll(p, fused=False)
+    assert np.all(np.isfinite(ll_f))
+    assert np.max(np.abs(ll_p - ll_o)) < 1e-6, (ll_p, ll_o)
+    assert np.max(np.abs(ll_f - ll_o)) < 1e-6, (ll_f, ll_o)

@@ -1,0 +1,405 @@
+"""Pins of the FP64 CPU oracle (CPU only).
+
+Each oracle function is checked against something that does not share its
+code: the reference's own CPU build (oracle/_ref, when built here), the golden
+vectors that build wrote (tests/golden/reference_kats.npz, committed), scipy,
+and closed forms / invariants chosen so that a dropped term, a wrong sign or a
+swapped index fails.
+"""
+
+from pathlib import Path
+
+import numpy as np
+import pytest
+from scipy.interpolate import CubicSpline
+
+import oracle
+from tests.params import MODES, YRSID_SI, draw
+
+MSUN_SI = 1.98848e+30
+MTSUN_SI = 4.925491025873693e-06
+GOLD = Path(__file__).resolve().parent / "golden" / "reference_kats.npz"
+
+
+@pytest.fixture(scope="module")
+def kats():
+    return np.load(GOLD, allow_pickle=False)
+
+
+def rel(a, b, axis=-1):
+    s = np.maximum(np.max(np.abs(b), axis=axis, keepdims=True), 1e-300)
+    return np.max(np.abs(a - b) / s)
+
+
+# ---------------------------------------------------------------- golden vectors
+
+
+def test_ringdown_vs_reference_golden(kats):
+    k = kats
+    frd, fdm = oracle.get_phenomd_ringdown_frequencies(k["phm_m1"], k["phm_m2"], k["phm_chi1"], k["phm_chi2"])
+    hr, hd = oracle.get_phenomhm_ringdown_frequencies(k["phm_m1"], k["phm_m2"], k["phm_chi1"], k["phm_chi2"],
+                                                      k["phm_ells"], k["phm_mms"])
+    np.testing.assert_allclose(frd, k["phm_frd"], rtol=1e-14)
+    np.testing.assert_allclose(fdm, k["phm_fdm"], rtol=1e-14)
+    np.testing.assert_allclose(hr, k["phm_hr"], rtol=1e-14)
+    np.testing.assert_allclose(hd, k["phm_hd"], rtol=1e-14)
+
+
+def test_phenomhm_vs_reference_golden(kats):
+    k = kats
+    B, M = k["phm_m1"].size, k["phm_ells"].size
+    L = k["phm_freqs"].size // (B * M)
+    out = np.zeros(3 * B * M * L)
+    oracle.waveform_amp_phase(out, k["phm_ells"], k["phm_mms"], k["phm_freqs"], k["phm_m1"] * MSUN_SI,
+                              k["phm_m2"] * MSUN_SI, k["phm_chi1"], k["phm_chi2"], k["phm_dist"], np.zeros(B), M, L,
+                              B, k["phm_Mf_RD"], k["phm_Mf_DM"], 0)
+    o = out.reshape(3, B, M, L)
+    g = k["phm_out"].reshape(3, B, M, L)
+    assert rel(o[0], g[0]) < 1e-13
+    assert rel(o[1], g[1]) < 1e-13  # phase: relative to the waveform's phase scale
+    assert rel(o[2], g[2]) < 1e-13
+    # the equal-mass binary (index 0): (3,3) and (4,3) are proportional to
+    # delta = sqrt(1 - 4 eta) = 0 and vanish in both
+    odd = (k["phm_ells"] + k["phm_mms"]) % 2 == 0
+    odd &= k["phm_mms"] % 2 == 1
+    assert np.all(g[0, 0, odd] == 0.0) and np.all(o[0, 0, odd] == 0.0)
+
+
+def test_phenomd_vs_reference_golden(kats):
+    k = kats
+    B = k["phm_m1"].size
+    L = k["phd_freqs"].size // B
+    out = np.zeros(3 * B * L)
+    oracle.waveform_amp_phase(out, np.array([2], np.int32), np.array([2], np.int32), k["phd_freqs"],
+                              k["phm_m1"] * MSUN_SI, k["phm_m2"] * MSUN_SI, k["phm_chi1"], k["phm_chi2"],
+                              k["phm_dist"], np.zeros(B), 1, L, B, k["phm_frd"], k["phm_fdm"], 1)
+    o = out.reshape(3, B, L)
+    g = k["phd_out"].reshape(3, B, L)
+    assert rel(o[0], g[0]) < 1e-13
+    assert rel(o[1], g[1]) < 1e-13  # phase: relative to the waveform's phase scale
+    assert rel(o[2], g[2]) < 1e-13
+
+
+def test_direct_sum_vs_reference_golden(kats):
+    k = kats
+    o = oracle.direct_sum(k["ds_buf"], 3, 40, 6, k["ds_ts"], k["ds_te"])
+    np.testing.assert_allclose(o, k["ds_out"], rtol=1e-13, atol=1e-13 * np.abs(k["ds_out"]).max())
+
+
+def test_interp_tdi_vs_reference_golden(kats):
+    k = kats
+    st, ln = k["it_st"], k["it_len"]
+    Bi, Mi, Li = k["it_x"].shape
+    offs = np.concatenate([[0], np.cumsum(Mi * ln)])
+    inds = [k["it_inds"][offs[i]:offs[i + 1]] for i in range(Bi)]
+    tm = oracle.interp_tdi(k["it_df"], k["it_x"].ravel(), k["it_y"], k["it_c1"], k["it_c2"], k["it_c3"], k["it_ts"],
+                           k["it_te"], Li, Bi, Mi, inds, st, ln)
+    got = np.concatenate([t.ravel() for t in tm])
+    np.testing.assert_allclose(got, k["it_out"], rtol=1e-13, atol=1e-13 * np.abs(k["it_out"]).max())
+
+
+def test_hdyn_vs_reference_golden(kats):
+    k = kats
+    o1, o2 = oracle.hdyn(k["hd_r"], k["hd_dc"], k["hd_f"], 5, 50, include_last_segment=False)
+    np.testing.assert_allclose(o1, k["hd_dh"], rtol=1e-13)
+    np.testing.assert_allclose(o2, k["hd_hh"], rtol=1e-13)
+
+
+def test_direct_like_vs_reference_golden(kats):
+    k = kats
+    st, ln = k["it_st"], k["it_len"]
+    offs = np.concatenate([[0], np.cumsum(3 * ln)])
+    tm = [k["it_out"][offs[i]:offs[i + 1]].reshape(3, -1).copy() for i in range(len(st))]
+    dh, hh = oracle.direct_like(k["dl_data"], k["dl_w"], tm, st, ln, 500)
+    np.testing.assert_allclose(dh, k["dl_dh"], rtol=1e-12)
+    np.testing.assert_allclose(hh, k["dl_hh"], rtol=1e-12)
+
+
+# ------------------------------------------------ live reference build (optional)
+
+
+@pytest.mark.skipif(not oracle.ref_available(), reason="reference CPU build not present")
+def test_phenomhm_vs_live_reference_random():
+    p = draw(40, 77, mtot=(1e4, 1e8))
+    m1, m2, c1, c2 = p[0], p[1], p[2], p[3]
+    sw = m1 < m2
+    m1[sw], m2[sw] = m2[sw].copy(), m1[sw].copy()
+    ells = np.array([l for l, m in MODES], np.int32)
+    mms = np.array([m for l, m in MODES], np.int32)
+    res = []
+    for use_ref in (False, True):
+        frd, fdm = oracle.get_phenomd_ringdown_frequencies(m1, m2, c1, c2, use_ref=use_ref)
+        hr, hd = oracle.get_phenomhm_ringdown_frequencies(m1, m2, c1, c2, ells, mms, use_ref=use_ref)
+        Mf_RD = np.concatenate([hr, frd[:, None]], 1).ravel()
+        Mf_DM = np.concatenate([hd, fdm[:, None]], 1).ravel()
+        Msec = (m1 + m2) * MTSUN_SI
+        base = np.logspace(-4, np.log10(0.6), 100)
+        freqs = (base[None, None, :] / Msec[:, None, None] * mms[None, :, None] / 2).ravel()
+        out = np.zeros(3 * 40 * 6 * 100)
+        oracle.waveform_amp_phase(out, ells, mms, freqs, m1 * MSUN_SI, m2 * MSUN_SI, c1, c2, p[4], p[6], 6, 100, 40,
+                                  Mf_RD, Mf_DM, 0, use_ref=use_ref)
+        res.append(out.reshape(3, 40, 6, 100))
+    o, r = res
+    assert rel(o[0], r[0]) < 1e-13
+    assert rel(o[1], r[1]) < 1e-13
+    assert rel(o[2], r[2]) < 1e-13
+
+
+# ------------------------------------------------------------ spline pins
+
+
+def test_spline_matches_scipy_not_a_knot():
+    rng = np.random.default_rng(1)
+    B, M, P, L = 3, 2, 4, 200
+    x = np.sort(np.exp(rng.uniform(np.log(1e-4), np.log(1e-1), (B, M, L))), axis=-1)
+    y = np.cos(30 * x[None]) * (1 + np.arange(P))[:, None, None, None] + 0.05 * rng.standard_normal((P, B, M, L))
+    _, c1, c2, c3 = oracle.interpolate(x.ravel(), y.ravel(), L, P, M, B)
+    c1, c2, c3 = (c.reshape(P, B, M, L) for c in (c1, c2, c3))
+    for p in range(P):
+        for b in range(B):
+            for m in range(M):
+                cs = CubicSpline(x[b, m], y[p, b, m], bc_type="not-a-knot")
+                np.testing.assert_allclose(c1[p, b, m, :-1], cs.c[2], rtol=1e-9, atol=1e-9 * np.abs(cs.c[2]).max())
+                np.testing.assert_allclose(c2[p, b, m, :-1], cs.c[1], rtol=1e-8, atol=1e-8 * np.abs(cs.c[1]).max())
+                np.testing.assert_allclose(c3[p, b, m, :-1], cs.c[0], rtol=1e-7, atol=1e-7 * np.abs(cs.c[0]).max())
+
+
+def test_spline_reproduces_cubics_exactly():
+    """A not-a-knot spline of a cubic is that cubic."""
+    x = np.sort(np.random.default_rng(2).uniform(0, 3, 37))
+    y = 0.3 - 1.2 * x + 0.7 * x**2 - 0.25 * x**3
+    _, c1, c2, c3 = oracle.interpolate(x, y, 37, 1, 1, 1)
+    xi = x[:-1]
+    # exact up to the conditioning of the random knot spacing
+    np.testing.assert_allclose(c1[:-1], -1.2 + 1.4 * xi - 0.75 * xi**2, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(c2[:-1], 0.7 - 0.75 * xi, rtol=0, atol=1e-10)
+    np.testing.assert_allclose(c3[:-1], -0.25, rtol=1e-8)
+
+
+# ------------------------------------------------------------ PhenomHM invariants
+
+
+def _phm(p, modes, L=300, run_phenomd=False, freqs=None, mf=(1e-4, 0.6)):
+    m1, m2, c1, c2 = p[0].copy(), p[1].copy(), p[2].copy(), p[3].copy()
+    sw = m1 < m2
+    m1[sw], m2[sw] = m2[sw].copy(), m1[sw].copy()
+    c1[sw], c2[sw] = c2[sw].copy(), c1[sw].copy()
+    B = m1.size
+    ells = np.array([l for l, m in modes], np.int32)
+    mms = np.array([m for l, m in modes], np.int32)
+    M = ells.size
+    frd, fdm = oracle.get_phenomd_ringdown_frequencies(m1, m2, c1, c2)
+    if run_phenomd:
+        Mf_RD, Mf_DM = frd, fdm
+    else:
+        hr, hd = oracle.get_phenomhm_ringdown_frequencies(m1, m2, c1, c2, ells, mms)
+        Mf_RD = np.concatenate([hr, frd[:, None]], 1).ravel()
+        Mf_DM = np.concatenate([hd, fdm[:, None]], 1).ravel()
+    Msec = (m1 + m2) * MTSUN_SI
+    if freqs is None:
+        base = np.logspace(np.log10(mf[0]), np.log10(mf[1]), L)
+        freqs = (base[None, None, :] / Msec[:, None, None] * mms[None, :, None] / 2).ravel()
+    out = np.zeros(3 * B * M * L)
+    oracle.waveform_amp_phase(out, ells, mms, freqs, m1 * MSUN_SI, m2 * MSUN_SI, c1, c2, p[4], p[6], M, L, B, Mf_RD,
+                              Mf_DM, run_phenomd)
+    return out.reshape(3, B, M, L), freqs.reshape(B, M, L), Msec
+
+
+def test_amplitude_inverse_distance():
+    p = draw(6, 3)
+    a, _, _ = _phm(p, MODES)
+    p2 = p.copy()
+    p2[4] *= 3.7
+    b, _, _ = _phm(p2, MODES)
+    np.testing.assert_allclose(b[0] * 3.7, a[0], rtol=1e-14)
+    np.testing.assert_array_equal(b[1], a[1])
+
+
+def test_mass_scaling():
+    """At fixed q, spins and Mf, phase and t(f)/M are mass independent and amp/(M^2/d) too."""
+    p = draw(4, 4)
+    p2 = p.copy()
+    p2[0] *= 13.0
+    p2[1] *= 13.0
+    a, fa, Ma = _phm(p, MODES)
+    b, fb, Mb = _phm(p2, MODES)
+    np.testing.assert_allclose(fa * Ma[:, None, None], fb * Mb[:, None, None], rtol=1e-13)
+    assert rel(a[1], b[1]) < 1e-12
+    np.testing.assert_allclose(a[2] / Ma[:, None, None], b[2] / Mb[:, None, None], rtol=1e-11, atol=1e-9)
+    np.testing.assert_allclose(a[0] / Ma[:, None, None] ** 2, b[0] / Mb[:, None, None] ** 2, rtol=1e-11)
+
+
+def test_hm22_equals_phenomd():
+    """With only (2,2), Rholm = Taulm = 1 and every HM map is the identity, so
+    PhenomHM's (2,2) mode is PhenomD."""
+    p = draw(5, 5)
+    hm, f, _ = _phm(p, [(2, 2)])
+    d, _, _ = _phm(p, [(2, 2)], run_phenomd=True, freqs=f.ravel())
+    assert rel(hm[0], d[0]) < 1e-12
+    assert rel(hm[1], d[1]) < 1e-12
+    assert rel(hm[2], d[2]) < 1e-12
+
+
+def test_tf_is_phase_derivative():
+    """t(f) = (1/2pi) d phase / df inside each region of every mode."""
+    p = draw(3, 6)
+    a, f, _ = _phm(p, MODES, L=4000)
+    phase, tf = a[1], a[2]
+    dphi = np.diff(phase, axis=-1) / np.diff(f, axis=-1) / (2 * np.pi)
+    tmid = 0.5 * (tf[..., 1:] + tf[..., :-1])
+    # error of the finite difference relative to each mode's time scale; a wrong
+    # offset (t0), factor (2 pi, M) or map slope would be O(1)
+    err = np.abs(dphi - tmid) / np.abs(tf).max(axis=-1, keepdims=True)
+    assert np.median(err) < 1e-8
+    assert np.max(err) < 1e-4
+
+
+def test_qnm_table_schwarzschild():
+    """PhenomD table at a = 0 is the Schwarzschild (2,2,0) QNM, M omega = 0.37367 - 0.08896 i."""
+    from bbhx_amd.waveforms.ringdownphenomd import QNMData_a, QNMData_fdamp, QNMData_fring
+
+    i0 = np.argmin(np.abs(QNMData_a))
+    assert QNMData_a[i0] == 0.0
+    assert abs(QNMData_fring[i0] * 2 * np.pi - 0.37367) < 2e-5
+    assert abs(QNMData_fdamp[i0] * 2 * np.pi - 0.08896) < 2e-5
+
+
+def test_equal_mass_odd_modes_vanish():
+    """For m1 = m2: (3,3) and (4,3) are proportional to delta = 0; (2,1) keeps its
+    spin-difference term, and vanishes too once chi1 = chi2."""
+    p = draw(3, 8)
+    p[1] = p[0]
+    a, _, _ = _phm(p, MODES)
+    lm = list(MODES)
+    for mode in ((3, 3), (4, 3)):
+        assert np.all(a[0][:, lm.index(mode)] == 0.0)
+    assert np.all(a[0][:, lm.index((2, 1))] > 0.0)
+    for mode in ((2, 2), (3, 2), (4, 4)):
+        assert np.all(a[0][:, lm.index(mode)] > 0.0)
+    p[3] = p[2]
+    a, _, _ = _phm(p, MODES)
+    assert np.all(a[0][:, lm.index((2, 1))] == 0.0)
+
+
+# ------------------------------------------------------------ response pins
+
+
+def _resp(TDItag, rescaled, tdi2, orbit, B=3, L=50):
+    p = draw(B, 9)
+    a, f, _ = _phm(p, MODES, L=L)
+    M = len(MODES)
+    nper = B * M * L
+    buf = np.zeros(9 * nper)
+    buf[:3 * nper] = a.ravel()
+    buf[2 * nper:3 * nper] += np.repeat(p[11], M * L)
+    ells = np.array([l for l, m in MODES], np.int32)
+    mms = np.array([m for l, m in MODES], np.int32)
+    oracle.lisa_response(buf, ells, mms, f.ravel(), p[5], p[7], p[8], p[9], p[10], TDItag, rescaled, tdi2, M, L, B,
+                         1, orbit)
+    b = buf.reshape(9, B, M, L)
+    T = [b[3] + 1j * b[4], b[5] + 1j * b[6], b[7] + 1j * b[8]]
+    return T, b[1], f
+
+
+@pytest.fixture(scope="module")
+def orbit():
+    from bbhx_amd.utils.orbits import EqualArmlengthOrbits
+
+    return EqualArmlengthOrbits(dt=1e4, t_max=2 * YRSID_SI).as_host_dict()
+
+
+def test_response_rescaled_and_tdi2_factors(orbit):
+    Tu, _, f = _resp(2, False, False, orbit)
+    Tr, _, _ = _resp(2, True, False, orbit)
+    T2, _, _ = _resp(2, False, True, orbit)
+    x = np.pi * f * orbit["armlength"] / 299792458.
+    fAE = 1j * np.sqrt(2) * np.sin(2 * x) * np.exp(2j * x)
+    fT = 2 * np.sqrt(2) * np.sin(2 * x) * np.sin(x) * np.exp(3j * x)
+    f2 = -2j * np.sin(4 * x) * np.exp(4j * x)
+    for k, fac in ((0, fAE), (1, fAE), (2, fT)):
+        np.testing.assert_allclose(Tu[k], fac * Tr[k], rtol=1e-10, atol=1e-12 * np.abs(Tu[k]).max())
+        np.testing.assert_allclose(T2[k], f2 * Tu[k], rtol=1e-10, atol=1e-12 * np.abs(T2[k]).max())
+
+
+def test_response_xyz_to_aet(orbit):
+    """A = (Z - X)/sqrt2, E = (X - 2Y + Z)/sqrt6, T = (X + Y + Z)/sqrt3 with the
+    reference's factor-2 AET convention."""
+    X, _, _ = _resp(1, False, False, orbit)
+    A, _, _ = _resp(2, False, False, orbit)
+    s = np.abs(A[0]).max()
+    np.testing.assert_allclose(A[0], (X[2] - X[0]) / np.sqrt(2), atol=1e-10 * s)
+    np.testing.assert_allclose(A[1], (X[0] - 2 * X[1] + X[2]) / np.sqrt(6), atol=1e-10 * s)
+    np.testing.assert_allclose(A[2], (X[0] + X[1] + X[2]) / np.sqrt(3), atol=1e-10 * np.abs(A[2]).max())
+
+
+def test_response_phase_shift_is_doppler(orbit):
+    """The response adds 2 pi f k.p0 / c to the phase, with p0 the guiding centre at 1 AU."""
+    B, L = 3, 50
+    p = draw(B, 9)
+    a, f, _ = _phm(p, MODES, L=L)
+    _, ph, _ = _resp(2, False, True, orbit, B=B, L=L)
+    tf = a[2] + p[11][:, None, None]
+    lam, beta = p[8][:, None, None], p[9][:, None, None]
+    alpha = 1.9909865927683788e-07 * tf
+    kR = -np.cos(beta) * np.cos(lam) * 149597870700.0 * np.cos(alpha) - np.cos(beta) * np.sin(
+        lam) * 149597870700.0 * np.sin(alpha)
+    np.testing.assert_allclose(ph - a[1], 2 * np.pi / 299792458. * f * kR, rtol=1e-9, atol=1e-9)
+
+
+# ------------------------------------------------------------ inner-product pins
+
+
+def test_hdyn_closed_forms():
+    rng = np.random.default_rng(4)
+    B, K = 3, 40
+    f = np.sort(rng.uniform(1e-4, 1e-2, K))
+    dc = (rng.standard_normal(12 * K) + 1j * rng.standard_normal(12 * K))
+    A0 = dc[:3 * K].reshape(3, K)
+    B0 = dc[6 * K:9 * K].reshape(3, K)
+    c = np.array([1.0, 0.7 - 0.2j, 1.3j])
+    r = np.broadcast_to(c[None, None, :], (K, 3, B)).copy()
+    for last, jend in ((True, K), (False, K - 1)):
+        dh, hh = oracle.hdyn(r.ravel(), dc, f, B, K, include_last_segment=last)
+        sA = A0[:, 1:jend].sum()
+        sB = B0[:, 1:jend].sum()
+        np.testing.assert_allclose(dh, np.conj(c) * sA, rtol=1e-12)
+        np.testing.assert_allclose(hh, np.abs(c) ** 2 * sB, rtol=1e-12)
+    # r linear in f: r0 = value at the segment midpoint, r1 = slope
+    a0, a1 = 0.9 + 0.1j, 30.0 - 5j
+    r = np.broadcast_to((a0 + a1 * f)[:, None, None], (K, 3, B)).copy()
+    dh, hh = oracle.hdyn(r.ravel(), dc, f, B, K, include_last_segment=True)
+    A1 = dc[3 * K:6 * K].reshape(3, K)
+    B1 = dc[9 * K:].reshape(3, K)
+    fm = 0.5 * (f[1:] + f[:-1])
+    r0 = a0 + a1 * fm
+    want_dh = (A0[:, 1:] * np.conj(r0) + A1[:, 1:] * np.conj(a1)).sum()
+    want_hh = (B0[:, 1:] * np.abs(r0) ** 2 + 2 * B1[:, 1:] * np.real(r0 * np.conj(a1))).sum()
+    np.testing.assert_allclose(dh, want_dh, rtol=1e-11)
+    np.testing.assert_allclose(hh, want_hh, rtol=1e-11)
+
+
+def test_direct_like_brute_force_and_invariants():
+    rng = np.random.default_rng(5)
+    N = 300
+    data = rng.standard_normal((3, N)) + 1j * rng.standard_normal((3, N))
+    w = rng.uniform(0.5, 2, 3 * N)
+    st = np.array([10, 0], np.int32)
+    ln = np.array([120, 300], np.int32)
+    t = [rng.standard_normal((3, n)) + 1j * rng.standard_normal((3, n)) for n in ln]
+    t0 = [x.copy() for x in t]
+    dh, hh = oracle.direct_like(data.ravel(), w, t, st, ln, N)
+    W = w.reshape(3, N)
+    for b in range(2):
+        s = slice(st[b], st[b] + ln[b])
+        hw = t0[b] * W[:, s]
+        np.testing.assert_allclose(dh[b], 4 * np.vdot(data[:, s].ravel(), hw.ravel()), rtol=1e-12)
+        np.testing.assert_allclose(hh[b], 4 * np.vdot(hw.ravel(), hw.ravel()), rtol=1e-12)
+        np.testing.assert_allclose(t[b], hw, rtol=1e-15)  # weighted in place
+    # h = d on the full band: <d|h> = <h|h>; h -> 2h doubles d_h and quadruples h_h
+    d_w = data * W
+    full = [data.copy()]
+    dh1, hh1 = oracle.direct_like(d_w.ravel(), w, full, np.array([0], np.int32), np.array([N], np.int32), N)
+    np.testing.assert_allclose(dh1, hh1, rtol=1e-12)
+    dh2, hh2 = oracle.direct_like(d_w.ravel(), w, [2 * data], np.array([0], np.int32), np.array([N], np.int32), N)
+    np.testing.assert_allclose(dh2, 2 * dh1, rtol=1e-12)
+    np.testing.assert_allclose(hh2, 4 * hh1, rtol=1e-12)
