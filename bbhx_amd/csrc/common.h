@@ -48,12 +48,93 @@ __host__ __device__ inline cd operator*(cd a, double s) { return cd{a.re * s, a.
 __host__ __device__ inline cd conj(cd a) { return cd{a.re, -a.im}; }
 __host__ __device__ inline cd& operator+=(cd& a, cd b) { a.re += b.re; a.im += b.im; return a; }
 
+// theta mod 2 pi in two Cody-Waite steps (2 pi = HI + LO to ~107 bits); exact
+// to the rounding of theta itself for |theta| up to ~1e13.
+__device__ inline double reduce_2pi(double theta)
+{
+    const double TWO_PI_HI = 6.283185307179586;
+    const double TWO_PI_LO = 2.4492935982947064e-16;
+    const double INV_TWO_PI = 0.15915494309189535;
+    double k = rint(theta * INV_TWO_PI);
+    double r = fma(-k, TWO_PI_HI, theta);
+    return fma(-k, TWO_PI_LO, r);
+}
+
+// 1/x for finite normal x != 0: the hardware reciprocal estimate refined by
+// two Newton steps e = 1 - x r, r += r e (within ~1 ulp of the rounded quotient).
+__device__ inline double rcp_pos(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
+// x^(1/3) for finite x > 0 in the single-precision range (all uses here are
+// frequencies and mass ratios): a single-precision estimate 2^(log2(x)/3)
+// (relative error ~1e-6) and one Halley step y (y^3 + 2x) / (2 y^3 + x), whose
+// cubic convergence leaves only FP64 rounding (within ~2 ulp of cbrt).
+__device__ inline double cbrt_pos(double x)
+{
+    const float yf = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf((float)x) * (1.0f / 3.0f));
+    const double y = (double)yf;
+    const double y3 = y * y * y;
+    return y * (y3 + 2.0 * x) * rcp_pos(2.0 * y3 + x);
+}
+
+// sin and cos for |x| <= ~4 (e.g. after reduce_2pi): quadrant reduction by
+// pi/2 = HI + LO, then the Taylor series of sin (to x^17) and cos (to x^18)
+// on |r| <= pi/4, whose truncation errors (< 1e-19) are below FP64 rounding.
+// Small and branch-light, unlike a general libm sincos with its large-argument path.
+__device__ inline void sincos_small(double x, double* s, double* c)
+{
+    const double PIO2_HI = 1.5707963267948966;
+    const double PIO2_LO = 6.123233995736766e-17;
+    const double TWO_OVER_PI = 0.6366197723675814;
+    const double q = rint(x * TWO_OVER_PI);
+    double r = fma(-q, PIO2_HI, x);
+    r = fma(-q, PIO2_LO, r);
+    const double r2 = r * r;
+    // sin(r) = r (1 - r^2/3! + r^4/5! - ...), cos(r) = 1 - r^2/2! + r^4/4! - ...
+    double ps = -1.0 / 355687428096000.0;            // -1/17!
+    ps = fma(ps, r2, 1.0 / 1307674368000.0);          // 1/15!
+    ps = fma(ps, r2, -1.0 / 6227020800.0);            // -1/13!
+    ps = fma(ps, r2, 1.0 / 39916800.0);               // 1/11!
+    ps = fma(ps, r2, -1.0 / 362880.0);                // -1/9!
+    ps = fma(ps, r2, 1.0 / 5040.0);                   // 1/7!
+    ps = fma(ps, r2, -1.0 / 120.0);                   // -1/5!
+    ps = fma(ps, r2, 1.0 / 6.0);                      // 1/3! (sign applied below)
+    const double sr = fma(-r * r2, ps, r);
+    double pc = 1.0 / 6402373705728000.0;             // 1/18!
+    pc = fma(pc, r2, -1.0 / 20922789888000.0);        // -1/16!
+    pc = fma(pc, r2, 1.0 / 87178291200.0);            // 1/14!
+    pc = fma(pc, r2, -1.0 / 479001600.0);             // -1/12!
+    pc = fma(pc, r2, 1.0 / 3628800.0);                // 1/10!
+    pc = fma(pc, r2, -1.0 / 40320.0);                 // -1/8!
+    pc = fma(pc, r2, 1.0 / 720.0);                    // 1/6!
+    pc = fma(pc, r2, -1.0 / 24.0);                    // -1/4!
+    pc = fma(pc, r2, 0.5);                            // 1/2! (sign applied below)
+    const double cr = fma(-r2, pc, 1.0);
+    const int k = ((int)q) & 3;
+    const double ss = (k & 1) ? cr : sr;
+    const double cc = (k & 1) ? sr : cr;
+    *s = (k & 2) ? -ss : ss;
+    *c = ((k + 1) & 2) ? -cc : cc;
+}
+
+// sin and cos of any moderate or large argument (reduction modulo 2 pi first)
+__device__ inline void sincos_red(double theta, double* s, double* c) { sincos_small(reduce_2pi(theta), s, c); }
+
 __device__ inline cd cexpi(double phase)
 {
     double s, c;
-    sincos(phase, &s, &c);
+    sincos_red(phase, &s, &c);
     return cd{c, s};
 }
+
+// e^{i theta} for large |theta| (phases of 1e5-1e8 rad)
+__device__ inline cd cexpi_red(double theta) { return cexpi(theta); }
 
 // i * a
 __host__ __device__ inline cd mul_i(cd a) { return cd{-a.im, a.re}; }

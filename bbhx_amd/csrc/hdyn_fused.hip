@@ -116,7 +116,7 @@ __global__ __launch_bounds__(64) void k_fused_eval(double* __restrict__ d_h, dou
             resp::resp_eval(r.s, orb, f, tf, TDItag, rescaled, tdi2, T, dphi);
             phase = phase + dphi;
             if ((tf >= ts) && ((tf <= te) || (te <= 0.0))) {
-                const cd ap = amp * cexpi(phase);
+                const cd ap = amp * cexpi_red(phase);
                 h[0] += conj(T[0] * ap);
                 h[1] += conj(T[1] * ap);
                 h[2] += conj(T[2] * ap);
@@ -125,8 +125,8 @@ __global__ __launch_bounds__(64) void k_fused_eval(double* __restrict__ d_h, dou
         // r = h / h0 (likelihood.py:603)
         for (int c = 0; c < 3; c++) {
             const cd d = h0[c * K + j];
-            const double den = d.re * d.re + d.im * d.im;
-            rs[c * K + j] = mk((h[c].re * d.re + h[c].im * d.im) / den, (h[c].im * d.re - h[c].re * d.im) / den);
+            const double iden = rcp_pos(d.re * d.re + d.im * d.im);
+            rs[c * K + j] = mk((h[c].re * d.re + h[c].im * d.im) * iden, (h[c].im * d.re - h[c].re * d.im) * iden);
         }
     }
     __syncthreads();

@@ -55,7 +55,7 @@ __device__ inline void interp_point(cd h[3], double f, int i, const InterpBin& d
         }
         const double tf = v[2];
         if ((tf >= d.t_start) && ((tf <= d.t_end) || (d.t_end <= 0.0))) {
-            const cd ap = v[0] * cexpi(v[1]);
+            const cd ap = v[0] * cexpi_red(v[1]);
             h[0] += conj(mk(v[3], v[4]) * ap);
             h[1] += conj(mk(v[5], v[6]) * ap);
             h[2] += conj(mk(v[7], v[8]) * ap);

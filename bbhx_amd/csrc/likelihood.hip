@@ -40,7 +40,7 @@ __global__ __launch_bounds__(SUM_THREADS) void k_direct_sum(cd* __restrict__ out
         const long idx = ((long)b * numModes + m) * L + i;
         const double tf = buf[2 * nper + idx];
         if ((tf >= ts) && ((tf <= te) || (te <= 0.0))) {
-            const cd ap = buf[idx] * cexpi(buf[nper + idx]);
+            const cd ap = buf[idx] * cexpi_red(buf[nper + idx]);
             h[0] += conj(mk(buf[3 * nper + idx], buf[4 * nper + idx]) * ap);
             h[1] += conj(mk(buf[5 * nper + idx], buf[6 * nper + idx]) * ap);
             h[2] += conj(mk(buf[7 * nper + idx], buf[8 * nper + idx]) * ap);

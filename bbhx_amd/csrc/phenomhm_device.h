@@ -106,9 +106,9 @@ struct P6 {
 __device__ inline P6 pow6(double x)
 {
     P6 p;
-    p.sixth = pow(x, 1 / 6.0);
+    p.sixth = cbrt_pos(sqrt(x));
     p.third = p.sixth * p.sixth;
-    p.two_thirds = x / p.third;
+    p.two_thirds = p.third * p.third;
     p.four_thirds = x * p.third;
     p.five_thirds = p.four_thirds * p.third;
     p.two = x * x;
@@ -125,6 +125,7 @@ struct PhmRec {
     double M_sec, amp_scale, t0, phi0, Mf_ref;
     // leading-order PN amplitude model (IMRPhenomHMOnePointFiveSpinPN inputs)
     double pn_M, pn_eta, pn_delta, pn_Xs, pn_Xa, pn_pref;
+    double pn_vscale, pn_v2ratio;  // 2 pi M / m and (2/m)^(1/3) of this mode
     // PhenomD amplitude (mode independent)
     double a_fRD, a_fDM, a_g1, a_g2, a_g3, a_fmax;
     double a_rho1, a_rho2, a_rho3, a_d0, a_d1, a_d2, a_d3, a_d4;
@@ -154,9 +155,10 @@ __device__ inline double amp_ins(const PhmRec& r, double Mf, const P6& p)
 
 __device__ inline double amp_mrd(const PhmRec& r, double f)
 {
-    double fDMg3 = r.a_fDM * r.a_g3;
-    double d = f - r.a_fRD;
-    return exp(-(d)*r.a_g2 / (fDMg3)) * (fDMg3 * r.a_g1) / (d * d + fDMg3 * fDMg3);
+    const double fDMg3 = r.a_fDM * r.a_g3;
+    const double d = f - r.a_fRD;
+    const double inv = rcp_pos(fDMg3);
+    return exp(-(d * r.a_g2) * inv) * (fDMg3 * r.a_g1) * rcp_pos(d * d + fDMg3 * fDMg3);
 }
 
 __device__ inline double amp_mrd_deriv(const PhmRec& r, double f)
@@ -171,67 +173,80 @@ __device__ inline double amp_mrd_deriv(const PhmRec& r, double f)
 // IMRPhenDAmplitude without the distance factor (PhenomHMWaveform.cu:1508-1537)
 __device__ inline double amp_imr(const PhmRec& r, double f, const P6& p)
 {
-    double pre = r.a_amp0 / (f * p.sixth);
+    const double pre = r.a_amp0 * rcp_pos(f * p.sixth);
     if (!(f >= AMP_JOIN_INS)) return pre * amp_ins(r, f, p);
     if (f >= r.a_fmax) return pre * amp_mrd(r, f);
     double f2 = f * f, f3 = f * f2, f4 = f * f3;
     return pre * (r.a_d0 + r.a_d1 * f + r.a_d2 * f2 + r.a_d3 * f3 + r.a_d4 * f4);
 }
 
-// Leading-order PN amplitude of the (l, m) multipole, IMRPhenomHMOnePointFiveSpinPN
-// (PhenomHMWaveform.cu:2375-2455), with the mass/spin combinations precomputed.
-__device__ inline double pn_amp(const PhmRec& r, double fM, int l, int m, bool spinless22)
+// |H_lm(v)| of the leading-order PN amplitude (IMRPhenomHMOnePointFiveSpinPN,
+// PhenomHMWaveform.cu:2405-2449) without the common v^-3.5 prefactor.
+__device__ inline double pn_habs(const PhmRec& r, double v)
 {
-    double v = pow(r.pn_M * 2.0 * PI * fM / m, 1.0 / 3.0);
-    double v2 = v * v, v3 = v * v2;
-    double habs;
+    const int l = r.ell, m = r.mm;
     const double eta = r.pn_eta, delta = r.pn_delta;
-    if (spinless22 || (l == 2 && m == 2)) {
-        habs = 1.0;
-    } else if (l == 2 && m == 1) {
-        double v4 = v * v3;
+    if (l == 2 && m == 1) {
+        const double v2 = v * v, v3 = v * v2, v4 = v * v3;
         const double Xs = r.pn_Xs, Xa = r.pn_Xa;
         double re = v * delta - v2 * 1.5 * (Xa + delta * Xs) + v3 * delta * ((335.0 / 672.0) + (eta * 117.0 / 56.0)) +
                     v4 * (Xa * (3427.0 / 1344 - eta * 2101.0 / 336) + delta * Xs * (3427.0 / 1344 - eta * 965 / 336) +
                           delta * (-PI));
         double im = v4 * (delta * (-0.5 - 2.0 * 0.69314718056));
         const double pre = (sqrt(2.0) / 3.0);
-        habs = hypot(pre * re, pre * im);
-    } else if (l == 3 && m == 3) {
-        habs = fabs(0.75 * sqrt(5.0 / 7.0) * (v * delta));
-    } else if (l == 3 && m == 2) {
-        habs = fabs((1.0 / 3.0) * sqrt(5.0 / 7.0) * (v2 * (1.0 - 3.0 * eta)));
-    } else if (l == 4 && m == 4) {
-        habs = fabs((4.0 / 9.0) * sqrt(10.0 / 7.0) * v2 * (1.0 - 3.0 * eta));
-    } else if (l == 4 && m == 3) {
-        habs = fabs(0.75 * sqrt(3.0 / 35.0) * v3 * delta * (1.0 - 2.0 * eta));
-    } else {
-        habs = 0.0;
+        return hypot(pre * re, pre * im);
     }
-    return r.pn_pref * pow(v, -3.5) * habs;
+    if (l == 3 && m == 3) return fabs(0.75 * sqrt(5.0 / 7.0) * (v * delta));
+    if (l == 3 && m == 2) return fabs((1.0 / 3.0) * sqrt(5.0 / 7.0) * (v * v * (1.0 - 3.0 * eta)));
+    if (l == 4 && m == 4) return fabs((4.0 / 9.0) * sqrt(10.0 / 7.0) * v * v * (1.0 - 3.0 * eta));
+    if (l == 4 && m == 3) return fabs(0.75 * sqrt(3.0 / 35.0) * v * v * v * delta * (1.0 - 2.0 * eta));
+    return 1.0;
+}
+
+// The HM amplitude factor beta * HM1 / HM2 of get_amp (PhenomHMWaveform.cu:2473-2511).
+// With v1 = (2 pi M Mf/m)^(1/3), v2 = (2/m)^(1/3) v1, v3 = (2 pi M Mf_amp/m)^(1/3) and
+// v4 = (m/2)^(1/3) v3, the v^-3.5 prefactors of the four PN amplitudes cancel
+// exactly, leaving |H(v1)| |H(v3)| / |H(v2)| (1 for the (2,2) mode; 0 where the
+// reference's beta_term1 vanishes).
+__device__ inline double hm_amp_factor(const PhmRec& r, double Mf_wf, double Mf_amp)
+{
+    if (r.ell == 2 && r.mm == 2) return 1.0;
+    const double v1 = cbrt_pos(r.pn_vscale * Mf_wf);
+    const double h1 = pn_habs(r, v1);
+    if (h1 == 0.0) return 0.0;
+    const double v2 = v1 * r.pn_v2ratio;
+    const double v3 = cbrt_pos(r.pn_vscale * Mf_amp);
+    return h1 * pn_habs(r, v3) / pn_habs(r, v2);
 }
 
 // ---------------------------------------------------------------- phase
 
-__device__ inline double phi_ins(const PhmRec& r, double Mf, const P6& p, double logv)
+// Point evaluations take iMf = 1/Mf (computed once per point): every negative
+// power below is a product of iMf and the positive powers, so the only
+// per-point divisions are that reciprocal and 1/v.
+
+__device__ inline double phi_ins(const PhmRec& r, double Mf, const P6& p, double logv, double iMf)
 {
+    const double im13 = p.two_thirds * iMf;  // Mf^(-1/3)
+    const double im23 = im13 * im13;
     double ph = r.q_init;
     ph += r.q_23 * p.two_thirds;
     ph += r.q_13 * p.third;
     ph += r.q_13log * logv * p.third;
     ph += r.q_logv * logv;
-    ph += r.q_m13 / p.third;
-    ph += r.q_m23 / p.two_thirds;
-    ph += r.q_m1 / Mf;
-    ph += r.q_m53 / p.five_thirds;
-    ph += (r.q_1 * Mf + r.q_43 * p.four_thirds + r.q_53 * p.five_thirds + r.q_2 * p.two) / r.p_eta;
+    ph += r.q_m13 * im13;
+    ph += r.q_m23 * im23;
+    ph += r.q_m1 * iMf;
+    ph += r.q_m53 * (im23 * iMf);
+    ph += (r.q_1 * Mf + r.q_43 * p.four_thirds + r.q_53 * p.five_thirds + r.q_2 * p.two) * r.p_etaInv;
     return ph;
 }
 
 // d phi_ins / dMf with v = (pi Mf)^(1/3) given
 __device__ inline double dphi_ins(const PhmRec& r, double v, double logv)
 {
-    const double v2 = v * v, v3 = v * v2, v4 = v * v3, v5 = v * v4, v6 = v * v5, v7 = v * v6, v8 = v * v7;
+    const double v2 = v * v, v3 = v * v2, v4 = v * v3, v5 = v * v4, v6 = v * v5, v7 = v * v6;
+    const double iv = rcp_pos(v), iv2 = iv * iv, iv4 = iv2 * iv2, iv8 = iv4 * iv4;
     double D = 0.0;
     D += +2.0 * r.pn_v7 * v7;
     D += (r.pn_v6 + r.pn_vl6 * (1.0 + logv)) * v6;
@@ -240,59 +255,62 @@ __device__ inline double dphi_ins(const PhmRec& r, double v, double logv)
     D += -2.0 * r.pn_v3 * v3;
     D += -3.0 * r.pn_v2 * v2;
     D += -5.0 * r.pn_v0;
-    D /= v8 * 3.0 / PI;
-    const double pi13 = 1.4645918875615231, pi23 = 2.1450293971110255;  // pi^(1/3), pi^(2/3)
-    D += (r.sigma1 + r.sigma2 * v / pi13 + r.sigma3 * v2 / pi23 + (r.sigma4 / PI) * v3) / r.p_eta;
+    D *= (PI / 3.0) * iv8;
+    // pi^(-1/3), pi^(-2/3), 1/pi
+    const double ipi13 = 0.68278406325529568, ipi23 = 0.46619407703541166, ipi = 0.31830988618379067;
+    D += (r.sigma1 + r.sigma2 * v * ipi13 + r.sigma3 * v2 * ipi23 + (r.sigma4 * ipi) * v3) * r.p_etaInv;
     return D;
 }
 
-__device__ inline double phi_int(const PhmRec& r, double Mf)
+__device__ inline double phi_int(const PhmRec& r, double Mf, double iMf)
 {
-    return r.beta1 * Mf - r.beta3 / (3. * (Mf * Mf * Mf)) + r.beta2 * log(Mf);
+    return r.beta1 * Mf - r.beta3 * (iMf * iMf * iMf) * (1.0 / 3.0) + r.beta2 * log(Mf);
 }
 
-__device__ inline double dphi_int(const PhmRec& r, double Mf)
+__device__ inline double dphi_int(const PhmRec& r, double iMf)
 {
-    double Mf2 = Mf * Mf;
-    return (r.beta1 + r.beta3 / (Mf2 * Mf2) + r.beta2 / Mf) / r.p_eta;
+    const double im2 = iMf * iMf;
+    return (r.beta1 + r.beta3 * (im2 * im2) + r.beta2 * iMf) * r.p_etaInv;
 }
 
-__device__ inline double phi_mrd(const PhmRec& r, double f)
+__device__ inline double phi_mrd(const PhmRec& r, double f, double iMf)
 {
-    double sf = sqrt(f);
-    double f075 = sqrt(f * sf);
-    return -(r.alpha2 / f) + (4.0 / 3.0) * (r.alpha3 * f075) + r.alpha1 * f +
-           r.alpha4 * r.p_Rholm * atan((f - r.alpha5 * r.p_fRD) / (r.p_Rholm * r.p_fDM * r.p_Taulm));
+    const double sf = sqrt(f);
+    const double f075 = sqrt(f * sf);
+    return -(r.alpha2 * iMf) + (4.0 / 3.0) * (r.alpha3 * f075) + r.alpha1 * f +
+           r.alpha4 * r.p_Rholm * atan((f - r.alpha5 * r.p_fRD) * rcp_pos(r.p_Rholm * r.p_fDM * r.p_Taulm));
 }
 
-__device__ inline double dphi_mrd(const PhmRec& r, double f)
+__device__ inline double dphi_mrd(const PhmRec& r, double f, double iMf)
 {
-    double num = f - r.alpha5 * r.p_fRD;
-    double den = r.p_fDM * r.p_Taulm * r.p_Rholm;
-    return (r.alpha1 + r.alpha2 / (f * f) + r.alpha3 / pow(f, 0.25) +
-            r.alpha4 / (r.p_fDM * r.p_Taulm * (1 + (num * num) / (den * den)))) *
+    const double num = f - r.alpha5 * r.p_fRD;
+    const double q = num * rcp_pos(r.p_fDM * r.p_Taulm * r.p_Rholm);
+    const double f075 = sqrt(f * sqrt(f));  // f^(-1/4) = f^(3/4) / f
+    return (r.alpha1 + r.alpha2 * (iMf * iMf) + r.alpha3 * (f075 * iMf) +
+            r.alpha4 * rcp_pos(r.p_fDM * r.p_Taulm * (1 + q * q))) *
            r.p_etaInv;
 }
 
 // IMRPhenDPhase and its derivative at the same Mf, sharing the powers.
 __device__ inline void phase_imr(const PhmRec& r, double f, double& ph, double* dph)
 {
+    const double iMf = rcp_pos(f);
     if (!(f >= PHI_JOIN_INS)) {
         P6 p = pow6(f);
         const double pi13 = 1.4645918875615231;
         double v = p.third * pi13;
         double logv = log(v);
-        ph = phi_ins(r, f, p, logv);
+        ph = phi_ins(r, f, p, logv, iMf);
         if (dph) *dph = dphi_ins(r, v, logv);
         return;
     }
     if (f >= r.fMRDJoin) {
-        ph = r.p_etaInv * phi_mrd(r, f) + r.C1MRD + r.C2MRD * f;
-        if (dph) *dph = dphi_mrd(r, f) + r.C2MRD;
+        ph = r.p_etaInv * phi_mrd(r, f, iMf) + r.C1MRD + r.C2MRD * f;
+        if (dph) *dph = dphi_mrd(r, f, iMf) + r.C2MRD;
         return;
     }
-    ph = r.p_etaInv * phi_int(r, f) + r.C1Int + r.C2Int * f;
-    if (dph) *dph = dphi_int(r, f) + r.C2Int;
+    ph = r.p_etaInv * phi_int(r, f, iMf) + r.C1Int + r.C2Int * f;
+    if (dph) *dph = dphi_int(r, iMf) + r.C2Int;
 }
 
 __device__ inline double phase_only(const PhmRec& r, double f)
@@ -300,16 +318,6 @@ __device__ inline double phase_only(const PhmRec& r, double f)
     double ph;
     phase_imr(r, f, ph, nullptr);
     return ph;
-}
-
-__device__ inline double dphase_only(const PhmRec& r, double f)
-{
-    if (!(f >= PHI_JOIN_INS)) {
-        double v = cbrt(PI * f);
-        return dphi_ins(r, v, log(v));
-    }
-    if (f >= r.fMRDJoin) return dphi_mrd(r, f) + r.C2MRD;
-    return dphi_int(r, f) + r.C2Int;
 }
 
 // -------------------------------------------------------- final state / QNM
@@ -515,18 +523,21 @@ __device__ inline void setup_phenomd(PhmRec& r, double m1, double m2, double chi
     r.C1Int = 0.0;
     r.C2Int = 0.0;
     {
+        const double ij = 1.0 / PHI_JOIN_INS;
         double vj = cbrt(PI * PHI_JOIN_INS);
         double dins = dphi_ins(r, vj, log(vj));
-        r.C2Int = dins - dphi_int(r, PHI_JOIN_INS);
+        r.C2Int = dins - dphi_int(r, ij);
         P6 pj = pow6(PHI_JOIN_INS);
         double logvj = log(pj.third * 1.4645918875615231);
-        r.C1Int = phi_ins(r, PHI_JOIN_INS, pj, logvj) - r.p_etaInv * phi_int(r, PHI_JOIN_INS) - r.C2Int * PHI_JOIN_INS;
+        r.C1Int = phi_ins(r, PHI_JOIN_INS, pj, logvj, ij) - r.p_etaInv * phi_int(r, PHI_JOIN_INS, ij) -
+                  r.C2Int * PHI_JOIN_INS;
         const double fj = r.fMRDJoin;
-        double phi_int_j = r.p_etaInv * phi_int(r, fj) + r.C1Int + r.C2Int * fj;
+        const double ifj = 1.0 / fj;
+        double phi_int_j = r.p_etaInv * phi_int(r, fj, ifj) + r.C1Int + r.C2Int * fj;
         double fj2 = fj * fj;
         double dphi_int_j = r.C2Int + (r.beta1 + r.beta3 / (fj2 * fj2) + r.beta2 / fj) / r.p_eta;
-        r.C2MRD = dphi_int_j - dphi_mrd(r, fj);
-        r.C1MRD = phi_int_j - r.p_etaInv * phi_mrd(r, fj) - r.C2MRD * fj;
+        r.C2MRD = dphi_int_j - dphi_mrd(r, fj, ifj);
+        r.C1MRD = phi_int_j - r.p_etaInv * phi_mrd(r, fj, ifj) - r.C2MRD * fj;
     }
 
     if (!with_amp) return;
@@ -715,6 +726,8 @@ __device__ inline bool phm_setup(PhmRec& r, double m1_SI, double m2_SI, double c
         r.pn_Xs = 0.5 * (chi1 + chi2);
         r.pn_Xa = 0.5 * (chi1 - chi2);
         r.pn_pref = M * M * PI * sqrt(r.pn_eta * 2.0 / 3);
+        r.pn_vscale = M * 2.0 * PI / (double)r.mm;
+        r.pn_v2ratio = cbrt(2.0 / (double)r.mm);
     }
 
     if (phenomd) {
@@ -762,18 +775,10 @@ __device__ inline bool phm_setup(PhmRec& r, double m1_SI, double m2_SI, double c
 // (calculate_modes / calculate_modes_phenomd, PhenomHMWaveform.cu:2461-2662).
 __device__ inline void phm_eval(const PhmRec& r, double f, double& amp, double& phase, double& tf)
 {
+    // A PhenomD-only record (run_phenomd) carries identity maps (a = 1, b = 0,
+    // breakpoints at 1e300) and l = m = 2, so it takes the same path: every map
+    // and offset below is then exact, and the amplitude factor is 1.
     const double Mf_wf = f * r.M_sec;
-    if (r.phenomd) {
-        P6 p = pow6(Mf_wf);
-        amp = amp_imr(r, Mf_wf, p) * r.amp_scale;
-        double ph, dph;
-        phase_imr(r, Mf_wf, ph, &dph);
-        double phi = r.cshift + ph;
-        phase = -r.t0 * (Mf_wf - r.Mf_ref) + (phi - (2 * r.phi0));
-        double dphidf = r.M_sec * dph / (2 * PI);
-        tf = dphidf - (r.t0 / (2. * PI) * r.M_sec);
-        return;
-    }
 
     // amplitude
     double a, b;
@@ -786,39 +791,25 @@ __device__ inline void phm_eval(const PhmRec& r, double f, double& amp, double& 
     const double Mf_amp = a * Mf_wf + b;
     P6 p = pow6(Mf_amp);
     double A = amp_imr(r, Mf_amp, p);
-    double beta1 = pn_amp(r, Mf_wf, r.ell, r.mm, false);
-    double beta = 0.0, hm1 = 1.0, hm2 = 1.0;
-    if (beta1 != 0.) {
-        double beta2 = pn_amp(r, 2.0 * Mf_wf / r.mm, r.ell, r.mm, false);
-        beta = beta1 / beta2;
-        hm1 = pn_amp(r, Mf_amp, r.ell, r.mm, false);
-        hm2 = pn_amp(r, Mf_amp, 2, 2, true);
-    }
-    A *= beta * hm1 / hm2;
+    A *= hm_amp_factor(r, Mf_wf, Mf_amp);
     amp = A * r.amp_scale;
 
-    // phase and t(f)
-    double phi = r.cshift;
-    double dph = 0.0;
+    // phase and t(f): one of three linear frequency maps, then one PhenomD phase
+    // evaluation; phi = ph / a - c1 + c2 with c1 = c2 = 0 on the first piece
+    double sa, sb, c1, c2;
     if (!(Mf_wf > r.fi)) {
-        double Mf = r.ai * Mf_wf + r.bi;
-        double ph;
-        phase_imr(r, Mf, ph, &dph);
-        phi += ph / r.ai;
+        sa = r.ai; sb = r.bi; c1 = 0.0; c2 = 0.0;
     } else if (!(Mf_wf > r.fr)) {
-        double Mf = r.am * Mf_wf + r.bm;
-        double ph;
-        phase_imr(r, Mf, ph, &dph);
-        phi += ph / r.am - r.PhDBconst + r.PhDBAterm;
+        sa = r.am; sb = r.bm; c1 = r.PhDBconst; c2 = r.PhDBAterm;
     } else {
-        double Mf = r.ar * Mf_wf + r.br;
-        double ph;
-        phase_imr(r, Mf, ph, &dph);
-        phi += ph / r.ar - r.PhDCconst + r.phaseC;
+        sa = r.ar; sb = r.br; c1 = r.PhDCconst; c2 = r.phaseC;
     }
+    double ph, dph;
+    phase_imr(r, sa * Mf_wf + sb, ph, &dph);
+    const double phi = r.cshift + ((ph * rcp_pos(sa) - c1) + c2);
     phase = -r.t0 * (Mf_wf - r.Mf_ref) + (phi - (r.mm * r.phi0));
-    double tf_i = dph / (2. * PI);
-    tf = (r.M_sec * tf_i) - (r.t0 / (2. * PI) * r.M_sec);
+    const double INV_TWO_PI = 0.15915494309189535;
+    tf = (r.M_sec * (dph * INV_TWO_PI)) - ((r.t0 * INV_TWO_PI) * r.M_sec);
 }
 
 }  // namespace phm

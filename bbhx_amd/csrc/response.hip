@@ -85,6 +85,7 @@ bool response_orbit_tab(const bbhx_response* r, OrbitTab& o)
     o.li23 = r->li23;
     o.li31 = r->li31;
     o.L = r->armlength;
+    o.inv_L = 1.0 / r->armlength;
     return true;
 }
 }  // namespace bbhx

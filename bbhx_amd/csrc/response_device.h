@@ -5,17 +5,24 @@
 // vectors come from orbit tables in the lisaanalysistools Orbits layout
 // (x: (N, 3, 3), n: (N, 6, 3), uniform step dt), linearly interpolated in time.
 //
-// Per (binary, mode) the polarisation matrix H and the wave vector are built
-// once (resp_setup); per frequency point resp_eval returns the three complex
-// transfer functions and the orbital Doppler phase.
+// Per (binary, mode) resp_setup builds the polarisation basis and the
+// spherical-harmonic factors once; per frequency point resp_eval returns the
+// three complex transfer functions and the orbital Doppler phase.
 //
-// Exact algebraic simplifications relative to the reference's operation order:
+// Exact rewrites relative to the reference's operation order (they change
+// results only at the level of FP64 rounding):
+//  * H = Y+ H+ + Yx Hx with H+ = u u^T - v v^T, Hx = u v^T + v u^T (u, v the
+//    first two columns of the source-frame rotation, Response.cu:540-567), so
+//    n^T H n = Y+ ((u.n)^2 - (v.n)^2) + 2 Yx (u.n)(v.n): no 3x3 complex matrix.
+//  * linear interpolation commutes with dot products: k.p_i, k.n, u.n, v.n are
+//    interpolated directly from their values at the two table nodes.
 //  * the guiding-centre factor exp(i 2 pi f k.p0 / c) (Response.cu:166) and the
-//    rotation exp(-i phaseRdelay) (Response.cu:304-309) cancel identically; both
-//    are skipped.
-//  * all functions of x = pi f L / c (z = e^{2ix}, sin 2x, e^{3ix}, sin 4x,
-//    e^{4ix}) are built from one sincos(x) by multiple-angle identities.
-// Both change results only at the level of FP64 rounding.
+//    rotation exp(-i phaseRdelay) (Response.cu:304-309) cancel identically.
+//  * divisions by L and by x (1 +- kn) become products with reciprocals.
+//  * sin(x(1 +- kn)) from sincos(x) and sincos(x kn) by angle addition; every
+//    function of x = pi f L / c from one sincos(x) by multiple-angle identities.
+//  * arguments of e^{i theta} with large |theta| are reduced modulo 2 pi in
+//    two steps (Cody-Waite) before sincos.
 #pragma once
 
 #include "common.h"
@@ -24,8 +31,9 @@ namespace bbhx {
 namespace resp {
 
 struct RespRec {
-    cd H[9];      // Yfactor+ H+ + Yfactorx Hx (Response.cu:580-601)
-    double k[3];  // wave propagation vector
+    double u[3], v[3];  // polarisation basis in the SSB frame
+    double k[3];        // wave propagation vector
+    cd Yp, Yc;          // (Ylm + (-1)^l Yl-m*)/2 and i (Ylm - (-1)^l Yl-m*)/2
 };
 
 struct OrbitTab {
@@ -35,6 +43,7 @@ struct OrbitTab {
     int N;
     int li12, li23, li31;  // row of links 12, 23, 31 in n
     double L;              // armlength [m]
+    double inv_L;          // 1 / L
 };
 
 // spin-weight -2 spherical harmonic for the PhenomHM (l, m) (Response.cu:383-424)
@@ -58,7 +67,8 @@ __device__ inline cd sYlm(int l, int m, double theta, double phi)
     return fac * cexpi(m * phi);
 }
 
-// H matrix and wave vector for one (binary, mode) (Response.cu:501-601)
+// Polarisation basis, wave vector and Ylm factors of one (binary, mode)
+// (Response.cu:501-601).
 __device__ inline void resp_setup(RespRec& r, int l, int m, double phi_ref, double inc, double lam, double beta,
                                   double psi)
 {
@@ -66,63 +76,49 @@ __device__ inline void resp_setup(RespRec& r, int l, int m, double phi_ref, doub
     r.k[0] = -cb * cl;
     r.k[1] = -cb * sl;
     r.k[2] = -sb;
-    // O1 and its inverse (transpose)
-    const double O1[9] = {cp * sl - cl * sb * sp, -cl * cp * sb - sl * sp, -cb * cl,
-                          -cl * cp - sb * sl * sp, -cp * sb * sl + cl * sp, -cb * sl,
-                          cb * sp, cb * cp, -sb};
-    // H+ = O1 diag(1,-1,0) O1^T, Hx = O1 offdiag(1,1) O1^T
-    double Hp[9], Hc[9];
-    for (int i = 0; i < 3; i++)
-        for (int j = 0; j < 3; j++) {
-            double u_i = O1[i * 3 + 0], v_i = O1[i * 3 + 1];
-            double u_j = O1[j * 3 + 0], v_j = O1[j * 3 + 1];
-            Hp[i * 3 + j] = u_i * u_j - v_i * v_j;
-            Hc[i * 3 + j] = u_i * v_j + v_i * u_j;
-        }
+    // first two columns of O1
+    r.u[0] = cp * sl - cl * sb * sp;
+    r.u[1] = -cl * cp - sb * sl * sp;
+    r.u[2] = cb * sp;
+    r.v[0] = -cl * cp * sb - sl * sp;
+    r.v[1] = -cp * sb * sl + cl * sp;
+    r.v[2] = cb * cp;
     cd Ylm = sYlm(l, m, inc, phi_ref);
     cd Yl_m = conj(sYlm(l, -m, inc, phi_ref));
     if (l & 1) Yl_m = mk(-Yl_m.re, -Yl_m.im);
-    cd Yp = 0.5 * (Ylm + Yl_m);
-    cd Yc = mul_i(0.5 * (Ylm - Yl_m));
-    for (int q = 0; q < 9; q++) r.H[q] = Yp * Hp[q] + Yc * Hc[q];
+    r.Yp = 0.5 * (Ylm + Yl_m);
+    r.Yc = mul_i(0.5 * (Ylm - Yl_m));
 }
 
 __device__ inline double dot3(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
 
-__device__ inline cd nHn(const double* n, const cd* H)
-{
-    cd out = mk(0.0, 0.0);
-    for (int i = 0; i < 3; i++) {
-        cd t = H[i * 3 + 0] * n[0] + H[i * 3 + 1] * n[1] + H[i * 3 + 2] * n[2];
-        out += n[i] * t;
-    }
-    return out;
-}
-
-__device__ inline double sinc(double x) { return x == 0.0 ? 1.0 : sin(x) / x; }
-
-// Interpolated orbit quantities at time t: p1+p2, p2+p3, p3+p1 and n12, n23, n31.
-__device__ inline void orbit_eval(const OrbitTab& o, double t, double s12[3], double s23[3], double s31[3],
-                                  double n1[3], double n2[3], double n3[3])
+// Orbit quantities at time t that the response needs:
+//   kp[i] = k . p_i (i = spacecraft 1..3), and for links 12, 23, 31:
+//   kn[j] = k . n_j, un[j] = u . n_j, vn[j] = v . n_j
+__device__ inline void orbit_eval(const OrbitTab& o, const RespRec& r, double t, double kp[3], double kn[3],
+                                  double un[3], double vn[3])
 {
     int w = int(floor(t * o.inv_dt));
     w = w < 0 ? 0 : (w > o.N - 2 ? o.N - 2 : w);
-    const double rel = t - o.dt * w;
+    const double a = (t - o.dt * w) * o.inv_dt;  // interpolation weight in [0, 1) inside the table
     const double* x0 = o.x + (long)w * 9;
-    const double* x1 = x0 + 9;
-    double p[9];
-    for (int q = 0; q < 9; q++) p[q] = ((x1[q] - x0[q]) * o.inv_dt) * rel + x0[q];
-    for (int k = 0; k < 3; k++) {
-        s12[k] = p[0 * 3 + k] + p[1 * 3 + k];
-        s23[k] = p[1 * 3 + k] + p[2 * 3 + k];
-        s31[k] = p[2 * 3 + k] + p[0 * 3 + k];
+    for (int i = 0; i < 3; i++) {
+        const double lo = dot3(r.k, x0 + 3 * i), hi = dot3(r.k, x0 + 9 + 3 * i);
+        kp[i] = fma(hi - lo, a, lo);
     }
     const double* n0 = o.n + (long)w * 18;
-    const double* nn = n0 + 18;
-    for (int k = 0; k < 3; k++) {
-        n1[k] = ((nn[o.li12 * 3 + k] - n0[o.li12 * 3 + k]) * o.inv_dt) * rel + n0[o.li12 * 3 + k];
-        n2[k] = ((nn[o.li23 * 3 + k] - n0[o.li23 * 3 + k]) * o.inv_dt) * rel + n0[o.li23 * 3 + k];
-        n3[k] = ((nn[o.li31 * 3 + k] - n0[o.li31 * 3 + k]) * o.inv_dt) * rel + n0[o.li31 * 3 + k];
+    const int li[3] = {o.li12, o.li23, o.li31};
+    for (int j = 0; j < 3; j++) {
+        const double* lo = n0 + 3 * li[j];
+        const double* hi = lo + 18;
+        double a0 = dot3(r.k, lo), a1 = dot3(r.k, hi);
+        kn[j] = fma(a1 - a0, a, a0);
+        a0 = dot3(r.u, lo);
+        a1 = dot3(r.u, hi);
+        un[j] = fma(a1 - a0, a, a0);
+        a0 = dot3(r.v, lo);
+        a1 = dot3(r.v, hi);
+        vn[j] = fma(a1 - a0, a, a0);
     }
 }
 
@@ -131,35 +127,50 @@ __device__ inline void orbit_eval(const OrbitTab& o, double t, double s12[3], do
 __device__ inline void resp_eval(const RespRec& r, const OrbitTab& o, double f, double t, int TDItag, int rescaled,
                                  int tdi2, cd out[3], double& phaseRdelay)
 {
-    double s12[3], s23[3], s31[3], n1[3], n2[3], n3[3];
-    orbit_eval(o, t, s12, s23, s31, n1, n2, n3);
-    const double* k = r.k;
+    double kp[3], kn[3], un[3], vn[3];
+    orbit_eval(o, r, t, kp, kn, un, vn);
     double sa, ca;
-    sincos(OMEGA0 * t, &sa, &ca);
-    const double kR = k[0] * (AU_SI * ca) + k[1] * (AU_SI * sa);
+    sincos_red(OMEGA0 * t, &sa, &ca);
+    const double kR = r.k[0] * (AU_SI * ca) + r.k[1] * (AU_SI * sa);
     phaseRdelay = 2. * PI / C_SI * f * kR;
 
-    const double kn1 = dot3(k, n1), kn2 = dot3(k, n2), kn3 = dot3(k, n3);
-    const cd n1Hn1 = nHn(n1, r.H), n2Hn2 = nHn(n2, r.H), n3Hn3 = nHn(n3, r.H);
     const double L = o.L;
-    const double pre = PI * f * L / C_SI;
-    // exp(i pre (1 + k.(pi+pj)/L)) (Response.cu:171-173)
-    const cd e12 = cexpi(pre * (1. + dot3(k, s12) / L));
-    const cd e23 = cexpi(pre * (1. + dot3(k, s23) / L));
-    const cd e31 = cexpi(pre * (1. + dot3(k, s31) / L));
-    // common factor i * pre
-    const cd g3 = mul_i(pre * n3Hn3), g1 = mul_i(pre * n1Hn1), g2 = mul_i(pre * n2Hn2);
-    const cd G12 = (g3 * sinc(pre * (1. - kn3))) * e12;
-    const cd G21 = (g3 * sinc(pre * (1. + kn3))) * e12;
-    const cd G23 = (g1 * sinc(pre * (1. - kn1))) * e23;
-    const cd G32 = (g1 * sinc(pre * (1. + kn1))) * e23;
-    const cd G31 = (g2 * sinc(pre * (1. - kn2))) * e31;
-    const cd G13 = (g2 * sinc(pre * (1. + kn2))) * e31;
+    const double x = PI * f * L / C_SI;  // = the reference's 'prefactor'
+    double sx, cx;
+    sincos_red(x, &sx, &cx);
+
+    // link j = 0, 1, 2 is n1 (12), n2 (23), n3 (31); n_j^T H n_j
+    cd nHn[3];
+    double sp[3], sm[3];  // sinc(x (1 - kn)), sinc(x (1 + kn))
+    for (int j = 0; j < 3; j++) {
+        nHn[j] = r.Yp * (un[j] * un[j] - vn[j] * vn[j]) + r.Yc * (2.0 * un[j] * vn[j]);
+        double s, c;
+        sincos_red(x * kn[j], &s, &c);
+        // 1/am and 1/ap from one reciprocal of their product
+        const double am = x * (1. - kn[j]), ap = x * (1. + kn[j]);
+        const double prod = am * ap;
+        if (prod != 0.0) {
+            const double iprod = rcp_pos(prod);
+            sm[j] = (sx * c - cx * s) * (ap * iprod);
+            sp[j] = (sx * c + cx * s) * (am * iprod);
+        } else {  // f = 0 or a link along k: sinc(0) = 1
+            sm[j] = am == 0.0 ? 1.0 : (sx * c - cx * s) / am;
+            sp[j] = ap == 0.0 ? 1.0 : (sx * c + cx * s) / ap;
+        }
+    }
+    // exp(i x (1 + k.(p_i + p_j)/L)) for the pairs 12, 23, 31 (Response.cu:171-173)
+    const cd e12 = cexpi_red(x * (1. + (kp[0] + kp[1]) * o.inv_L));
+    const cd e23 = cexpi_red(x * (1. + (kp[1] + kp[2]) * o.inv_L));
+    const cd e31 = cexpi_red(x * (1. + (kp[2] + kp[0]) * o.inv_L));
+    const cd g3 = mul_i(x * nHn[2]), g1 = mul_i(x * nHn[0]), g2 = mul_i(x * nHn[1]);
+    const cd G12 = (g3 * sm[2]) * e12;
+    const cd G21 = (g3 * sp[2]) * e12;
+    const cd G23 = (g1 * sm[0]) * e23;
+    const cd G32 = (g1 * sp[0]) * e23;
+    const cd G31 = (g2 * sm[1]) * e31;
+    const cd G13 = (g2 * sp[1]) * e31;
 
     // TDI combinations (Response.cu:197-261)
-    const double x = pre;
-    double sx, cx;
-    sincos(x, &sx, &cx);
     const double s2x = 2.0 * sx * cx, c2x = cx * cx - sx * sx;
     const cd z = mk(c2x, s2x);
     if (TDItag == 1) {

@@ -64,8 +64,10 @@ def build_problem(batch, nodes, dense_bits, seed=1000):
     return dict(ref=ref, templates=templates, f=f, data=data, psd=psd, het=het, wave=wave, orbits=orbits)
 
 
-def cpu_baseline(prob, sample, nodes):
-    """The FP64 CPU oracle (OpenMP over templates on the host cores) on a bounded sample."""
+def cpu_baseline(prob, sample, nodes, min_seconds=10.0):
+    """The FP64 CPU oracle (OpenMP over templates on the host cores) on a bounded
+    sample: the first `sample` templates of the batch, repeated for at least
+    `min_seconds` of CPU time."""
     from bbhx_amd.utils.sensitivity import AET1SensitivityMatrix
     from oracle.pipeline import OracleHeterodyne, OracleWaveform
     from tests.params import MODES
@@ -82,11 +84,11 @@ def cpu_baseline(prob, sample, nodes):
     while True:
         ll = ohet.get_ll(p)[0]
         reps += 1
-        if time.perf_counter() - t0 > 10.0 or reps >= 5:
+        if time.perf_counter() - t0 > min_seconds:
             break
     dt = (time.perf_counter() - t0) / reps
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return dict(value=sample / dt, ll=ll, cores=cores, seconds_per_batch=dt)
+    return dict(value=sample / dt, ll=ll, cores=cores, seconds_per_batch=dt, reps=reps)
 
 
 def main():
@@ -97,17 +99,29 @@ def main():
     ap.add_argument("--batch", type=int, default=4096)
     ap.add_argument("--nodes", type=int, default=128)
     ap.add_argument("--dense-bits", type=int, default=20)
-    ap.add_argument("--cpu-sample", type=int, default=256)
+    ap.add_argument("--cpu-sample", type=int, default=4096)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one 256 MiB device-to-device copy (reference kernel for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); the template batch shards by template "
-              "with no inter-GPU exchange, see DESIGN.md", file=sys.stderr)
+              "with no inter-GPU exchange, so N GPUs would each run this on B/N templates", file=sys.stderr)
         sys.exit(2)
 
     import torch
 
     from bbhx_amd.cutils import timing_enable, timing_read
+
+    if args.copy_calibration:
+        src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy calibration: 268435456 bytes read, 268435456 bytes written")
+        del src, dst
 
     log("building problem")
     prob = build_problem(args.batch, args.nodes, args.dense_bits)
@@ -148,27 +162,31 @@ def main():
     log(f"plain FP64 path: {plain_rate:.4g} evals/s; max |ll_fused - ll_plain| = "
         f"{np.max(np.abs(ll_fused - ll_plain)):.3e}")
 
-    # algorithmic FP64 flops per launch of the fused evaluation kernel
+    # algorithmic FP64 flops per launch of the fused evaluation kernel: the
+    # oracle's per-node operation count per template (counted on this workload) x templates
     flop_model = json.loads((ROOT / "bbhx_amd" / "flop_model.json").read_text())
-    M = 6
-    flops = flop_model["fused_eval_flops_per_mode_node"] * B * M * args.nodes
+    flops = flop_model["eval_flops_per_template"] * B
     eval_s = (ms_eval / max(ncalls, 1)) * 1e-3
     achieved = flops / eval_s / 1e12 if eval_s > 0 else None
-    roofline = {"bound": "mfma", "compute": "FP64 VALU (no dense contraction; no MFMA)", "achieved": achieved,
-                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": (achieved / FP64_PEAK_TFLOPS) if achieved else None, "traffic": None,
-                "kernel": "k_fused_eval", "kernel_ms": ms_eval / max(ncalls, 1),
-                "flops_per_launch": flops}
+    traffic = None
+    tfile = ROOT / "profiles" / "k_fused_eval.hdyn_4096x128.traffic.json"
+    if tfile.exists() and B == 4096 and args.nodes == 128:
+        traffic = json.loads(tfile.read_text())["hbm_bytes_per_launch"]
+    roofline = {"bound": "mfma", "compute": "FP64 VALU (no dense contraction, no MFMA; peak = FP64 vector peak)",
+                "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": (achieved / FP64_PEAK_TFLOPS) if achieved else None, "traffic": traffic,
+                "kernel": "k_fused_eval", "kernel_ms": ms_eval / max(ncalls, 1), "flops_per_launch": flops,
+                "flop_rule": flop_model["counting_rules"]}
 
     cpu = None
     dll = None
     if not args.no_cpu:
         log("timing the CPU oracle")
-        c = cpu_baseline(prob, min(args.cpu_sample, B), args.nodes)
+        c = cpu_baseline(prob, min(args.cpu_sample, B), args.nodes, args.cpu_seconds)
         dll = float(np.max(np.abs(ll_fused[:len(c["ll"])] - c["ll"])))
         cpu = {"value": c["value"], "unit": "logL evals/s", "cores": c["cores"], "kind": "port",
                "sample": f"FP64 CPU oracle (OpenMP over templates), first {len(c['ll'])} templates of the batch, "
-                         f"{c['seconds_per_batch']:.2f} s per pass"}
+                         f"{c['reps']} passes of {c['seconds_per_batch']:.3f} s"}
         log(f"cpu oracle: {c['value']:.4g} evals/s on {c['cores']} threads; max |dlogL| vs GPU = {dll:.3e}")
 
     out = {

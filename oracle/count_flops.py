@@ -1,0 +1,156 @@
+"""Counts the algorithmic floating-point work of the heterodyned-likelihood
+forward path and writes it to bbhx_amd/flop_model.json.
+
+TEST INFRASTRUCTURE ONLY. Runs the oracle's scalar code on an operation-counting
+number type (opcount.h / opcount.cpp, built as oracle/libopcount.so) for a
+sample of templates of the heterodyned-likelihood workload: reference binary =
+first draw of seed 1000, templates = its N(0, 1e-3 x prior range)
+perturbations (seed 1001), K sparse nodes, 2^20 dense bins at df = 1/yr, six
+modes, TDI-2 A/E/T. The chain counted per template is the one
+HeterodynedLikelihood.get_ll runs (likelihood.py:550-648): ringdown
+frequencies, PhenomHM setup and evaluation on the K nodes, time/phase shift,
+TDI response, mode sum, r = h/h0, heterodyned sums.
+
+Counting rules: add, sub, mul, div, sqrt = 1 flop each; each libm call
+(sin, cos, exp, log, pow, cbrt, atan, atan2, hypot) = 1 flop plus a separate
+per-function tally.
+
+The per-template work splits into a part independent of the node count
+(ringdown frequencies and the per-mode coefficient setup) and a part linear in
+it; the two are separated by counting at K and at 2K nodes (same band, same
+log spacing, so the same mix of frequency regions).
+
+    python oracle/count_flops.py [--sample 64] [--nodes 128]
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import sys
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent.parent
+sys.path.insert(0, str(ROOT))
+
+LIBM = ("sin", "cos", "exp", "log", "pow", "cbrt", "atan", "atan2", "hypot")
+ARITH = ("add", "mul", "div", "sqrt")
+
+
+def _lib():
+    path = ROOT / "oracle" / "libopcount.so"
+    if not path.exists():
+        raise FileNotFoundError("build it first: make -C oracle")
+    lib = ctypes.CDLL(str(path))
+    lib.opc_op_names.restype = ctypes.c_char_p
+    return lib
+
+
+def build_workload(nodes, dense_bits, seed=1000):
+    from bbhx_amd.utils.constants import YRSID_SI
+    from bbhx_amd.utils.orbits import EqualArmlengthOrbits
+    from bbhx_amd.utils.sensitivity import AET1SensitivityMatrix
+    from oracle.pipeline import OracleHeterodyne, OracleWaveform
+    from tests.params import MODES, draw, perturb
+
+    ref = draw(16, seed)[:, 0]
+    f = np.arange(1, 2**dense_bits + 1) / YRSID_SI
+    orbit = EqualArmlengthOrbits(dt=1e4, t_max=3.0 * YRSID_SI).as_host_dict()
+    wave = OracleWaveform(orbit, tdi2=True)
+    psd_fn = lambda x: [AET1SensitivityMatrix(x)[i] for i in range(3)]  # noqa: E731
+    inj = wave(*ref, freqs=f, length=1024, modes=MODES, direct=False, fill=True, squeeze=True)[0]
+    het = OracleHeterodyne(wave, f, inj, ref, nodes, psd_fn, reference_length=8096, modes=MODES)
+    return ref, orbit, wave, het, perturb
+
+
+def count(K, sample, dense_bits):
+    """Operation counts summed over `sample` templates at K nodes, and the
+    largest relative logL difference of the counted run against the oracle."""
+    from oracle import _phenomd_rd_args, qnm_table
+    from oracle.pipeline import tSSBfromLframe
+
+    ref, orbit, wave, het, perturb = build_workload(K, dense_bits)
+    p = perturb(ref, sample, 1001, scale=1e-3)
+    B = p.shape[1]
+    rows = np.ascontiguousarray(np.stack([p[0], p[1], p[2], p[3], p[4], p[5], p[7], p[8], p[9], p[10], p[11]]))
+    YR = 31558149.763545603
+    t_start = np.ascontiguousarray(tSSBfromLframe(0.0, p[8], p[9], 0.0) * np.ones(B))
+    t_end = np.ascontiguousarray(tSSBfromLframe(1.0 * YR, p[8], p[9], 0.0) * np.ones(B))
+    tab = qnm_table()
+    arrs = _phenomd_rd_args(tab)
+    qnm = np.ascontiguousarray(np.concatenate(arrs))
+    table_len = len(arrs[0])
+    h0 = np.ascontiguousarray(het.h0_sparse[0]).astype(np.complex128)
+    dc = np.ascontiguousarray(het.data_constants).astype(np.complex128)
+    freqs = np.ascontiguousarray(het.freqs, dtype=np.float64)
+    links = np.ascontiguousarray(orbit["links"], dtype=np.int32)
+    n_arr = np.ascontiguousarray(orbit["n_arr"], dtype=np.float64)
+    x_arr = np.ascontiguousarray(orbit["x_arr"], dtype=np.float64)
+
+    lib = _lib()
+    nops = lib.opc_num_ops()
+    names = lib.opc_op_names().decode().split(",")
+    counts = np.zeros(nops)
+    ll = np.zeros(B)
+    D = ctypes.c_double
+    P = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+    lib.opc_hdyn_forward(P(counts), P(ll), P(rows), B, P(freqs), K, P(h0), P(dc), P(qnm), table_len, D(tab["dspin"]),
+                         P(t_start), P(t_end), D(0.0), D(orbit["dt"]), int(orbit["N"]), P(n_arr), P(x_arr), P(links),
+                         D(orbit["armlength"]), 1)
+
+    # the counted run must be the oracle's own computation
+    ll_oracle = het.get_ll(p)[0] + 0.5 * het.reference_d_d
+    err = float(np.max(np.abs(ll - ll_oracle) / np.maximum(1.0, np.abs(ll_oracle))))
+    if err > 1e-9:
+        raise RuntimeError(f"counted run differs from the oracle: {err}")
+
+    return counts, err, B, names
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--sample", type=int, default=64)
+    ap.add_argument("--nodes", type=int, default=128)
+    ap.add_argument("--dense-bits", type=int, default=20)
+    ap.add_argument("--out", default=str(ROOT / "bbhx_amd" / "flop_model.json"))
+    args = ap.parse_args()
+
+    res = {}
+    for K in (args.nodes, 2 * args.nodes):
+        res[K] = count(K, args.sample, args.dense_bits)
+    K = args.nodes
+    counts, err, B, names = res[K]
+    counts2 = res[2 * K][0]
+    per_t = {n: counts[i] / B for i, n in enumerate(names)}
+    tot = lambda c: sum(c[i] for i, n in enumerate(names) if n in ARITH + LIBM) / B  # noqa: E731
+    libm_of = lambda c: sum(c[i] for i, n in enumerate(names) if n in LIBM) / B  # noqa: E731
+    total_k, total_2k = tot(counts), tot(counts2)
+    setup = 2 * total_k - total_2k
+    evalk = total_k - setup
+    libm_setup = 2 * libm_of(counts) - libm_of(counts2)
+    M = 6
+    model = {
+        "workload": f"heterodyned logL forward, {M} modes, K = {K} nodes, TDI-2 AET, "
+                    f"sample of {B} templates of the 1e-3 perturbation batch",
+        "counting_rules": "add/sub/mul/div/sqrt = 1 flop; each libm call = 1 flop plus a per-function tally",
+        "flops_per_template": total_k,
+        "setup_flops_per_template": setup,
+        "eval_flops_per_template": evalk,
+        "libm_calls_per_template": libm_of(counts),
+        "eval_libm_calls_per_template": libm_of(counts) - libm_setup,
+        "per_template_counts": per_t,
+        "eval_flops_per_mode_node": evalk / (M * K),
+        "modes": M,
+        "nodes": K,
+        "generated_by": "python oracle/count_flops.py",
+        "max_rel_ll_diff_vs_oracle": max(err, res[2 * K][1]),
+    }
+    Path(args.out).write_text(json.dumps(model, indent=1) + "\n")
+    print(json.dumps(model, indent=1))
+
+
+if __name__ == "__main__":
+    main()
