@@ -1,0 +1,624 @@
+// Fused direct log-likelihood of a ragged template batch on gfx950 (no
+// counterpart entry point in the reference; it computes what
+// Likelihood.get_ll does, likelihood.py:140-225, for templates whose sparse
+// frequency grids have different lengths, in one launch sequence with no host
+// synchronisation, so a caller can capture it in a hipGraph).
+//
+// Per binary b with sparse length L_b (layout: points of all binaries
+// concatenated, offset off_b; within a binary [i][mode] for the grid and
+// [i][mode][quantity] for the nine spline quantities):
+//   k_dir_setup   one lane per (binary, mode): coefficient records
+//   k_dir_grid    one wavefront per (binary, mode): default grid
+//                 Mf in [mf_min, mf_max] * m / 2, t(f) there, the crossings of
+//                 t = 0 and t = Tobs, the grid refitted to that window
+//                 (phenomhm.py:208-224, :509-614), then amplitude, phase, t(f)
+//                 and the TDI response on the refitted grid (phenomhm.py:487-507,
+//                 Response.cu)
+//   k_dir_spline  one wavefront per binary, one lane per (mode, quantity):
+//                 not-a-knot spline coefficients (Interpolate.cu:15-359)
+//   k_dir_band    dense band [st, et) of each binary (waveformbuild.py:139-156)
+//                 and the exclusive prefix of its chunks of dense bins
+//   k_dir_like    persistent: workgroups loop over the (binary, chunk) work
+//                 items; each thread evaluates dense bins (spline segment per
+//                 mode, WaveformBuild.cu:80-143, window and mode sum :48-60)
+//                 and accumulates the noise-weighted inner products
+//                 (Likelihood.cu:357-471); one partial sum per work item
+//   k_dir_reduce  per binary, the partials in chunk order (deterministic)
+
+#include <vector>
+
+#include "bbhx_hip.h"
+#include "common.h"
+#include "fused_device.h"
+
+using namespace bbhx;
+using namespace bbhx::fused;
+
+namespace bbhx {
+bool response_orbit_tab(const bbhx_response* r, resp::OrbitTab& o);
+}
+
+struct bbhx_direct_plan {
+    int device = -1;
+    int B = 0, M = 0, N = 0, Lmax = 0;
+    int chunk = 0, block = 0, grid = 0;
+    long S = 0;          // total sparse points
+    long max_items = 0;  // bound on (binary, chunk) work items
+    // device buffers
+    long* off = nullptr;   // (B,) first point of each binary
+    int* len = nullptr;    // (B,)
+    FusedRec* recs = nullptr;
+    double* X = nullptr;   // (S, M)
+    double* Y = nullptr;   // (S, M, 9)
+    double* C1 = nullptr;
+    double* C2 = nullptr;
+    double* C3 = nullptr;
+    double* grid_lo = nullptr;    // (B, M) log10 of the first refitted grid point
+    double* grid_step = nullptr;  // (B, M) log10 step
+    int* band = nullptr;          // (B, 3): st, et, chunk_begin
+    int* total = nullptr;         // (1,) number of work items
+    double* partial = nullptr;    // (max_items, 4)
+};
+
+namespace {
+
+constexpr int NQ = 9;
+
+__global__ void k_dir_setup(FusedRec* __restrict__ recs, const double* __restrict__ params, int B, ModeList modes,
+                            int M, const double* __restrict__ qnm, int table_len, double dspin)
+{
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= B * M) return;
+    const int b = idx / M, mi = idx % M;
+    FusedRec r;
+    setup_one(r, params, B, b, modes.ell[mi], modes.mm[mi], qnm, table_len, dspin);
+    recs[idx] = r;
+}
+
+// numpy.logspace point i of num between 10^lo and 10^hi (start + i step, the
+// last point exactly 10^hi)
+__device__ inline double logspace_pt(double lo, double step, double hi, int i, int num)
+{
+    return pow(10.0, i == num - 1 ? hi : lo + i * step);
+}
+
+__device__ inline int wsum_i(int v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ inline int wmax_i(int v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ inline double wmin_d(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ inline double wmax_d(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// One 64-lane wavefront per (binary, mode). LDS: t(f) of the default grid (L_b doubles).
+__global__ __launch_bounds__(64) void k_dir_grid(double* __restrict__ X, double* __restrict__ Y,
+                                                 double* __restrict__ grid_lo, double* __restrict__ grid_step,
+                                                 int* __restrict__ flags, const FusedRec* __restrict__ recs,
+                                                 const long* __restrict__ off, const int* __restrict__ len,
+                                                 const double* __restrict__ params,
+                                                 const double* __restrict__ t_start,
+                                                 const double* __restrict__ t_end, int B, int M, ModeList modes,
+                                                 resp::OrbitTab orb, int TDItag, int rescaled, int tdi2,
+                                                 double initial_t_val, double mf_min, double mf_max)
+{
+    extern __shared__ double tfs[];
+    const int bm = blockIdx.x;
+    const int b = bm / M, m = bm % M;
+    const int lane = threadIdx.x;
+    const int L = len[b];
+    const long base = off[b];
+    const FusedRec& r = recs[bm];
+    const double t_ref = params[P_TREF * B + b];
+    const double Tobs = t_end[b] - t_start[b];
+    const double Msec = (params[P_M1 * B + b] + params[P_M2 * B + b]) * MTSUN_SI;
+    const double mm = modes.mm[m];
+
+    // default grid (phenomhm.py:208-224) and t(f) on it; the frequencies go to X for now
+    const double lo = log10(mf_min), hi = log10(mf_max), step = (hi - lo) / (L - 1);
+    for (int i = lane; i < L; i += 64) {
+        const double f = logspace_pt(lo, step, hi, i, L) / Msec * mm / 2;
+        double amp, phase, tf;
+        phm::phm_eval(r.w, f, amp, phase, tf);
+        tfs[i] = tf + t_ref;
+        X[(base + i) * M + m] = f;
+    }
+    __syncthreads();
+
+    // window search (phenomhm.py:520-614): the single segment where t crosses 0 / Tobs
+    double tmin = 1e300, tmax = -1e300;
+    int n0 = 0, nT = 0, s0 = -1, sT = -1;
+    for (int i = lane; i < L; i += 64) {
+        const double t = tfs[i];
+        tmin = fmin(tmin, t);
+        tmax = fmax(tmax, t);
+        if (i + 1 < L) {
+            const double t1 = tfs[i + 1];
+            if ((0.0 < t1) && (0.0 > t)) {
+                n0++;
+                s0 = i;
+            }
+            if ((Tobs < t1) && (Tobs > t)) {
+                nT++;
+                sT = i;
+            }
+        }
+    }
+    tmin = wmin_d(tmin);
+    tmax = wmax_d(tmax);
+    n0 = wsum_i(n0);
+    nT = wsum_i(nT);
+    s0 = wmax_i(s0);
+    sT = wmax_i(sT);
+    const double* xf = X + base * M + m;
+    double f_min = xf[0], f_max = xf[(long)(L - 1) * M];
+    int flag = 0;
+    if (tmin < 0.0) {
+        if (n0 != 1) flag |= 1;
+        else {
+            const double y1 = xf[(long)s0 * M], y2 = xf[(long)(s0 + 1) * M];
+            const double x1 = tfs[s0], x2 = tfs[s0 + 1];
+            f_min = (y2 - y1) / (x2 - x1) * (0.0 - x1) + y1;
+        }
+    }
+    if (tmax > Tobs) {
+        if (nT != 1) flag |= 2;
+        else {
+            const double y1 = xf[(long)sT * M], y2 = xf[(long)(sT + 1) * M];
+            const double x1 = tfs[sT], x2 = tfs[sT + 1];
+            f_max = (y2 - y1) / (x2 - x1) * (Tobs - x1) + y1;
+        }
+    }
+    if (lane == 0 && flag) atomicOr(&flags[b], flag);
+    __syncthreads();  // every lane has read the default grid before it is overwritten
+
+    // refitted grid and the nine quantities on it
+    const double lo2 = log10(f_min), hi2 = log10(f_max), step2 = (hi2 - lo2) / (L - 1);
+    if (lane == 0) {
+        grid_lo[bm] = lo2;
+        grid_step[bm] = step2;
+    }
+    const double dt_shift = t_ref - initial_t_val;
+    for (int i = lane; i < L; i += 64) {
+        const double f = logspace_pt(lo2, step2, hi2, i, L);
+        double amp, phase, tf;
+        phm::phm_eval(r.w, f, amp, phase, tf);
+        phase = phase + ((f * dt_shift) * 2) * PI;
+        tf = tf + t_ref;
+        cd T[3];
+        double dphi;
+        resp::resp_eval(r.s, orb, f, tf, TDItag, rescaled, tdi2, T, dphi);
+        phase = phase + dphi;
+        const long p = (base + i) * M + m;
+        X[p] = f;
+        double* y = Y + p * NQ;
+        y[0] = amp;
+        y[1] = phase;
+        y[2] = tf;
+        y[3] = T[0].re;
+        y[4] = T[0].im;
+        y[5] = T[1].re;
+        y[6] = T[1].im;
+        y[7] = T[2].re;
+        y[8] = T[2].im;
+    }
+}
+
+// Not-a-knot spline of one binary: lane s = mode * 9 + quantity runs the
+// Thomas algorithm over the binary's L knots (the system of spline.hip);
+// consecutive lanes read consecutive quantities of the same knot.
+__global__ __launch_bounds__(64) void k_dir_spline(const double* __restrict__ X, const double* __restrict__ Y,
+                                                   double* __restrict__ C1, double* __restrict__ C2,
+                                                   double* __restrict__ C3, const long* __restrict__ off,
+                                                   const int* __restrict__ len, int M)
+{
+    const int b = blockIdx.x;
+    const int s = threadIdx.x;
+    if (s >= M * NQ) return;
+    const int m = s / NQ;
+    const int n = len[b];
+    const long base = off[b];
+    // element i of this system
+    auto xi = [&](int i) { return X[(base + i) * M + m]; };
+    auto yi = [&](int i) { return Y[(base + i) * M * NQ + s]; };
+    auto at = [&](int i) { return (base + i) * M * NQ + s; };
+    double* dp = C1;  // d' then the slopes (= c1)
+    double* cp = C3;  // c' then c3
+
+    double x0 = xi(0), x1 = xi(1), x2 = xi(2);
+    double y0 = yi(0), y1 = yi(1), y2 = yi(2);
+    double cprev, dprev;
+    {
+        double dx1 = x1 - x0, dx2 = x2 - x1, dd = x2 - x0;
+        double s1 = (y1 - y0) / dx1, s2 = (y2 - y1) / dx2;
+        double bb = ((dx1 + 2 * dd) * dx2 * s1 + dx1 * dx1 * s2) / dd;
+        cprev = dd / dx2;
+        dprev = bb / dx2;
+        cp[at(0)] = cprev;
+        dp[at(0)] = dprev;
+    }
+    double xa = x0, xb = x1, ya = y0, yb = y1;
+    for (int i = 1; i < n - 1; i++) {
+        const double xc = xi(i + 1), yc = yi(i + 1);
+        const double ex1 = xb - xa, ex2 = xc - xb;
+        const double t1 = (yb - ya) / ex1, t2 = (yc - yb) / ex2;
+        const double rhs = 3.0 * (ex2 * t1 + ex1 * t2);
+        const double diag = 2 * (ex1 + ex2), lo = ex2, up = ex1;
+        const double mm = diag - lo * cprev;
+        cprev = up / mm;
+        dprev = (rhs - lo * dprev) / mm;
+        cp[at(i)] = cprev;
+        dp[at(i)] = dprev;
+        xa = xb;
+        xb = xc;
+        ya = yb;
+        yb = yc;
+    }
+    {
+        const double xm3 = xi(n - 3), xm2 = xi(n - 2), xm1 = xi(n - 1);
+        const double ym3 = yi(n - 3), ym2 = yi(n - 2), ym1 = yi(n - 1);
+        const double ex1 = xm2 - xm3, ex2 = xm1 - xm2, ed = xm1 - xm3;
+        const double t1 = (ym2 - ym3) / ex1, t2 = (ym1 - ym2) / ex2;
+        const double rhs = ((ex2 * ex2 * t1 + (2 * ed + ex2) * ex1 * t2) / ed);
+        const double mm = ex1 - ed * cprev;
+        dp[at(n - 1)] = (rhs - ed * dprev) / mm;
+    }
+    double snext = dp[at(n - 1)];
+    double xn = xi(n - 1), yn = yi(n - 1);
+    dp[at(n - 1)] = 0.0;
+    C2[at(n - 1)] = 0.0;
+    cp[at(n - 1)] = 0.0;
+    for (int i = n - 2; i >= 0; i--) {
+        const double si = dp[at(i)] - cp[at(i)] * snext;
+        const double x_i = xi(i), y_i = yi(i);
+        const double dx = xn - x_i;
+        const double slope = (yn - y_i) / dx;
+        const double t = (si + snext - 2 * slope) / dx;
+        dp[at(i)] = si;
+        C2[at(i)] = (slope - si) / dx - t;
+        cp[at(i)] = t / dx;
+        snext = si;
+        xn = x_i;
+        yn = y_i;
+    }
+}
+
+// number of entries of the sorted array a[0..n) that are <= v (searchsorted side='right')
+__device__ inline int search_right(const double* __restrict__ a, int n, double v)
+{
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (a[mid] <= v) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Dense band of each binary and the exclusive prefix of its chunks; one
+// workgroup of 1024 threads, each owning a contiguous run of binaries.
+__global__ __launch_bounds__(1024) void k_dir_band(int* __restrict__ band, int* __restrict__ total,
+                                                   const double* __restrict__ X, const long* __restrict__ off,
+                                                   const int* __restrict__ len, const double* __restrict__ freqs,
+                                                   int N, int B, int M, int chunk)
+{
+    __shared__ int part[1024];
+    const int t = threadIdx.x;
+    const int per = (B + 1023) / 1024;
+    const int b0 = t * per, b1 = min(B, b0 + per);
+    int sum = 0;
+    for (int b = b0; b < b1; b++) {
+        const long base = off[b];
+        const int L = len[b];
+        double lo = 1e300, hi = -1e300;
+        for (int m = 0; m < M; m++) {
+            lo = fmin(lo, X[base * M + m]);
+            hi = fmax(hi, X[(base + L - 1) * M + m]);
+        }
+        const int st = search_right(freqs, N, lo), et = search_right(freqs, N, hi);
+        const int nch = et > st ? (et - st + chunk - 1) / chunk : 0;
+        band[3 * b] = st;
+        band[3 * b + 1] = et;
+        band[3 * b + 2] = nch;  // count for now
+        sum += nch;
+    }
+    part[t] = sum;
+    __syncthreads();
+    // inclusive scan of the 1024 partial counts (Hillis-Steele)
+    for (int o = 1; o < 1024; o <<= 1) {
+        const int v = t >= o ? part[t - o] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int run = t > 0 ? part[t - 1] : 0;
+    for (int b = b0; b < b1; b++) {
+        const int nch = band[3 * b + 2];
+        band[3 * b + 2] = run;
+        run += nch;
+    }
+    if (t == 1023) total[0] = part[1023];
+}
+
+// binary owning work item 'it' (largest b with chunk_begin <= it)
+__device__ inline int owner(const int* __restrict__ band, int B, int it)
+{
+    int lo = 0, hi = B - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (band[3 * mid + 2] <= it) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Persistent dense-band kernel: partial[it] = (Re, Im of sum conj(d~) h~, Re, Im of sum |h~|^2)
+// over the dense bins of work item it, where h~ = w h (Likelihood.cu:357-375).
+__global__ __launch_bounds__(256) void k_dir_like(double* __restrict__ partial, const int* __restrict__ band,
+                                                  const int* __restrict__ total, const double* __restrict__ X,
+                                                  const double* __restrict__ Y, const double* __restrict__ C1,
+                                                  const double* __restrict__ C2, const double* __restrict__ C3,
+                                                  const double* __restrict__ grid_lo,
+                                                  const double* __restrict__ grid_step,
+                                                  const long* __restrict__ off, const int* __restrict__ len,
+                                                  const double* __restrict__ t_start,
+                                                  const double* __restrict__ t_end, const double* __restrict__ freqs,
+                                                  const cd* __restrict__ dch, const double* __restrict__ wts, int N,
+                                                  int B, int M, int chunk)
+{
+    __shared__ double red[4][256 / 64];
+    const int nitems = total[0];
+    const int tid = threadIdx.x, nth = blockDim.x;
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const int b = owner(band, B, it);
+        const int c = it - band[3 * b + 2];
+        const int k0 = band[3 * b] + c * chunk;
+        const int k1 = min(band[3 * b + 1], k0 + chunk);
+        const long base = off[b];
+        const int L = len[b];
+        const double ts = t_start[b], te = t_end[b];
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+        for (int k = k0 + tid; k < k1; k += nth) {
+            const double f = freqs[k];
+            const double lf = log10(f);
+            cd h[3] = {mk(0.0, 0.0), mk(0.0, 0.0), mk(0.0, 0.0)};
+            for (int m = 0; m < M; m++) {
+                const double* xm = X + base * M + m;
+                // segment: searchsorted(x, f, 'right') - 1, first estimated from the
+                // log-spaced grid, then made exact against the stored knots
+                const double gst = grid_step[b * M + m];
+                const double u = gst > 0.0 ? (lf - grid_lo[b * M + m]) / gst : -2.0;
+                int i = !(u >= -1.0) ? -1 : (u >= (double)(L - 1) ? L - 1 : (int)floor(u));
+                while (i + 1 < L && xm[(long)(i + 1) * M] <= f) i++;
+                while (i >= 0 && xm[(long)i * M] > f) i--;
+                if (i <= -1 || i >= L - 1) continue;  // WaveformBuild.cu:92-93
+                const long p = (base + i) * M + m;
+                const double x = f - X[p];
+                const double x2 = x * x, x3 = x * x2;
+                double v[NQ];
+#pragma unroll
+                for (int q = 0; q < NQ; q++) {
+                    const long a = p * NQ + q;
+                    v[q] = Y[a] + C1[a] * x + C2[a] * x2 + C3[a] * x3;
+                }
+                const double tf = v[2];
+                if ((tf >= ts) && ((tf <= te) || (te <= 0.0))) {
+                    const cd ap = v[0] * cexpi_red(v[1]);
+                    h[0] += conj(mk(v[3], v[4]) * ap);
+                    h[1] += conj(mk(v[5], v[6]) * ap);
+                    h[2] += conj(mk(v[7], v[8]) * ap);
+                }
+            }
+            for (int ch = 0; ch < 3; ch++) {
+                const double w = wts[(long)ch * N + k];
+                const cd ht = h[ch] * w;
+                const cd d = dch[(long)ch * N + k];
+                const cd dh = conj(d) * ht;
+                a0 += dh.re;
+                a1 += dh.im;
+                a2 += ht.re * ht.re + ht.im * ht.im;
+            }
+        }
+        // block reduction in a fixed order
+        a0 = wsum(a0);
+        a1 = wsum(a1);
+        a2 = wsum(a2);
+        const int wv = tid >> 6;
+        if ((tid & 63) == 0) {
+            red[0][wv] = a0;
+            red[1][wv] = a1;
+            red[2][wv] = a2;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+            for (int w = 0; w < (nth >> 6); w++) {
+                s0 += red[0][w];
+                s1 += red[1][w];
+                s2 += red[2][w];
+            }
+            partial[4 * (long)it] = s0;
+            partial[4 * (long)it + 1] = s1;
+            partial[4 * (long)it + 2] = s2;
+            partial[4 * (long)it + 3] = 0.0;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_dir_reduce(double* __restrict__ d_h, double* __restrict__ h_h, const double* __restrict__ partial,
+                             const int* __restrict__ band, const int* __restrict__ total, int B)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    const int c0 = band[3 * b + 2];
+    const int c1 = b + 1 < B ? band[3 * (b + 1) + 2] : total[0];
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int c = c0; c < c1; c++) {
+        s0 += partial[4 * (long)c];
+        s1 += partial[4 * (long)c + 1];
+        s2 += partial[4 * (long)c + 2];
+    }
+    d_h[2 * b] = 4.0 * s0;
+    d_h[2 * b + 1] = 4.0 * s1;
+    h_h[2 * b] = 4.0 * s2;
+    h_h[2 * b + 1] = 0.0;
+}
+
+template <class T>
+T* dalloc(size_t n, bool& ok)
+{
+    void* p = nullptr;
+    if (!ok || hipMalloc(&p, n * sizeof(T) + 256) != hipSuccess) {
+        ok = false;
+        return nullptr;
+    }
+    return static_cast<T*>(p);
+}
+
+void plan_free(bbhx_direct_plan* p)
+{
+    void* bufs[] = {p->off, p->len, p->recs, p->X, p->Y, p->C1, p->C2, p->C3, p->grid_lo, p->grid_step, p->band,
+                    p->total, p->partial};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+}
+
+}  // namespace
+
+extern "C" int bbhx_hip_direct_plan_create(bbhx_direct_plan** out, int numBinAll, const int* lengths_host,
+                                           int numModes, const double* dataFreqs_host_unused, int data_length,
+                                           int chunk, int block, int grid)
+{
+    (void)dataFreqs_host_unused;
+    if (!out || numBinAll < 1 || !lengths_host || numModes < 1 || numModes > MAX_MODES || data_length < 1 ||
+        chunk < 1 || (block != 64 && block != 128 && block != 256) || grid < 1)
+        return ERR_ARGS;
+    std::vector<long> off(numBinAll);
+    long S = 0;
+    int Lmax = 0;
+    for (int b = 0; b < numBinAll; b++) {
+        if (lengths_host[b] < 4) return ERR_ARGS;
+        off[b] = S;
+        S += lengths_host[b];
+        Lmax = lengths_host[b] > Lmax ? lengths_host[b] : Lmax;
+    }
+    // the default-grid t(f) of one (binary, mode) lives in LDS
+    if ((size_t)Lmax * sizeof(double) > 48 * 1024) return ERR_ARGS;
+    auto* p = new bbhx_direct_plan;
+    (void)hipGetDevice(&p->device);
+    p->B = numBinAll;
+    p->M = numModes;
+    p->N = data_length;
+    p->Lmax = Lmax;
+    p->chunk = chunk;
+    p->block = block;
+    p->grid = grid;
+    p->S = S;
+    p->max_items = (long)numBinAll * ((data_length + chunk - 1) / chunk) + numBinAll;
+    if (p->max_items > 2147483647L) {
+        delete p;
+        return ERR_ARGS;
+    }
+    bool ok = true;
+    const size_t SM = (size_t)S * numModes;
+    p->off = dalloc<long>(numBinAll, ok);
+    p->len = dalloc<int>(numBinAll, ok);
+    p->recs = dalloc<FusedRec>((size_t)numBinAll * numModes, ok);
+    p->X = dalloc<double>(SM, ok);
+    p->Y = dalloc<double>(SM * NQ, ok);
+    p->C1 = dalloc<double>(SM * NQ, ok);
+    p->C2 = dalloc<double>(SM * NQ, ok);
+    p->C3 = dalloc<double>(SM * NQ, ok);
+    p->grid_lo = dalloc<double>((size_t)numBinAll * numModes, ok);
+    p->grid_step = dalloc<double>((size_t)numBinAll * numModes, ok);
+    p->band = dalloc<int>((size_t)numBinAll * 3, ok);
+    p->total = dalloc<int>(1, ok);
+    p->partial = dalloc<double>((size_t)p->max_items * 4, ok);
+    if (!ok) {
+        plan_free(p);
+        delete p;
+        return ERR_ALLOC;
+    }
+    if (hipMemcpy(p->off, off.data(), sizeof(long) * numBinAll, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(p->len, lengths_host, sizeof(int) * numBinAll, hipMemcpyHostToDevice) != hipSuccess) {
+        plan_free(p);
+        delete p;
+        return ERR_LAUNCH;
+    }
+    *out = p;
+    return OK;
+}
+
+extern "C" int bbhx_hip_direct_plan_destroy(bbhx_direct_plan* p)
+{
+    if (!p) return ERR_ARGS;
+    (void)hipDeviceSynchronize();
+    plan_free(p);
+    delete p;
+    return OK;
+}
+
+extern "C" long bbhx_hip_direct_plan_bytes(const bbhx_direct_plan* p)
+{
+    if (!p) return -1;
+    const long SM = p->S * p->M;
+    return SM * 8 * (1 + 4 * NQ) + (long)p->B * p->M * (sizeof(FusedRec) + 16) + p->max_items * 32 + p->B * 24;
+}
+
+extern "C" int bbhx_hip_direct_fused(bbhx_direct_plan* p, double* d_h, double* h_h, int* flags, const double* params,
+                                     const double* t_start, const double* t_end, const int* ells_host,
+                                     const int* mms_host, int numModes, const double* dataFreqs,
+                                     const double* dataChannels, const double* noiseFactors, const double* qnm,
+                                     int table_len, double dspin, double initial_t_val, double mf_min, double mf_max,
+                                     const bbhx_response* orbits, int TDItag, int rescaled, int tdi2, void* stream)
+{
+    if (!p) return ERR_ARGS;
+    int dev = -1;
+    (void)hipGetDevice(&dev);
+    if (dev != p->device || numModes != p->M) return ERR_ARGS;
+    resp::OrbitTab orb;
+    if (!response_orbit_tab(orbits, orb)) return ERR_ARGS;
+    ModeList ml;
+    const int mst = check_modes(ells_host, mms_host, numModes, ml);
+    if (mst != OK) return mst;
+    if (!d_h || !h_h || !flags || !params || !t_start || !t_end || !dataFreqs || !dataChannels || !noiseFactors ||
+        !qnm || table_len < 2 || !(mf_min > 0.0) || !(mf_max > mf_min))
+        return ERR_ARGS;
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const int B = p->B, M = p->M;
+    hipEvent_t ev[3];
+    const bool timed = timer_take3(ev);
+    if (timed) (void)hipEventRecord(ev[0], st);
+    hipLaunchKernelGGL(k_dir_setup, dim3(ceil_div((long)B * M, 64)), dim3(64), 0, st, p->recs, params, B, ml, M, qnm,
+                       table_len, dspin);
+    hipLaunchKernelGGL(k_dir_grid, dim3(B * M), dim3(64), sizeof(double) * (size_t)p->Lmax, st, p->X, p->Y,
+                       p->grid_lo, p->grid_step, flags, p->recs, p->off, p->len, params, t_start, t_end, B, M, ml,
+                       orb, TDItag, rescaled, tdi2, initial_t_val, mf_min, mf_max);
+    hipLaunchKernelGGL(k_dir_spline, dim3(B), dim3(64), 0, st, p->X, p->Y, p->C1, p->C2, p->C3, p->off, p->len, M);
+    hipLaunchKernelGGL(k_dir_band, dim3(1), dim3(1024), 0, st, p->band, p->total, p->X, p->off, p->len, dataFreqs,
+                       p->N, B, M, p->chunk);
+    if (timed) (void)hipEventRecord(ev[1], st);
+    hipLaunchKernelGGL(k_dir_like, dim3(p->grid), dim3(p->block), 0, st, p->partial, p->band, p->total, p->X, p->Y,
+                       p->C1, p->C2, p->C3, p->grid_lo, p->grid_step, p->off, p->len, t_start, t_end, dataFreqs,
+                       reinterpret_cast<const cd*>(dataChannels), noiseFactors, p->N, B, M, p->chunk);
+    hipLaunchKernelGGL(k_dir_reduce, dim3(ceil_div(B, 256)), dim3(256), 0, st, d_h, h_h, p->partial, p->band,
+                       p->total, B);
+    if (timed) (void)hipEventRecord(ev[2], st);
+    return hipGetLastError() == hipSuccess ? OK : ERR_LAUNCH;
+}

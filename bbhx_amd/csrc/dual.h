@@ -12,6 +12,18 @@
 
 namespace bbhx {
 
+// the double versions stay visible next to the Dual overloads below
+using ::atan;
+using ::cbrt;
+using ::cos;
+using ::exp;
+using ::fabs;
+using ::hypot;
+using ::log;
+using ::pow;
+using ::sin;
+using ::sqrt;
+
 template <int N>
 struct Dual {
     double v;
@@ -106,7 +118,7 @@ __device__ inline Dual<N> operator/(const Dual<N>& a, const Dual<N>& b)
     // (a / b)' = (a' - q b') / b
     const double ib = 1.0 / b.v;
     Dual<N> r;
-    r.v = a.v * ib;
+    r.v = a.v / b.v;
 #pragma unroll
     for (int i = 0; i < N; i++) r.d[i] = fma(-r.v, b.d[i], a.d[i]) * ib;
     return r;
@@ -168,9 +180,8 @@ __device__ inline Dual<N> operator/(const Dual<N>& a, double b)
 template <int N>
 __device__ inline Dual<N> operator/(double a, const Dual<N>& b)
 {
-    const double ib = 1.0 / b.v;
-    const double q = a * ib;
-    return chain(q, -q * ib, b);
+    const double q = a / b.v;
+    return chain(q, -q / b.v, b);
 }
 
 template <int N, class T>

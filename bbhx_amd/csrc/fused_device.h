@@ -1,0 +1,157 @@
+// Device pieces shared by the fused heterodyned-likelihood kernels (forward
+// and gradient): the per-(template, mode) record set-up, the residual pass
+// r = h / h0 on the sparse nodes and the heterodyned segment sums.
+//
+// Reference chain: HeterodynedLikelihood.get_ll (likelihood.py:550-648) ->
+// PhenomHM (PhenomHMWaveform.cu), t_ref shift (phenomhm.py:487-507), response
+// (Response.cu), windowed mode sum (WaveformBuild.cu:48-60), r = h / h0
+// (likelihood.py:603), hdyn (Likelihood.cu:52-111).
+#pragma once
+
+#include "common.h"
+#include "dual.h"
+#include "phenomhm_device.h"
+#include "response_device.h"
+
+namespace bbhx {
+namespace fused {
+
+constexpr int NPARAM = 11;
+enum { P_M1, P_M2, P_CHI1, P_CHI2, P_DIST, P_PHI, P_INC, P_LAM, P_BETA, P_PSI, P_TREF };
+
+struct FusedRec {
+    phm::PhmRec w;
+    resp::RespRec s;
+};
+
+struct ModeList {
+    int ell[MAX_MODES];
+    int mm[MAX_MODES];
+};
+
+// PhenomD (2,2) ringdown frequencies in units of 1/M from the QNM table spline
+// (PhenomHMWaveform.cu:1963-1992); the spin index is clamped into the table.
+template <class S>
+__device__ inline void phenomd_ringdown(S finmass, S finspin, const double* __restrict__ T, int table_len,
+                                        double dspin, S& rd, S& dm)
+{
+    int k = int((value(finspin) - (-1.0)) / dspin);
+    k = k < 0 ? 0 : (k > table_len - 1 ? table_len - 1 : k);
+    const S x = finspin - (dspin * k + (-1.0));
+    const S x2 = x * x, x3 = x2 * x;
+    rd = (T[k] + T[table_len + k] * x + T[2 * table_len + k] * x2 + T[3 * table_len + k] * x3) / finmass;
+    dm = (T[4 * table_len + k] + T[5 * table_len + k] * x + T[6 * table_len + k] * x2 + T[7 * table_len + k] * x3) /
+         finmass;
+}
+
+// Record of template b, mode mi (params: (11, B) rows).
+__device__ inline void setup_one(FusedRec& r, const double* __restrict__ params, int B, int b, int l, int m,
+                                 const double* __restrict__ qnm, int table_len, double dspin)
+{
+    const double m1 = params[P_M1 * B + b], m2 = params[P_M2 * B + b];
+    const double c1 = params[P_CHI1 * B + b], c2 = params[P_CHI2 * B + b];
+    double finmass, finspin;
+    phm::final_state(m1, m2, c1, c2, finmass, finspin);
+    double rdD, dmD, rd, dm;
+    phenomd_ringdown(finmass, finspin, qnm, table_len, dspin, rdD, dmD);
+    phm::hm_ringdown(l, m, finmass, finspin, rd, dm);
+    phm::phm_setup(r.w, m1 * MSUN_SI, m2 * MSUN_SI, c1, c2, params[P_DIST * B + b], l, m, rd, dm, rdD, dmD, false);
+    resp::resp_setup(r.s, l, m, params[P_PHI * B + b], params[P_INC * B + b], params[P_LAM * B + b],
+                     params[P_BETA * B + b], params[P_PSI * B + b]);
+}
+
+__device__ inline double wsum(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+struct Window {
+    double t_ref, ts, te, dt_shift;
+};
+
+__device__ inline bool in_window(const Window& w, double tf)
+{
+    return (tf >= w.ts) && ((tf <= w.te) || (w.te <= 0.0));
+}
+
+// Residual pass of one 64-lane wavefront: lane l owns nodes l, l+64, ...; for
+// each node the three channels are summed over the modes and r = h / h0 is
+// written to rs[c * K + j] (LDS).
+__device__ inline void residuals(cd* rs, const FusedRec* __restrict__ rb, int M, const Window& win,
+                                 const double* __restrict__ freqs, const cd* __restrict__ h0, int K,
+                                 const resp::OrbitTab& orb, int TDItag, int rescaled, int tdi2)
+{
+    const int lane = threadIdx.x;
+    for (int j = lane; j < K; j += 64) {
+        const double f = freqs[j];
+        cd h[3] = {mk(0.0, 0.0), mk(0.0, 0.0), mk(0.0, 0.0)};
+        for (int mi = 0; mi < M; mi++) {
+            const FusedRec& r = rb[mi];
+            double amp, phase, tf;
+            phm::phm_eval(r.w, f, amp, phase, tf);
+            // phenomhm.py:487-507: phase += 2 pi f (t_ref - t0); tf += t_ref
+            phase = phase + ((f * win.dt_shift) * 2) * PI;
+            tf = tf + win.t_ref;
+            cd T[3];
+            double dphi;
+            resp::resp_eval(r.s, orb, f, tf, TDItag, rescaled, tdi2, T, dphi);
+            phase = phase + dphi;
+            if (in_window(win, tf)) {
+                const cd ap = amp * cexpi_red(phase);
+                h[0] += conj(T[0] * ap);
+                h[1] += conj(T[1] * ap);
+                h[2] += conj(T[2] * ap);
+            }
+        }
+        // r = h / h0 (likelihood.py:603)
+        for (int c = 0; c < 3; c++) {
+            const cd d = h0[c * K + j];
+            const double iden = rcp_pos(d.re * d.re + d.im * d.im);
+            rs[c * K + j] = mk((h[c].re * d.re + h[c].im * d.im) * iden, (h[c].im * d.re - h[c].re * d.im) * iden);
+        }
+    }
+}
+
+// Heterodyned sums (Likelihood.cu:52-111) over the segments j = lane+1,
+// lane+65, ... < jend; segment j joins nodes j-1 and j. Returns the wavefront
+// totals (d_h re, im, h_h re, im) in every lane.
+__device__ inline void segment_sums(const cd* rs, const double* __restrict__ freqs, const cd* __restrict__ dc, int K,
+                                    int jend, double out[4])
+{
+    const int lane = threadIdx.x;
+    double l1r = 0.0, l1i = 0.0, l2r = 0.0, l2i = 0.0;
+    for (int j = lane + 1; j < jend; j += 64) {
+        const double fj = freqs[j], fprev = freqs[j - 1];
+        if (fprev == 0.0) continue;
+        const double inv = 1.0 / (fj - fprev);
+        const double fm = (fj + fprev) / 2.0;
+        for (int c = 0; c < 3; c++) {
+            const cd cur = rs[c * K + j], prev = rs[c * K + j - 1];
+            const cd A0 = dc[(0 * 3 + c) * K + j], A1 = dc[(1 * 3 + c) * K + j];
+            const cd B0 = dc[(2 * 3 + c) * K + j], B1 = dc[(3 * 3 + c) * K + j];
+            const cd r1 = mk((cur.re - prev.re) * inv, (cur.im - prev.im) * inv);
+            const cd r0 = cur - r1 * (fj - fm);
+            const cd r1c = conj(r1);
+            const cd t1 = A0 * conj(r0) + A1 * r1c;
+            const double mag2 = r0.re * r0.re + r0.im * r0.im;
+            const cd t2 = B0 * mag2 + 2. * B1 * (r0 * r1c).re;
+            l1r += t1.re;
+            l1i += t1.im;
+            l2r += t2.re;
+            l2i += t2.im;
+        }
+    }
+    out[0] = wsum(l1r);
+    out[1] = wsum(l1i);
+    out[2] = wsum(l2r);
+    out[3] = wsum(l2i);
+}
+
+// Host helpers shared by the fused entry points (hdyn_fused.hip): three timing
+// events when bbhx_hip_timing_enable(1) is on, and the (l, m) list check.
+bool timer_take3(hipEvent_t ev[3]);
+int check_modes(const int* ells_host, const int* mms_host, int numModes, ModeList& ml);
+
+}  // namespace fused
+}  // namespace bbhx

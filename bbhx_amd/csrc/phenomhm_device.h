@@ -17,6 +17,15 @@
 namespace bbhx {
 namespace phm {
 
+// Non-deduced parameter type: the scalar type of a template below comes from
+// its record argument; plain doubles passed elsewhere convert to it.
+template <class T>
+struct nd_t {
+    using type = T;
+};
+template <class T>
+using nd = typename nd_t<T>::type;
+
 constexpr double AMP_JOIN_INS = 0.014;  // PhenomHMWaveform.hh:62
 constexpr double PHI_JOIN_INS = 0.018;  // PhenomHMWaveform.hh:68
 
@@ -89,23 +98,26 @@ __device__ const double FITS[F_COUNT][11] = {
      4.396844346849777e6},
 };
 
-__device__ inline double fit(int id, double eta, double chi)
+template <class S>
+__device__ inline S fit(int id, S eta, S chi)
 {
     const double* k = FITS[id];
-    double xi = -1 + chi;
-    double xi2 = xi * xi, xi3 = xi2 * xi, eta2 = eta * eta;
+    S xi = -1 + chi;
+    S xi2 = xi * xi, xi3 = xi2 * xi, eta2 = eta * eta;
     return k[0] + k[1] * eta + (k[2] + k[3] * eta + k[4] * eta2) * xi + (k[5] + k[6] * eta + k[7] * eta2) * xi2 +
            (k[8] + k[9] * eta + k[10] * eta2) * xi3;
 }
 
 // Powers of x built from x^(1/6) (PhenomHMWaveform.cu:354-368).
-struct P6 {
-    double sixth, third, two_thirds, four_thirds, five_thirds, two, seven_thirds, eight_thirds;
+template <class S>
+struct P6T {
+    S sixth, third, two_thirds, four_thirds, five_thirds, two, seven_thirds, eight_thirds;
 };
 
-__device__ inline P6 pow6(double x)
+template <class S>
+__device__ inline P6T<S> pow6(S x)
 {
-    P6 p;
+    P6T<S> p;
     p.sixth = cbrt_pos(sqrt(x));
     p.third = p.sixth * p.sixth;
     p.two_thirds = p.third * p.third;
@@ -120,80 +132,87 @@ __device__ inline P6 pow6(double x)
 // ------------------------------------------------------------- record
 
 // Everything point evaluation needs for one (binary, mode). ~100 doubles.
-struct PhmRec {
+template <class S>
+struct PhmRecT {
     // binary level
-    double M_sec, amp_scale, t0, phi0, Mf_ref;
+    S M_sec, amp_scale, t0, phi0, Mf_ref;
     // leading-order PN amplitude model (IMRPhenomHMOnePointFiveSpinPN inputs)
-    double pn_M, pn_eta, pn_delta, pn_Xs, pn_Xa, pn_pref;
-    double pn_vscale, pn_v2ratio;  // 2 pi M / m and (2/m)^(1/3) of this mode
+    S pn_M, pn_eta, pn_delta, pn_Xs, pn_Xa, pn_pref;
+    S pn_vscale, pn_v2ratio;  // 2 pi M / m and (2/m)^(1/3) of this mode
     // PhenomD amplitude (mode independent)
-    double a_fRD, a_fDM, a_g1, a_g2, a_g3, a_fmax;
-    double a_rho1, a_rho2, a_rho3, a_d0, a_d1, a_d2, a_d3, a_d4;
-    double a_pf23, a_pf1, a_pf43, a_pf53, a_pf2, a_amp0;
+    S a_fRD, a_fDM, a_g1, a_g2, a_g3, a_fmax;
+    S a_rho1, a_rho2, a_rho3, a_d0, a_d1, a_d2, a_d3, a_d4;
+    S a_pf23, a_pf1, a_pf43, a_pf53, a_pf2, a_amp0;
     // PhenomD phase with this mode's ringdown ratios
-    double p_eta, p_etaInv, p_fRD, p_fDM, p_Rholm, p_Taulm;
-    double beta1, beta2, beta3, alpha1, alpha2, alpha3, alpha4, alpha5;
-    double sigma1, sigma2, sigma3, sigma4;
-    double C1Int, C2Int, C1MRD, C2MRD, fMRDJoin;
-    double pn_v0, pn_v2, pn_v3, pn_v4, pn_v5, pn_v6, pn_v7, pn_vl5, pn_vl6;
-    double q_init, q_23, q_13, q_13log, q_logv, q_m13, q_m23, q_m1, q_m53, q_1, q_43, q_53, q_2;
+    S p_eta, p_etaInv, p_fRD, p_fDM, p_Rholm, p_Taulm;
+    S beta1, beta2, beta3, alpha1, alpha2, alpha3, alpha4, alpha5;
+    S sigma1, sigma2, sigma3, sigma4;
+    S C1Int, C2Int, C1MRD, C2MRD, fMRDJoin;
+    S pn_v0, pn_v2, pn_v3, pn_v4, pn_v5, pn_v6, pn_v7, pn_vl5, pn_vl6;
+    S q_init, q_23, q_13, q_13log, q_logv, q_m13, q_m23, q_m1, q_m53, q_1, q_43, q_53, q_2;
     // HM frequency maps: phase (a*, b*, fi, fr) and amplitude (amp_*)
-    double ai, bi, am, bm, ar, br, fi, fr, PhDBconst, PhDCconst, PhDBAterm, phaseC;
-    double amp_fi, amp_fr, amp_Ai, amp_Am, amp_Bm, amp_Br;
-    double cshift;
+    S ai, bi, am, bm, ar, br, fi, fr, PhDBconst, PhDCconst, PhDBAterm, phaseC;
+    S amp_fi, amp_fr, amp_Ai, amp_Am, amp_Bm, amp_Br;
+    S cshift;
     int ell, mm, phenomd, pad;
 };
+using PhmRec = PhmRecT<double>;
 
 // ------------------------------------------------------------ amplitude
 
-__device__ inline double amp_ins(const PhmRec& r, double Mf, const P6& p)
+template <class S>
+__device__ inline S amp_ins(const PhmRecT<S>& r, nd<S> Mf, const P6T<S>& p)
 {
-    double Mf2 = p.two, Mf3 = Mf * Mf2;
+    S Mf2 = p.two, Mf3 = Mf * Mf2;
     return 1 + p.two_thirds * r.a_pf23 + Mf * r.a_pf1 + p.four_thirds * r.a_pf43 + p.five_thirds * r.a_pf53 +
            Mf2 * r.a_pf2 + p.seven_thirds * r.a_rho1 + p.eight_thirds * r.a_rho2 + Mf3 * r.a_rho3;
 }
 
-__device__ inline double amp_mrd(const PhmRec& r, double f)
+template <class S>
+__device__ inline S amp_mrd(const PhmRecT<S>& r, nd<S> f)
 {
-    const double fDMg3 = r.a_fDM * r.a_g3;
-    const double d = f - r.a_fRD;
-    const double inv = rcp_pos(fDMg3);
+    const S fDMg3 = r.a_fDM * r.a_g3;
+    const S d = f - r.a_fRD;
+    const S inv = rcp_pos(fDMg3);
     return exp(-(d * r.a_g2) * inv) * (fDMg3 * r.a_g1) * rcp_pos(d * d + fDMg3 * fDMg3);
 }
 
-__device__ inline double amp_mrd_deriv(const PhmRec& r, double f)
+template <class S>
+__device__ inline S amp_mrd_deriv(const PhmRecT<S>& r, nd<S> f)
 {
-    double fDMg3 = r.a_fDM * r.a_g3;
-    double d = f - r.a_fRD;
-    double e = exp((d * r.a_g2) / (fDMg3));
-    double s = d * d + fDMg3 * fDMg3;
+    S fDMg3 = r.a_fDM * r.a_g3;
+    S d = f - r.a_fRD;
+    S e = exp((d * r.a_g2) / (fDMg3));
+    S s = d * d + fDMg3 * fDMg3;
     return (-2 * r.a_fDM * (d)*r.a_g3 * r.a_g1) / (e * (s * s)) - (r.a_g2 * r.a_g1) / (e * (s));
 }
 
 // IMRPhenDAmplitude without the distance factor (PhenomHMWaveform.cu:1508-1537)
-__device__ inline double amp_imr(const PhmRec& r, double f, const P6& p)
+template <class S>
+__device__ inline S amp_imr(const PhmRecT<S>& r, nd<S> f, const P6T<S>& p)
 {
-    const double pre = r.a_amp0 * rcp_pos(f * p.sixth);
+    const S pre = r.a_amp0 * rcp_pos(f * p.sixth);
     if (!(f >= AMP_JOIN_INS)) return pre * amp_ins(r, f, p);
     if (f >= r.a_fmax) return pre * amp_mrd(r, f);
-    double f2 = f * f, f3 = f * f2, f4 = f * f3;
+    S f2 = f * f, f3 = f * f2, f4 = f * f3;
     return pre * (r.a_d0 + r.a_d1 * f + r.a_d2 * f2 + r.a_d3 * f3 + r.a_d4 * f4);
 }
 
 // |H_lm(v)| of the leading-order PN amplitude (IMRPhenomHMOnePointFiveSpinPN,
 // PhenomHMWaveform.cu:2405-2449) without the common v^-3.5 prefactor.
-__device__ inline double pn_habs(const PhmRec& r, double v)
+template <class S>
+__device__ inline S pn_habs(const PhmRecT<S>& r, nd<S> v)
 {
     const int l = r.ell, m = r.mm;
-    const double eta = r.pn_eta, delta = r.pn_delta;
+    const S eta = r.pn_eta, delta = r.pn_delta;
     if (l == 2 && m == 1) {
-        const double v2 = v * v, v3 = v * v2, v4 = v * v3;
-        const double Xs = r.pn_Xs, Xa = r.pn_Xa;
-        double re = v * delta - v2 * 1.5 * (Xa + delta * Xs) + v3 * delta * ((335.0 / 672.0) + (eta * 117.0 / 56.0)) +
+        const S v2 = v * v, v3 = v * v2, v4 = v * v3;
+        const S Xs = r.pn_Xs, Xa = r.pn_Xa;
+        S re = v * delta - v2 * 1.5 * (Xa + delta * Xs) + v3 * delta * ((335.0 / 672.0) + (eta * 117.0 / 56.0)) +
                     v4 * (Xa * (3427.0 / 1344 - eta * 2101.0 / 336) + delta * Xs * (3427.0 / 1344 - eta * 965 / 336) +
                           delta * (-PI));
-        double im = v4 * (delta * (-0.5 - 2.0 * 0.69314718056));
-        const double pre = (sqrt(2.0) / 3.0);
+        S im = v4 * (delta * (-0.5 - 2.0 * 0.69314718056));
+        const S pre = (sqrt(2.0) / 3.0);
         return hypot(pre * re, pre * im);
     }
     if (l == 3 && m == 3) return fabs(0.75 * sqrt(5.0 / 7.0) * (v * delta));
@@ -208,14 +227,15 @@ __device__ inline double pn_habs(const PhmRec& r, double v)
 // v4 = (m/2)^(1/3) v3, the v^-3.5 prefactors of the four PN amplitudes cancel
 // exactly, leaving |H(v1)| |H(v3)| / |H(v2)| (1 for the (2,2) mode; 0 where the
 // reference's beta_term1 vanishes).
-__device__ inline double hm_amp_factor(const PhmRec& r, double Mf_wf, double Mf_amp)
+template <class S>
+__device__ inline S hm_amp_factor(const PhmRecT<S>& r, nd<S> Mf_wf, nd<S> Mf_amp)
 {
     if (r.ell == 2 && r.mm == 2) return 1.0;
-    const double v1 = cbrt_pos(r.pn_vscale * Mf_wf);
-    const double h1 = pn_habs(r, v1);
+    const S v1 = cbrt_pos(r.pn_vscale * Mf_wf);
+    const S h1 = pn_habs(r, v1);
     if (h1 == 0.0) return 0.0;
-    const double v2 = v1 * r.pn_v2ratio;
-    const double v3 = cbrt_pos(r.pn_vscale * Mf_amp);
+    const S v2 = v1 * r.pn_v2ratio;
+    const S v3 = cbrt_pos(r.pn_vscale * Mf_amp);
     return h1 * pn_habs(r, v3) / pn_habs(r, v2);
 }
 
@@ -225,11 +245,12 @@ __device__ inline double hm_amp_factor(const PhmRec& r, double Mf_wf, double Mf_
 // power below is a product of iMf and the positive powers, so the only
 // per-point divisions are that reciprocal and 1/v.
 
-__device__ inline double phi_ins(const PhmRec& r, double Mf, const P6& p, double logv, double iMf)
+template <class S>
+__device__ inline S phi_ins(const PhmRecT<S>& r, nd<S> Mf, const P6T<S>& p, nd<S> logv, nd<S> iMf)
 {
-    const double im13 = p.two_thirds * iMf;  // Mf^(-1/3)
-    const double im23 = im13 * im13;
-    double ph = r.q_init;
+    const S im13 = p.two_thirds * iMf;  // Mf^(-1/3)
+    const S im23 = im13 * im13;
+    S ph = r.q_init;
     ph += r.q_23 * p.two_thirds;
     ph += r.q_13 * p.third;
     ph += r.q_13log * logv * p.third;
@@ -243,11 +264,12 @@ __device__ inline double phi_ins(const PhmRec& r, double Mf, const P6& p, double
 }
 
 // d phi_ins / dMf with v = (pi Mf)^(1/3) given
-__device__ inline double dphi_ins(const PhmRec& r, double v, double logv)
+template <class S>
+__device__ inline S dphi_ins(const PhmRecT<S>& r, nd<S> v, nd<S> logv)
 {
-    const double v2 = v * v, v3 = v * v2, v4 = v * v3, v5 = v * v4, v6 = v * v5, v7 = v * v6;
-    const double iv = rcp_pos(v), iv2 = iv * iv, iv4 = iv2 * iv2, iv8 = iv4 * iv4;
-    double D = 0.0;
+    const S v2 = v * v, v3 = v * v2, v4 = v * v3, v5 = v * v4, v6 = v * v5, v7 = v * v6;
+    const S iv = rcp_pos(v), iv2 = iv * iv, iv4 = iv2 * iv2, iv8 = iv4 * iv4;
+    S D = 0.0;
     D += +2.0 * r.pn_v7 * v7;
     D += (r.pn_v6 + r.pn_vl6 * (1.0 + logv)) * v6;
     D += r.pn_vl5 * v5;
@@ -262,44 +284,49 @@ __device__ inline double dphi_ins(const PhmRec& r, double v, double logv)
     return D;
 }
 
-__device__ inline double phi_int(const PhmRec& r, double Mf, double iMf)
+template <class S>
+__device__ inline S phi_int(const PhmRecT<S>& r, nd<S> Mf, nd<S> iMf)
 {
     return r.beta1 * Mf - r.beta3 * (iMf * iMf * iMf) * (1.0 / 3.0) + r.beta2 * log(Mf);
 }
 
-__device__ inline double dphi_int(const PhmRec& r, double iMf)
+template <class S>
+__device__ inline S dphi_int(const PhmRecT<S>& r, nd<S> iMf)
 {
-    const double im2 = iMf * iMf;
+    const S im2 = iMf * iMf;
     return (r.beta1 + r.beta3 * (im2 * im2) + r.beta2 * iMf) * r.p_etaInv;
 }
 
-__device__ inline double phi_mrd(const PhmRec& r, double f, double iMf)
+template <class S>
+__device__ inline S phi_mrd(const PhmRecT<S>& r, nd<S> f, nd<S> iMf)
 {
-    const double sf = sqrt(f);
-    const double f075 = sqrt(f * sf);
+    const S sf = sqrt(f);
+    const S f075 = sqrt(f * sf);
     return -(r.alpha2 * iMf) + (4.0 / 3.0) * (r.alpha3 * f075) + r.alpha1 * f +
            r.alpha4 * r.p_Rholm * atan((f - r.alpha5 * r.p_fRD) * rcp_pos(r.p_Rholm * r.p_fDM * r.p_Taulm));
 }
 
-__device__ inline double dphi_mrd(const PhmRec& r, double f, double iMf)
+template <class S>
+__device__ inline S dphi_mrd(const PhmRecT<S>& r, nd<S> f, nd<S> iMf)
 {
-    const double num = f - r.alpha5 * r.p_fRD;
-    const double q = num * rcp_pos(r.p_fDM * r.p_Taulm * r.p_Rholm);
-    const double f075 = sqrt(f * sqrt(f));  // f^(-1/4) = f^(3/4) / f
+    const S num = f - r.alpha5 * r.p_fRD;
+    const S q = num * rcp_pos(r.p_fDM * r.p_Taulm * r.p_Rholm);
+    const S f075 = sqrt(f * sqrt(f));  // f^(-1/4) = f^(3/4) / f
     return (r.alpha1 + r.alpha2 * (iMf * iMf) + r.alpha3 * (f075 * iMf) +
             r.alpha4 * rcp_pos(r.p_fDM * r.p_Taulm * (1 + q * q))) *
            r.p_etaInv;
 }
 
 // IMRPhenDPhase and its derivative at the same Mf, sharing the powers.
-__device__ inline void phase_imr(const PhmRec& r, double f, double& ph, double* dph)
+template <class S>
+__device__ inline void phase_imr(const PhmRecT<S>& r, nd<S> f, S& ph, S* dph)
 {
-    const double iMf = rcp_pos(f);
+    const S iMf = rcp_pos(f);
     if (!(f >= PHI_JOIN_INS)) {
-        P6 p = pow6(f);
+        P6T<S> p = pow6(f);
         const double pi13 = 1.4645918875615231;
-        double v = p.third * pi13;
-        double logv = log(v);
+        S v = p.third * pi13;
+        S logv = log(v);
         ph = phi_ins(r, f, p, logv, iMf);
         if (dph) *dph = dphi_ins(r, v, logv);
         return;
@@ -313,41 +340,44 @@ __device__ inline void phase_imr(const PhmRec& r, double f, double& ph, double* 
     if (dph) *dph = dphi_int(r, iMf) + r.C2Int;
 }
 
-__device__ inline double phase_only(const PhmRec& r, double f)
+template <class S>
+__device__ inline S phase_only(const PhmRecT<S>& r, nd<S> f)
 {
-    double ph;
-    phase_imr(r, f, ph, nullptr);
+    S ph;
+    phase_imr(r, f, ph, (S*)nullptr);
     return ph;
 }
 
 // -------------------------------------------------------- final state / QNM
 
-__device__ inline void enforce_primary(double& m1, double& m2, double& c1, double& c2)
+template <class S>
+__device__ inline void enforce_primary(S& m1, S& m2, S& c1, S& c2)
 {
     if (!(m1 > m2)) {
-        double t = m1; m1 = m2; m2 = t;
+        S t = m1; m1 = m2; m2 = t;
         t = c1; c1 = c2; c2 = t;
     }
 }
 
 // IMRPhenomDFinalMass / FinalSpin (PhenomHMWaveform.cu:1679-1814)
-__device__ inline void final_state(double m1, double m2, double chi1, double chi2, double& finmass, double& finspin)
+template <class S>
+__device__ inline void final_state(S m1, S m2, S chi1, S chi2, S& finmass, S& finspin)
 {
     enforce_primary(m1, m2, chi1, chi2);
-    double M = m1 + m2;
-    double eta = m1 * m2 / (M * M);
-    double Seta = sqrt(1.0 - 4.0 * eta);
-    double q1 = 0.5 * (1.0 + Seta), q2 = 0.5 * (1.0 - Seta);
-    double q1s = q1 * q1, q2s = q2 * q2;
-    double eta2 = eta * eta, eta3 = eta2 * eta, eta4 = eta3 * eta;
-    double s = (q1s * chi1 + q2s * chi2) / (q1s + q2s);
-    double erad = ((0.055974469826360077 * eta + 0.5809510763115132 * eta2 - 0.9606726679372312 * eta3 +
+    S M = m1 + m2;
+    S eta = m1 * m2 / (M * M);
+    S Seta = sqrt(1.0 - 4.0 * eta);
+    S q1 = 0.5 * (1.0 + Seta), q2 = 0.5 * (1.0 - Seta);
+    S q1s = q1 * q1, q2s = q2 * q2;
+    S eta2 = eta * eta, eta3 = eta2 * eta, eta4 = eta3 * eta;
+    S s = (q1s * chi1 + q2s * chi2) / (q1s + q2s);
+    S erad = ((0.055974469826360077 * eta + 0.5809510763115132 * eta2 - 0.9606726679372312 * eta3 +
                     3.352411249771192 * eta4) *
                    (1. + (-0.0030302335878845507 - 2.0066110851351073 * eta + 7.7050567802399215 * eta2) * s)) /
                   (1. + (-0.6714403054720589 - 1.4756929437702908 * eta + 7.304676214885011 * eta2) * s);
     finmass = 1.0 - erad;
-    double sf = (q1s * chi1 + q2s * chi2);
-    double sf2 = sf * sf, sf3 = sf2 * sf, sf4 = sf3 * sf;
+    S sf = (q1s * chi1 + q2s * chi2);
+    S sf2 = sf * sf, sf3 = sf2 * sf, sf4 = sf3 * sf;
     finspin = 3.4641016151377544 * eta - 4.399247300629289 * eta2 + 9.397292189321194 * eta3 -
               13.180949901606242 * eta4 + (1 - 0.0850917821418767 * eta - 5.837029316602263 * eta2) * sf +
               (0.1014665242971878 * eta - 2.0967746996832157 * eta2) * sf2 +
@@ -357,7 +387,8 @@ __device__ inline void final_state(double m1, double m2, double chi1, double chi
 
 // Complex QNM frequency fit CW07102016 (n = 0) of the (l, m) mode in units of 1/M_final;
 // returns false for a mode without a fit. (PhenomHMWaveform.cu:56-211)
-__device__ inline bool qnm_cw(double kappa, int l, int m_in, cd& out)
+template <class S>
+__device__ inline bool qnm_cw(S kappa, int l, int m_in, S& out_re, S& out_im)
 {
     int m = m_in < 0 ? -m_in : m_in;
     // amplitude / phase pairs per power of kappa; 'base' is the kappa^0 real offset for
@@ -397,32 +428,37 @@ __device__ inline bool qnm_cw(double kappa, int l, int m_in, cd& out)
         n = 5; base = 1.5; zd = true;
         for (int i = 0; i < 5; i++) { A[i] = a[i]; P[i] = p[i]; }
     } else {
-        out = mk(0.0, 0.0);
+        out_re = 0.0;
+        out_im = 0.0;
         return false;
     }
-    cd sum = mk(0.0, 0.0);
-    double kp = 1.0;
+    S sum_re = 0.0, sum_im = 0.0;
+    S kp = 1.0;
     for (int i = 0; i < n; i++) {
         double s, c;
         sincos(P[i], &s, &c);
-        sum += (A[i] * kp) * mk(c, s);
+        sum_re += (A[i] * kp) * c;
+        sum_im += (A[i] * kp) * s;
         kp *= kappa;
     }
-    cd ans = zd ? mk(base, 0.0) + kappa * sum : sum;
-    if (m_in < 0) ans = mk(-ans.re, ans.im);
-    out = ans;
+    S ans_re = zd ? base + kappa * sum_re : sum_re;
+    S ans_im = zd ? kappa * sum_im : sum_im;
+    if (m_in < 0) ans_re = -ans_re;
+    out_re = ans_re;
+    out_im = ans_im;
     return true;
 }
 
-__device__ inline bool hm_ringdown(int ell, int mm, double finmass, double finspin, double& frd, double& fdm)
+template <class S>
+__device__ inline bool hm_ringdown(int ell, int mm, S finmass, S finspin, S& frd, S& fdm)
 {
     int am = mm < 0 ? -mm : mm;
-    double kappa = pow(log(2.0 - finspin) / log(3.0), 1.0 / (2.0 + ell - am));
-    cd z;
-    bool ok = qnm_cw(kappa, ell, mm, z);
+    S kappa = pow(log(2.0 - finspin) / log(3.0), 1.0 / (2.0 + ell - am));
+    S zr, zi;
+    bool ok = qnm_cw(kappa, ell, mm, zr, zi);
     const double inv2pi = 0.5 / PI;
-    frd = inv2pi * z.re / finmass;
-    fdm = inv2pi * z.im / finmass;
+    frd = inv2pi * zr / finmass;
+    fdm = inv2pi * zi / finmass;
     return ok;
 }
 
@@ -431,14 +467,15 @@ __device__ inline bool hm_ringdown(int ell, int mm, double finmass, double finsp
 // TaylorF2 + PhenomD phase coefficients + C1 joins; amplitude coefficients when
 // 'with_amp' (PhenomHMWaveform.cu:253-335, :446-874, :1261-1339, :1587-1644).
 // The primary is enforced on the copies again, as the reference does.
-__device__ inline void setup_phenomd(PhmRec& r, double m1, double m2, double chi1, double chi2, double Rholm,
-                                     double Taulm, double fring, double fdamp, bool with_amp)
+template <class S>
+__device__ inline void setup_phenomd(PhmRecT<S>& r, nd<S> m1, nd<S> m2, nd<S> chi1, nd<S> chi2, nd<S> Rholm,
+                                     nd<S> Taulm, nd<S> fring, nd<S> fdamp, bool with_amp)
 {
     enforce_primary(m1, m2, chi1, chi2);
-    const double Mtot = m1 + m2;
-    const double eta = m1 * m2 / (Mtot * Mtot);
-    const double Seta = sqrt(1.0 - 4.0 * eta);
-    const double chi = (chi1 + chi2) / 2.0 * (1.0 - eta * 76.0 / 113.0) + Seta * ((chi1 - chi2) / 2.0);
+    const S Mtot = m1 + m2;
+    const S eta = m1 * m2 / (Mtot * Mtot);
+    const S Seta = sqrt(1.0 - 4.0 * eta);
+    const S chi = (chi1 + chi2) / 2.0 * (1.0 - eta * 76.0 / 113.0) + Seta * ((chi1 - chi2) / 2.0);
 
     r.p_eta = eta;
     r.p_etaInv = 1.0 / eta;
@@ -461,36 +498,36 @@ __device__ inline void setup_phenomd(PhmRec& r, double m1, double m2, double chi
 
     // TaylorF2 aligned-spin phasing minus the 3PN spin-spin term
     {
-        const double d = (m1 - m2) / (m1 + m2);
-        const double eta_pn = m1 * m2 / Mtot / Mtot;
-        const double m1M = m1 / Mtot, m2M = m2 / Mtot;
-        const double SL = m1M * m1M * chi1 + m2M * m2M * chi2;
-        const double dSigmaL = d * (m2M * chi2 - m1M * chi1);
-        const double pfaN = 3. / (128. * eta_pn);
-        const double e = eta_pn;
-        double v0 = 1. * pfaN;
-        double v2 = pfaN * (5. * (743. / 84. + 11. * e) / 9.);
-        double v3 = pfaN * (-16. * PI);
-        double v4 = pfaN * (5. * (3058.673 / 7.056 + 5429. / 7. * e + 617. * e * e) / 72.);
-        double v5 = pfaN * (5. / 9. * (7729. / 84. - 13. * e) * PI);
-        double vl5 = pfaN * (5. / 3. * (7729. / 84. - 13. * e) * PI);
-        double v6 = pfaN * ((11583.231236531 / 4.694215680 - 640. / 3. * PI * PI - 6848. / 21. * GAMMA_E) +
+        const S d = (m1 - m2) / (m1 + m2);
+        const S eta_pn = m1 * m2 / Mtot / Mtot;
+        const S m1M = m1 / Mtot, m2M = m2 / Mtot;
+        const S SL = m1M * m1M * chi1 + m2M * m2M * chi2;
+        const S dSigmaL = d * (m2M * chi2 - m1M * chi1);
+        const S pfaN = 3. / (128. * eta_pn);
+        const S e = eta_pn;
+        S v0 = 1. * pfaN;
+        S v2 = pfaN * (5. * (743. / 84. + 11. * e) / 9.);
+        S v3 = pfaN * (-16. * PI);
+        S v4 = pfaN * (5. * (3058.673 / 7.056 + 5429. / 7. * e + 617. * e * e) / 72.);
+        S v5 = pfaN * (5. / 9. * (7729. / 84. - 13. * e) * PI);
+        S vl5 = pfaN * (5. / 3. * (7729. / 84. - 13. * e) * PI);
+        S v6 = pfaN * ((11583.231236531 / 4.694215680 - 640. / 3. * PI * PI - 6848. / 21. * GAMMA_E) +
                             e * (-15737.765635 / 3.048192 + 2255. / 12. * PI * PI) + e * e * 76055. / 1728. -
                             e * e * e * 127825. / 1296.);
         v6 += pfaN * (-6848. / 21.) * 1.3862943611198906;  // log(4)
-        double vl6 = pfaN * (-6848. / 21.);
-        double v7 = pfaN * (PI * (77096675. / 254016. + 378515. / 1512. * e - 74045. / 756. * e * e));
-        double sig = e * (721. / 48. * chi1 * chi2 - 247. / 48. * (chi1 * chi2));
+        S vl6 = pfaN * (-6848. / 21.);
+        S v7 = pfaN * (PI * (77096675. / 254016. + 378515. / 1512. * e - 74045. / 756. * e * e));
+        S sig = e * (721. / 48. * chi1 * chi2 - 247. / 48. * (chi1 * chi2));
         sig += (720. - 1.) / 96.0 * m1M * m1M * chi1 * chi1;
         sig += (720. - 1.) / 96.0 * m2M * m2M * chi2 * chi2;
         sig -= (240. - 7.) / 96.0 * m1M * m1M * (chi1 * chi1);
         sig -= (240. - 7.) / 96.0 * m2M * m2M * (chi2 * chi2);
-        double ss1 = (4703.5 / 8.4 + 2935. / 6. * m1M - 120. * m1M * m1M) + (-4108.25 / 6.72 - 108.5 / 1.2 * m1M + 125.5 / 3.6 * m1M * m1M);
-        double ss2 = (4703.5 / 8.4 + 2935. / 6. * m2M - 120. * m2M * m2M) + (-4108.25 / 6.72 - 108.5 / 1.2 * m2M + 125.5 / 3.6 * m2M * m2M);
-        double ss3 = (326.75 / 1.12 + 557.5 / 1.8 * e) * e * chi1 * chi2;
+        S ss1 = (4703.5 / 8.4 + 2935. / 6. * m1M - 120. * m1M * m1M) + (-4108.25 / 6.72 - 108.5 / 1.2 * m1M + 125.5 / 3.6 * m1M * m1M);
+        S ss2 = (4703.5 / 8.4 + 2935. / 6. * m2M - 120. * m2M * m2M) + (-4108.25 / 6.72 - 108.5 / 1.2 * m2M + 125.5 / 3.6 * m2M * m2M);
+        S ss3 = (326.75 / 1.12 + 557.5 / 1.8 * e) * e * chi1 * chi2;
         ss3 += ss1 * m1M * m1M * (chi1 * chi1);
         ss3 += ss2 * m2M * m2M * (chi2 * chi2);
-        double gam = (554345. / 1134. + 110. * e / 9.) * SL + (13915. / 84. - 10. * e / 3.) * dSigmaL;
+        S gam = (554345. / 1134. + 110. * e / 9.) * SL + (13915. / 84. - 10. * e / 3.) * dSigmaL;
         v7 += pfaN * ((-8980424995. / 762048. + 6586595. * e / 756. - 305. * e * e / 36.) * SL -
                       (170978035. / 48384. - 2876425. * e / 672. - 4735. * e * e / 144.) * dSigmaL);
         v6 += pfaN * (PI * (3760. * SL + 1490. * dSigmaL) / 3. + ss3);
@@ -503,7 +540,7 @@ __device__ inline void setup_phenomd(PhmRec& r, double m1, double m2, double chi
         r.pn_v6 = v6; r.pn_v7 = v7; r.pn_vl5 = vl5; r.pn_vl6 = vl6;
     }
 
-    const P6 pp = pow6(PI);
+    const P6T<double> pp = pow6(PI);
     r.q_init = r.pn_v5 - PI_4;
     r.q_23 = r.pn_v7 * pp.two_thirds;
     r.q_13 = r.pn_v6 * pp.third;
@@ -523,19 +560,19 @@ __device__ inline void setup_phenomd(PhmRec& r, double m1, double m2, double chi
     r.C1Int = 0.0;
     r.C2Int = 0.0;
     {
-        const double ij = 1.0 / PHI_JOIN_INS;
-        double vj = cbrt(PI * PHI_JOIN_INS);
-        double dins = dphi_ins(r, vj, log(vj));
+        const S ij = 1.0 / PHI_JOIN_INS;
+        S vj = cbrt(PI * PHI_JOIN_INS);
+        S dins = dphi_ins(r, vj, log(vj));
         r.C2Int = dins - dphi_int(r, ij);
-        P6 pj = pow6(PHI_JOIN_INS);
-        double logvj = log(pj.third * 1.4645918875615231);
+        P6T<S> pj = pow6(S(PHI_JOIN_INS));
+        S logvj = log(pj.third * 1.4645918875615231);
         r.C1Int = phi_ins(r, PHI_JOIN_INS, pj, logvj, ij) - r.p_etaInv * phi_int(r, PHI_JOIN_INS, ij) -
                   r.C2Int * PHI_JOIN_INS;
-        const double fj = r.fMRDJoin;
-        const double ifj = 1.0 / fj;
-        double phi_int_j = r.p_etaInv * phi_int(r, fj, ifj) + r.C1Int + r.C2Int * fj;
-        double fj2 = fj * fj;
-        double dphi_int_j = r.C2Int + (r.beta1 + r.beta3 / (fj2 * fj2) + r.beta2 / fj) / r.p_eta;
+        const S fj = r.fMRDJoin;
+        const S ifj = 1.0 / fj;
+        S phi_int_j = r.p_etaInv * phi_int(r, fj, ifj) + r.C1Int + r.C2Int * fj;
+        S fj2 = fj * fj;
+        S dphi_int_j = r.C2Int + (r.beta1 + r.beta3 / (fj2 * fj2) + r.beta2 / fj) / r.p_eta;
         r.C2MRD = dphi_int_j - dphi_mrd(r, fj, ifj);
         r.C1MRD = phi_int_j - r.p_etaInv * phi_mrd(r, fj, ifj) - r.C2MRD * fj;
     }
@@ -556,19 +593,19 @@ __device__ inline void setup_phenomd(PhmRec& r, double m1, double m2, double chi
     r.a_rho2 = fit(F_RHO2, eta, chi);
     r.a_rho3 = fit(F_RHO3, eta, chi);
 
-    const double c12 = chi1 * chi1, c22 = chi2 * chi2, eta2 = eta * eta, eta3 = eta * eta2;
-    const double Pi = PI, Pi2 = pp.two;
+    const S c12 = chi1 * chi1, c22 = chi2 * chi2, eta2 = eta * eta, eta3 = eta * eta2;
+    const S Pi = PI, Pi2 = pp.two;
     r.a_amp0 = (sqrt(2.0 / 3.0) * sqrt(eta)) / pow(PI, 1. / 6.);
     r.a_pf23 = ((-969 + 1804 * eta) * pp.two_thirds) / 672.;
     r.a_pf1 = ((chi1 * (81 * (1 + Seta) - 44 * eta) + chi2 * (81 - 81 * Seta - 44 * eta)) * Pi) / 48.;
-    const double k43 = (-27312085.0 - 10287648 * c22 - 10287648 * c12 * (1 + Seta) + 10287648 * c22 * Seta +
+    const S k43 = (-27312085.0 - 10287648 * c22 - 10287648 * c12 * (1 + Seta) + 10287648 * c22 * Seta +
                         24 * (-1975055 + 857304 * c12 - 994896 * chi1 * chi2 + 857304 * c22) * eta + 35371056 * eta2);
     r.a_pf43 = (k43 * pp.four_thirds) / 8.128512e6;
-    const double k53 = (chi2 * (-285197 * (-1 + Seta) + 4 * (-91902 + 1579 * Seta) * eta - 35632 * eta2) +
+    const S k53 = (chi2 * (-285197 * (-1 + Seta) + 4 * (-91902 + 1579 * Seta) * eta - 35632 * eta2) +
                         chi1 * (285197 * (1 + Seta) - 4 * (91902 + 1579 * Seta) * eta - 35632 * eta2) +
                         42840 * (-1.0 + 4 * eta) * Pi);
     r.a_pf53 = (pp.five_thirds * k53) / 32256.;
-    const double k2 = (-336 * (-3248849057.0 + 2943675504 * c12 - 3339284256 * chi1 * chi2 + 2943675504 * c22) * eta2 -
+    const S k2 = (-336 * (-3248849057.0 + 2943675504 * c12 - 3339284256 * chi1 * chi2 + 2943675504 * c22) * eta2 -
                        324322727232 * eta3 -
                        7 * (-177520268561 + 107414046432 * c22 + 107414046432 * c12 * (1 + Seta) - 107414046432 * c22 * Seta +
                             11087290368 * (chi1 + chi2 + chi1 * Seta - chi2 * Seta) * Pi) +
@@ -578,24 +615,24 @@ __device__ inline void setup_phenomd(PhmRec& r, double m1, double m2, double chi
     r.a_pf2 = -(Pi2 * k2) / 6.0085960704e10;
 
     // intermediate amplitude by collocation (PhenomHMWaveform.cu:656-842)
-    const double f1 = AMP_JOIN_INS, f3 = r.a_fmax;
-    const double f2 = f1 + (f3 - f1) / 2.0;
-    const P6 p1 = pow6(f1);
-    const double v1 = amp_ins(r, f1, p1);
+    const S f1 = AMP_JOIN_INS, f3 = r.a_fmax;
+    const S f2 = f1 + (f3 - f1) / 2.0;
+    const P6T<S> p1 = pow6(S(f1));
+    const S v1 = amp_ins(r, f1, p1);
     // d/dMf of the inspiral ansatz at f1 (PhenomHMWaveform.cu:487-506)
-    const double d1 = ((-969 + 1804 * eta) * pow(Pi, 2.0 / 3.0)) / (1008. * pow(f1, 1.0 / 3.0)) +
+    const S d1 = ((-969 + 1804 * eta) * pow(Pi, 2.0 / 3.0)) / (1008. * pow(f1, 1.0 / 3.0)) +
                       ((chi1 * (81 * (1 + Seta) - 44 * eta) + chi2 * (81 - 81 * Seta - 44 * eta)) * Pi) / 48. +
                       (k43 * pow(f1, 1.0 / 3.0) * pow(Pi, 4.0 / 3.0)) / 6.096384e6 +
                       (5 * pow(f1, 2.0 / 3.0) * pow(Pi, 5.0 / 3.0) * k53) / 96768. -
                       (f1 * (Pi * Pi) * k2) / 3.0042980352e10 + (7.0 / 3.0) * pow(f1, 4.0 / 3.0) * r.a_rho1 +
                       (8.0 / 3.0) * pow(f1, 5.0 / 3.0) * r.a_rho2 + 3 * (f1 * f1) * r.a_rho3;
-    const double v3 = amp_mrd(r, f3);
-    const double d2 = amp_mrd_deriv(r, f3);
-    const double v2 = fit(F_V2, eta, chi);
-    const double f12 = f1 * f1, f13 = f1 * f12, f14 = f1 * f13, f15 = f1 * f14;
-    const double f22 = f2 * f2, f23 = f2 * f22, f24 = f2 * f23;
-    const double f32 = f3 * f3, f33 = f3 * f32, f34 = f3 * f33, f35 = f3 * f34;
-    const double den = ((f1 - f2) * (f1 - f2)) * ((f1 - f3) * (f1 - f3) * (f1 - f3)) * ((f3 - f2) * (f3 - f2));
+    const S v3 = amp_mrd(r, f3);
+    const S d2 = amp_mrd_deriv(r, f3);
+    const S v2 = fit(F_V2, eta, chi);
+    const S f12 = f1 * f1, f13 = f1 * f12, f14 = f1 * f13, f15 = f1 * f14;
+    const S f22 = f2 * f2, f23 = f2 * f22, f24 = f2 * f23;
+    const S f32 = f3 * f3, f33 = f3 * f32, f34 = f3 * f33, f35 = f3 * f34;
+    const S den = ((f1 - f2) * (f1 - f2)) * ((f1 - f3) * (f1 - f3) * (f1 - f3)) * ((f3 - f2) * (f3 - f2));
     r.a_d0 = -((d2 * f15 * f22 * f3 - 2 * d2 * f14 * f23 * f3 + d2 * f13 * f24 * f3 - d2 * f15 * f2 * f32 + d2 * f14 * f22 * f32 -
                 d1 * f13 * f23 * f32 + d2 * f13 * f23 * f32 + d1 * f12 * f24 * f32 - d2 * f12 * f24 * f32 + d2 * f14 * f2 * f33 +
                 2 * d1 * f13 * f22 * f33 - 2 * d2 * f13 * f22 * f33 - d1 * f12 * f23 * f33 + d2 * f12 * f23 * f33 -
@@ -651,18 +688,19 @@ __device__ inline void setup_phenomd(PhmRec& r, double m1, double m2, double chi
 }
 
 // HM frequency-map parameters (PhenomHMWaveform.cu:2107-2286)
-__device__ inline void hm_map(double Rholm, double Mf_RD_lm, double Mf_RD_22, int mm, double flm, bool for_amp,
-                              double& a, double& b, double& fi, double& fr)
+template <class S>
+__device__ inline void hm_map(S Rholm, S Mf_RD_lm, S Mf_RD_22, int mm, nd<S> flm, bool for_amp,
+                              S& a, S& b, S& fi, S& fr)
 {
-    double Mf_1_22 = for_amp ? AMP_JOIN_INS : PHI_JOIN_INS;
+    S Mf_1_22 = for_amp ? AMP_JOIN_INS : PHI_JOIN_INS;
     fi = Mf_1_22 / Rholm;
     fr = Mf_RD_lm;
-    double Ai = 2.0 / mm, Bi = 0.0;
-    double Trd = for_amp ? (fr - Mf_RD_lm + Mf_RD_22) : (Rholm * fr);
-    double Ti = 2.0 * fi / mm;
-    double Am = (Trd - Ti) / (fr - fi);
-    double Bm = Ti - fi * Am;
-    double Ar = 1.0, Br = 0.0;
+    S Ai = 2.0 / mm, Bi = 0.0;
+    S Trd = for_amp ? (fr - Mf_RD_lm + Mf_RD_22) : (Rholm * fr);
+    S Ti = 2.0 * fi / mm;
+    S Am = (Trd - Ti) / (fr - fi);
+    S Bm = Ti - fi * Am;
+    S Ar = 1.0, Br = 0.0;
     if (for_amp)
         Br = -Mf_RD_lm + Mf_RD_22;
     else
@@ -679,15 +717,16 @@ __device__ inline void hm_map(double Rholm, double Mf_RD_lm, double Mf_RD_22, in
 // Mf_RD_lm/Mf_DM_lm: this mode's QNM frequencies; Mf_RD_D/Mf_DM_D: the PhenomD
 // (2,2) ringdown frequencies from the QNM table (IMRPhenomHMCore, PhenomHMWaveform.cu:2671-2844).
 // Returns false for an unsupported mode.
-__device__ inline bool phm_setup(PhmRec& r, double m1_SI, double m2_SI, double chi1_in, double chi2_in,
-                                 double distance, int ell, int mm, double Mf_RD_lm, double Mf_DM_lm,
-                                 double Mf_RD_D, double Mf_DM_D, bool phenomd)
+template <class S>
+__device__ inline bool phm_setup(PhmRecT<S>& r, nd<S> m1_SI, nd<S> m2_SI, nd<S> chi1_in, nd<S> chi2_in,
+                                 nd<S> distance, int ell, int mm, nd<S> Mf_RD_lm, nd<S> Mf_DM_lm,
+                                 nd<S> Mf_RD_D, nd<S> Mf_DM_D, bool phenomd)
 {
-    double m1 = m1_SI / MSUN_SI, m2 = m2_SI / MSUN_SI, chi1 = chi1_in, chi2 = chi2_in;
+    S m1 = m1_SI / MSUN_SI, m2 = m2_SI / MSUN_SI, chi1 = chi1_in, chi2 = chi2_in;
     enforce_primary(m1, m2, chi1, chi2);
-    double finmass, finspin;
+    S finmass, finspin;
     final_state(m1, m2, chi1, chi2, finmass, finspin);
-    double Mf_RD_22, Mf_DM_22;
+    S Mf_RD_22, Mf_DM_22;
     bool ok = true;
     if (!phenomd)
         ok = hm_ringdown(2, 2, finmass, finspin, Mf_RD_22, Mf_DM_22);
@@ -705,11 +744,11 @@ __device__ inline bool phm_setup(PhmRec& r, double m1_SI, double m2_SI, double c
     // (2,2) PhenomD: f_ref -> amplitude peak, phi0 and t0 there (PhenomHMWaveform.cu:2752-2784)
     setup_phenomd(r, m1, m2, chi1, chi2, 1.0, 1.0, Mf_RD_D, Mf_DM_D, true);
     r.Mf_ref = r.a_fmax;
-    double ph22, dph22;
+    S ph22, dph22;
     phase_imr(r, r.Mf_ref, ph22, &dph22);
     r.phi0 = 0.5 * (ph22 + 0.0);
     r.t0 = dph22;
-    const double Mtot = (m1_SI + m2_SI) / MSUN_SI;
+    const S Mtot = (m1_SI + m2_SI) / MSUN_SI;
     r.amp_scale = Mtot * MRSUN_SI * Mtot * MTSUN_SI / distance;
     r.M_sec = (m1 + m2) * MTSUN_SI;
     const double cs[7] = {0.0, PI_2, 0.0, -PI_2, PI, PI_2, 0.0};
@@ -717,17 +756,17 @@ __device__ inline bool phm_setup(PhmRec& r, double m1_SI, double m2_SI, double c
 
     // PN amplitude-ratio inputs (IMRPhenomHMOnePointFiveSpinPN with the enforced masses/spins)
     {
-        double Min = m1 + m2;
-        double M1 = m1 / Min, M2 = m2 / Min;
-        double M = M1 + M2;
+        S Min = m1 + m2;
+        S M1 = m1 / Min, M2 = m2 / Min;
+        S M = M1 + M2;
         r.pn_M = M;
         r.pn_eta = M1 * M2 / (M * M);
         r.pn_delta = sqrt(1.0 - 4 * r.pn_eta);
         r.pn_Xs = 0.5 * (chi1 + chi2);
         r.pn_Xa = 0.5 * (chi1 - chi2);
         r.pn_pref = M * M * PI * sqrt(r.pn_eta * 2.0 / 3);
-        r.pn_vscale = M * 2.0 * PI / (double)r.mm;
-        r.pn_v2ratio = cbrt(2.0 / (double)r.mm);
+        r.pn_vscale = M * 2.0 * PI / (S)r.mm;
+        r.pn_v2ratio = cbrt(2.0 / (S)r.mm);
     }
 
     if (phenomd) {
@@ -738,13 +777,13 @@ __device__ inline bool phm_setup(PhmRec& r, double m1_SI, double m2_SI, double c
     }
 
     // this mode's ratios and the PhenomD phase re-joined with them
-    const double Rholm = Mf_RD_22 / Mf_RD_lm;
-    const double Taulm = Mf_DM_lm / Mf_DM_22;
+    const S Rholm = Mf_RD_22 / Mf_RD_lm;
+    const S Taulm = Mf_DM_lm / Mf_DM_22;
     setup_phenomd(r, m1, m2, chi1, chi2, Rholm, Taulm, Mf_RD_D, Mf_DM_D, false);
 
     // phase map + constants (IMRPhenomHMPhasePreComp, PhenomHMWaveform.cu:2317-2369)
     const double shift = 0.0001;
-    double fi, fr;
+    S fi, fr;
     hm_map(Rholm, Mf_RD_lm, Mf_RD_22, mm, shift, false, r.ai, r.bi, fi, fr);
     hm_map(Rholm, Mf_RD_lm, Mf_RD_22, mm, fi + shift, false, r.am, r.bm, fi, fr);
     hm_map(Rholm, Mf_RD_lm, Mf_RD_22, mm, fr + shift, false, r.ar, r.br, fi, fr);
@@ -757,13 +796,13 @@ __device__ inline bool phm_setup(PhmRec& r, double m1_SI, double m2_SI, double c
 
     // amplitude map (region boundaries and linear pieces)
     {
-        double a, b, afi, afr;
+        S a, b, afi, afr;
         hm_map(Rholm, Mf_RD_lm, Mf_RD_22, mm, 0.0, true, a, b, afi, afr);
         r.amp_fi = afi;
         r.amp_fr = afr;
         r.amp_Ai = 2.0 / mm;
-        double Trd = afr - Mf_RD_lm + Mf_RD_22;
-        double Ti = 2.0 * afi / mm;
+        S Trd = afr - Mf_RD_lm + Mf_RD_22;
+        S Ti = 2.0 * afi / mm;
         r.amp_Am = (Trd - Ti) / (afr - afi);
         r.amp_Bm = Ti - afi * r.amp_Am;
         r.amp_Br = -Mf_RD_lm + Mf_RD_22;
@@ -773,30 +812,31 @@ __device__ inline bool phm_setup(PhmRec& r, double m1_SI, double m2_SI, double c
 
 // Amplitude, phase and t(f) of one (binary, mode) at frequency f in Hz
 // (calculate_modes / calculate_modes_phenomd, PhenomHMWaveform.cu:2461-2662).
-__device__ inline void phm_eval(const PhmRec& r, double f, double& amp, double& phase, double& tf)
+template <class S>
+__device__ inline void phm_eval(const PhmRecT<S>& r, nd<S> f, S& amp, S& phase, S& tf)
 {
     // A PhenomD-only record (run_phenomd) carries identity maps (a = 1, b = 0,
     // breakpoints at 1e300) and l = m = 2, so it takes the same path: every map
     // and offset below is then exact, and the amplitude factor is 1.
-    const double Mf_wf = f * r.M_sec;
+    const S Mf_wf = f * r.M_sec;
 
     // amplitude
-    double a, b;
+    S a, b;
     if (Mf_wf > r.amp_fi) {
         if (Mf_wf > r.amp_fr) { a = 1.0; b = r.amp_Br; }
         else { a = r.amp_Am; b = r.amp_Bm; }
     } else {
         a = r.amp_Ai; b = 0.0;
     }
-    const double Mf_amp = a * Mf_wf + b;
-    P6 p = pow6(Mf_amp);
-    double A = amp_imr(r, Mf_amp, p);
+    const S Mf_amp = a * Mf_wf + b;
+    P6T<S> p = pow6(Mf_amp);
+    S A = amp_imr(r, Mf_amp, p);
     A *= hm_amp_factor(r, Mf_wf, Mf_amp);
     amp = A * r.amp_scale;
 
     // phase and t(f): one of three linear frequency maps, then one PhenomD phase
     // evaluation; phi = ph / a - c1 + c2 with c1 = c2 = 0 on the first piece
-    double sa, sb, c1, c2;
+    S sa, sb, c1, c2;
     if (!(Mf_wf > r.fi)) {
         sa = r.ai; sb = r.bi; c1 = 0.0; c2 = 0.0;
     } else if (!(Mf_wf > r.fr)) {
@@ -804,9 +844,9 @@ __device__ inline void phm_eval(const PhmRec& r, double f, double& amp, double& 
     } else {
         sa = r.ar; sb = r.br; c1 = r.PhDCconst; c2 = r.phaseC;
     }
-    double ph, dph;
+    S ph, dph;
     phase_imr(r, sa * Mf_wf + sb, ph, &dph);
-    const double phi = r.cshift + ((ph * rcp_pos(sa) - c1) + c2);
+    const S phi = r.cshift + ((ph * rcp_pos(sa) - c1) + c2);
     phase = -r.t0 * (Mf_wf - r.Mf_ref) + (phi - (r.mm * r.phi0));
     const double INV_TWO_PI = 0.15915494309189535;
     tf = (r.M_sec * (dph * INV_TWO_PI)) - ((r.t0 * INV_TWO_PI) * r.M_sec);

@@ -26,6 +26,7 @@
 #pragma once
 
 #include "common.h"
+#include "dual.h"
 
 namespace bbhx {
 namespace resp {
@@ -46,11 +47,13 @@ struct OrbitTab {
     double inv_L;          // 1 / L
 };
 
-// spin-weight -2 spherical harmonic for the PhenomHM (l, m) (Response.cu:383-424)
-__device__ inline cd sYlm(int l, int m, double theta, double phi)
+// Real factor of the spin-weight -2 spherical harmonic -2Y_lm(theta, phi) =
+// fac(theta) e^{i m phi} for the PhenomHM (l, m) (Response.cu:383-424).
+template <class S>
+__device__ inline S ylm_fac(int l, int m, S theta)
 {
-    double fac = 0.0;
-    double c = cos(theta), ch = cos(theta / 2.0), sh = sin(theta / 2.0), st = sin(theta);
+    S fac = 0.0;
+    const S c = cos(theta), ch = cos(theta / 2.0), sh = sin(theta / 2.0), st = sin(theta);
     if (l == 2 && m == -2) fac = sqrt(5.0 / (64.0 * PI)) * (1.0 - c) * (1.0 - c);
     else if (l == 2 && m == -1) fac = sqrt(5.0 / (16.0 * PI)) * st * (1.0 - c);
     else if (l == 2 && m == 1) fac = sqrt(5.0 / (16.0 * PI)) * st * (1.0 + c);
@@ -63,6 +66,12 @@ __device__ inline cd sYlm(int l, int m, double theta, double phi)
     else if (l == 4 && m == -3) fac = 3.0 * sqrt(7.0 / (2.0 * PI)) * ch * (1.0 + 2.0 * c) * pow(sh, 5.0);
     else if (l == 4 && m == 3) fac = -3.0 * sqrt(7.0 / (2.0 * PI)) * pow(ch, 5.0) * (-1.0 + 2.0 * c) * sh;
     else if (l == 4 && m == 4) fac = 3.0 * sqrt(7.0 / PI) * pow(ch, 6.0) * pow(sh, 2.0);
+    return fac;
+}
+
+__device__ inline cd sYlm(int l, int m, double theta, double phi)
+{
+    const double fac = ylm_fac(l, m, theta);
     if (m == 0) return mk(fac, 0.0);
     return fac * cexpi(m * phi);
 }
@@ -141,7 +150,7 @@ __device__ inline void resp_eval(const RespRec& r, const OrbitTab& o, double f, 
 
     // link j = 0, 1, 2 is n1 (12), n2 (23), n3 (31); n_j^T H n_j
     cd nHn[3];
-    double sp[3], sm[3];  // sinc(x (1 - kn)), sinc(x (1 + kn))
+    double sp[3], sm[3];  // sm: sinc(x (1 - kn)), sp: sinc(x (1 + kn))
     for (int j = 0; j < 3; j++) {
         nHn[j] = r.Yp * (un[j] * un[j] - vn[j] * vn[j]) + r.Yc * (2.0 * un[j] * vn[j]);
         double s, c;

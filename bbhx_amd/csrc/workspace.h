@@ -19,7 +19,8 @@ enum WsSlot : int {
     WS_TABLE = 2,
     WS_PARTIAL = 3,
     WS_FUSED = 4,
-    WS_SLOTS = 5,
+    WS_GRAD = 5,
+    WS_SLOTS = 6,
 };
 
 // Returns a device pointer with at least 'bytes' bytes on the current device,

@@ -72,6 +72,16 @@ def load_library():
         "bbhx_hip_hdyn_fused": [_vp, _vp, _vp, _vp, _vp, C.c_int, _ip, _ip, C.c_int, _vp, _vp, _vp, C.c_int, _vp,
                                 C.c_int, C.c_double, C.c_double, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                 _vp],
+        "bbhx_hip_hdyn_fused_grad": [_vp, _vp, _vp, _vp, _vp, _vp, C.c_int, _ip, _ip, C.c_int, _vp, _vp, _vp,
+                                     C.c_int, _vp, C.c_int, C.c_double, C.c_double, _vp, C.c_int, C.c_int, C.c_int,
+                                     C.c_int, _vp],
+        "bbhx_hip_direct_plan_create": [C.POINTER(_vp), C.c_int, _ip, C.c_int, _vp, C.c_int, C.c_int, C.c_int,
+                                        C.c_int],
+        "bbhx_hip_direct_plan_destroy": [_vp],
+        "bbhx_hip_direct_plan_bytes": [_vp],
+        "bbhx_hip_direct_fused": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _ip, _ip, C.c_int, _vp, _vp, _vp, _vp, C.c_int,
+                                  C.c_double, C.c_double, C.c_double, C.c_double, _vp, C.c_int, C.c_int, C.c_int,
+                                  _vp],
         "bbhx_hip_release_workspace": [],
         "bbhx_hip_version": [],
         "bbhx_hip_timing_enable": [C.c_int],
@@ -80,7 +90,7 @@ def load_library():
     for name, args in sigs.items():
         fn = getattr(lib, name)
         fn.argtypes = args
-        fn.restype = C.c_char_p if name == "bbhx_hip_version" else C.c_int
+        fn.restype = {"bbhx_hip_version": C.c_char_p, "bbhx_hip_direct_plan_bytes": C.c_long}.get(name, C.c_int)
     _lib = lib
     return lib
 
@@ -92,6 +102,8 @@ def exported_symbols():
         "bbhx_hip_get_phenomd_ringdown_frequencies", "bbhx_hip_response_create", "bbhx_hip_response_destroy",
         "bbhx_hip_response_add_orbits", "bbhx_hip_response_compute", "bbhx_hip_interpolate", "bbhx_hip_InterpTDI",
         "bbhx_hip_direct_sum", "bbhx_hip_hdyn", "bbhx_hip_direct_like", "bbhx_hip_hdyn_fused",
+        "bbhx_hip_hdyn_fused_grad", "bbhx_hip_direct_plan_create", "bbhx_hip_direct_plan_destroy",
+        "bbhx_hip_direct_plan_bytes", "bbhx_hip_direct_fused",
         "bbhx_hip_release_workspace", "bbhx_hip_version", "bbhx_hip_timing_enable", "bbhx_hip_timing_read",
     ]
 
@@ -311,6 +323,86 @@ def hdyn_fused(d_h, h_h, params, t_start, t_end, ells, mms, freqs, h0, data_cons
     _check(st, "hdyn_fused")
 
 
+def hdyn_fused_grad(d_h, h_h, grad, params, t_start, t_end, ells, mms, freqs, h0, data_constants, qnm, dspin,
+                    initial_t_val, response, TDItag, rescaled, tdi2, include_last_segment=True):
+    """Heterodyned likelihood and its gradient (11, B) (no reference counterpart; see include/bbhx_hip.h)."""
+    torch = _torch()
+    f64, c128 = torch.float64, torch.complex128
+    B = int(params.shape[1])
+    K = int(freqs.numel())
+    el, elp = _host_i32(ells)
+    mm, mmp = _host_i32(mms)
+    table_len = int(qnm.numel()) // 8
+    if tuple(grad.shape) != (11, B):
+        raise ValueError("grad must be (11, B)")
+    st = load_library().bbhx_hip_hdyn_fused_grad(
+        _dev(d_h, c128), _dev(h_h, c128), _dev(grad, f64, "grad"), _dev(params, f64, "params"), _dev(t_start, f64),
+        _dev(t_end, f64), B, elp, mmp, int(el.size), _dev(freqs, f64), _dev(h0, c128, "h0"),
+        _dev(data_constants, c128), K, _dev(qnm, f64), table_len, float(dspin), float(initial_t_val),
+        response.handle, int(TDItag), int(bool(rescaled)), int(bool(tdi2)), int(bool(include_last_segment)),
+        _stream())
+    _check(st, "hdyn_fused_grad")
+
+
+class DirectPlan:
+    """Sizes and device buffers of the fused direct likelihood of a ragged batch
+    (bbhx_hip_direct_plan_*). lengths: (B,) sparse grid lengths."""
+
+    def __init__(self, lengths, num_modes, data_length, chunk=16384, block=256, grid=None):
+        torch = _torch()
+        lib = load_library()
+        lens, lp = _host_i32(lengths)
+        if grid is None:
+            grid = 8 * torch.cuda.get_device_properties(torch.cuda.current_device()).multi_processor_count
+        self.lengths = lens.copy()
+        self.num_modes, self.data_length = int(num_modes), int(data_length)
+        self.chunk, self.block, self.grid = int(chunk), int(block), int(grid)
+        h = _vp()
+        _check(lib.bbhx_hip_direct_plan_create(C.byref(h), int(lens.size), lp, self.num_modes, None,
+                                               self.data_length, self.chunk, self.block, self.grid),
+               "direct_plan_create")
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    @property
+    def num_bin_all(self):
+        return int(self.lengths.size)
+
+    def nbytes(self):
+        return int(load_library().bbhx_hip_direct_plan_bytes(self._h))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                load_library().bbhx_hip_direct_plan_destroy(h)
+            except Exception:  # pragma: no cover - interpreter shutdown
+                pass
+            self._h = None
+
+
+def direct_fused(plan, d_h, h_h, flags, params, t_start, t_end, ells, mms, data_freqs, data_channels, noise_factors,
+                 qnm, dspin, initial_t_val, mf_min, mf_max, response, TDItag, rescaled, tdi2):
+    """Fused direct likelihood of a ragged batch (no reference counterpart; see include/bbhx_hip.h)."""
+    torch = _torch()
+    f64, c128 = torch.float64, torch.complex128
+    el, elp = _host_i32(ells)
+    mm, mmp = _host_i32(mms)
+    B = plan.num_bin_all
+    if tuple(params.shape) != (11, B):
+        raise ValueError("params must be (11, B) for this plan")
+    st = load_library().bbhx_hip_direct_fused(
+        plan.handle, _dev(d_h, c128), _dev(h_h, c128), _dev(flags, torch.int32, "flags"), _dev(params, f64, "params"),
+        _dev(t_start, f64), _dev(t_end, f64), elp, mmp, int(el.size), _dev(data_freqs, f64),
+        _dev(data_channels, c128), _dev(noise_factors, f64), _dev(qnm, f64), int(qnm.numel()) // 8, float(dspin),
+        float(initial_t_val), float(mf_min), float(mf_max), response.handle, int(TDItag), int(bool(rescaled)),
+        int(bool(tdi2)), _stream())
+    _check(st, "direct_fused")
+
+
 def timing_enable(on=True):
     """Start (or stop) event timing of the fused heterodyne launches."""
     _check(load_library().bbhx_hip_timing_enable(int(bool(on))), "timing_enable")
@@ -369,6 +461,9 @@ class BBHxHipBackend:
         for f in dataclasses.fields(m):
             setattr(self, f.name, getattr(m, f.name))
         self.hdyn_fused = hdyn_fused
+        self.hdyn_fused_grad = hdyn_fused_grad
+        self.direct_fused = direct_fused
+        self.DirectPlan = DirectPlan
         self.device = torch.device("cuda", torch.cuda.current_device())
 
     @property

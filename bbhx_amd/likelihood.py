@@ -11,6 +11,7 @@ r = h/h0 -> hdyn).
 from __future__ import annotations
 
 import numpy as np
+import torch
 
 from .utils.arrays import as_np1d, to_dev, to_host
 from .utils.parallelbase import BBHxParallelModule
@@ -65,6 +66,83 @@ class Likelihood(BBHxParallelModule):
             snr = (d_h_temp.real / torch.sqrt(self.h_h.real)).cpu().numpy()
             return np.array([out, snr]).T
         return out
+
+
+    # ---------------------------------------------------------- fused ragged path
+
+    def direct_plan(self, lengths, modes=None, chunk=16384, block=256, grid=None):
+        """Plan (sizes + device buffers) of the fused direct likelihood for a
+        batch whose binary b has a sparse grid of lengths[b] points."""
+        gen = self.waveform_gen
+        M = len(modes) if modes is not None else len(gen.amp_phase_gen.allowable_modes)
+        return self.backend.DirectPlan(lengths, M, self.data_stream_length, chunk=chunk, block=block, grid=grid)
+
+    def get_ll_fused_device(self, plan, rows, t_start, t_end, flags, modes=None):
+        """Fused direct log-likelihood of device inputs (``fused_inputs``): a
+        device tensor, with no host synchronisation (graph-capturable; flags
+        must be zeroed before the call)."""
+        torch = self.xp
+        gen = self.waveform_gen
+        if modes is None:
+            modes = gen.amp_phase_gen.allowable_modes
+        ells = np.asarray([l for l, m in modes], dtype=np.int32)
+        mms = np.asarray([m for l, m in modes], dtype=np.int32)
+        B = plan.num_bin_all
+        if not hasattr(self, "_dfused_out") or self._dfused_out[0].numel() != B:
+            self._dfused_out = (torch.empty(B, dtype=torch.complex128, device=self.device),
+                                torch.empty(B, dtype=torch.complex128, device=self.device))
+        d_h, h_h = self._dfused_out
+        ap, resp = gen.amp_phase_gen, gen.response_gen
+        self.backend.direct_fused(plan, d_h, h_h, flags, rows, t_start, t_end, ells, mms, self.data_freqs,
+                                  self.data_channels, self.noise_factors, ap.qnm_packed, ap.dspin, ap.initial_t_val,
+                                  ap.mf_min, ap.mf_max, resp.cpp_response, resp.TDItag_int, resp.rescaled, resp.tdi2)
+        self.d_h, self.h_h = d_h, h_h
+        return -1 / 2 * (self.d_d + h_h - 2 * d_h).real
+
+    def get_ll_fused(self, params, lengths, return_extracted_snr=False, phase_marginalize=False, plan=None,
+                     modes=None, t_obs_start=0.0, t_obs_end=1.0):
+        """Direct log-likelihood of templates with per-template sparse grid
+        lengths (``lengths[b]``), fused: no template is materialised. Same
+        params layout and result as ``get_ll(params, length=...)``."""
+        torch = self.xp
+        rows, t_start, t_end = fused_param_rows(self.waveform_gen, params, t_obs_start, t_obs_end, self.device)
+        lengths = np.asarray(lengths, dtype=np.int32)
+        if lengths.shape != (rows.shape[1],):
+            raise ValueError("lengths must have one entry per template")
+        if plan is None:
+            plan = self.direct_plan(lengths, modes)
+        flags = torch.zeros(rows.shape[1], dtype=torch.int32, device=self.device)
+        ll = self.get_ll_fused_device(plan, rows, t_start, t_end, flags, modes)
+        fl = flags.cpu().numpy()
+        if np.any(fl & 1):
+            raise ValueError("The search for a window at x_min did not run correctly. If t_ref is close to zero, "
+                             "the time-frequency correspondence may be crossing zero twice.")
+        if np.any(fl & 2):
+            raise ValueError("The search for a window at x_max did not run correctly. If t_ref is close to the "
+                             "observation time, the time-frequency correspondence may be crossing zero twice.")
+        d_h_temp = self.d_h if not phase_marginalize else torch.abs(self.d_h)
+        out = (-1 / 2 * (self.d_d + self.h_h - 2 * d_h_temp).real).cpu().numpy()
+        if return_extracted_snr:
+            snr = (d_h_temp.real / torch.sqrt(self.h_h.real)).cpu().numpy()
+            return np.array([out, snr]).T
+        return out
+
+
+def fused_param_rows(template_gen, params, t_obs_start, t_obs_end, device):
+    """Device inputs of the fused kernels: (11, B) parameter rows (m1 >= m2
+    enforced as phenomhm.py:175-199 does; f_ref dropped), t_start, t_end
+    (SSB-frame window, waveformbuild.py:477-484)."""
+    p = np.atleast_2d(np.asarray(to_host(params), dtype=np.float64))
+    if p.shape[0] != 12:
+        p = p.T
+    if p.shape[0] != 12:
+        raise ValueError("params must have 12 rows (m1, m2, chi1z, chi2z, distance, phi_ref, f_ref, inc, lam, "
+                         "beta, psi, t_ref)")
+    m1, m2, c1, c2 = p[0].copy(), p[1].copy(), p[2].copy(), p[3].copy()
+    template_gen.amp_phase_gen._sanity_check_params(m1, m2, c1, c2)
+    rows = np.stack([m1, m2, c1, c2, p[4], p[5], p[7], p[8], p[9], p[10], p[11]])
+    t_start, t_end = template_gen.time_window(p[8], p[9], t_obs_start, t_obs_end)
+    return to_dev(rows, device), to_dev(t_start, device), to_dev(t_end, device)
 
 
 class HeterodynedLikelihood(BBHxParallelModule):
@@ -224,18 +302,8 @@ class HeterodynedLikelihood(BBHxParallelModule):
         """Device inputs of the fused kernel: (11, B) parameters (m1 >= m2
         enforced as phenomhm.py:175-199 does), t_start, t_end."""
         waveform_kwargs = {} if waveform_kwargs is None else waveform_kwargs
-        p = np.atleast_2d(np.asarray(to_host(params), dtype=np.float64))
-        if p.shape[0] != 12:
-            p = p.T
-        if p.shape[0] != 12:
-            raise ValueError("params must have 12 rows (m1, m2, chi1z, chi2z, distance, phi_ref, f_ref, inc, lam, "
-                             "beta, psi, t_ref)")
-        m1, m2, c1, c2 = p[0].copy(), p[1].copy(), p[2].copy(), p[3].copy()
-        self.template_gen.amp_phase_gen._sanity_check_params(m1, m2, c1, c2)
-        rows = np.stack([m1, m2, c1, c2, p[4], p[5], p[7], p[8], p[9], p[10], p[11]])
-        t_start, t_end = self.template_gen.time_window(p[8], p[9], waveform_kwargs.get("t_obs_start", 0.0),
-                                                       waveform_kwargs.get("t_obs_end", 1.0))
-        return (to_dev(rows, self.device), to_dev(t_start, self.device), to_dev(t_end, self.device))
+        return fused_param_rows(self.template_gen, params, waveform_kwargs.get("t_obs_start", 0.0),
+                                waveform_kwargs.get("t_obs_end", 1.0), self.device)
 
     def get_ll_device(self, rows, t_start, t_end, precision=0, modes=None):
         """Fused log-likelihood of device-resident inputs (from ``fused_inputs``);
@@ -267,3 +335,85 @@ class HeterodynedLikelihood(BBHxParallelModule):
     def _fused_inner_products(self, params, waveform_kwargs, precision=0):
         rows, t_start, t_end = self.fused_inputs(params, waveform_kwargs)
         return self._fused_core(rows, t_start, t_end, waveform_kwargs.get("modes"), precision)
+
+    # ---------------------------------------------------------------- gradient
+
+    GRAD_COORDS = ("lnM", "q", "chi1z", "chi2z", "distance", "phi_ref", "cos_inc", "lam", "sin_beta", "psi", "t_ref")
+
+    @staticmethod
+    def sampling_to_params(x):
+        """(B, 11) sampling coordinates (ln(m1 + m2), q = m1/m2 >= 1, chi1z,
+        chi2z, distance [m], phi_ref, cos inc, lam, sin beta, psi, t_ref [s]) ->
+        the (12, B) parameter rows of ``get_ll`` (f_ref = 0)."""
+        x = np.atleast_2d(np.asarray(to_host(x), dtype=np.float64))
+        if x.shape[1] != 11:
+            raise ValueError("x must be (B, 11)")
+        M = np.exp(x[:, 0])
+        q = x[:, 1]
+        out = np.zeros((12, x.shape[0]))
+        out[0] = M * q / (1 + q)
+        out[1] = M / (1 + q)
+        out[2], out[3], out[4], out[5] = x[:, 2], x[:, 3], x[:, 4], x[:, 5]
+        out[7] = np.arccos(x[:, 6])
+        out[8] = x[:, 7]
+        out[9] = np.arcsin(x[:, 8])
+        out[10], out[11] = x[:, 9], x[:, 10]
+        return out
+
+    def get_ll_grad_device(self, rows, t_start, t_end, modes=None):
+        """log-likelihood (B,) and its gradient (11, B) in the sampling
+        coordinates, from device inputs of ``fused_inputs``; device tensors."""
+        torch = self.xp
+        gen = self.template_gen
+        if modes is None:
+            modes = self.template_gen_kwargs.get("modes") or gen.amp_phase_gen.allowable_modes
+        ells = np.asarray([l for l, m in modes], dtype=np.int32)
+        mms = np.asarray([m for l, m in modes], dtype=np.int32)
+        B = rows.shape[1]
+        d_h = torch.empty(B, dtype=torch.complex128, device=self.device)
+        h_h = torch.empty(B, dtype=torch.complex128, device=self.device)
+        grad = torch.empty((11, B), dtype=torch.float64, device=self.device)
+        resp = gen.response_gen
+        if not hasattr(self, "_h0_flat"):
+            self._h0_flat = self.h0_sparse.reshape(3, -1).contiguous()
+        self.backend.hdyn_fused_grad(d_h, h_h, grad, rows, t_start, t_end, ells, mms, self.freqs, self._h0_flat,
+                                     self.data_constants, gen.amp_phase_gen.qnm_packed, gen.amp_phase_gen.dspin,
+                                     gen.amp_phase_gen.initial_t_val, resp.cpp_response, resp.TDItag_int,
+                                     resp.rescaled, resp.tdi2, self.include_last_segment)
+        gen.num_bin_all = B
+        self.hdyn_d_h, self.hdyn_h_h = d_h, h_h
+        return -1 / 2.0 * (self.reference_d_d + h_h - 2 * d_h).real, grad
+
+    def get_ll_and_grad(self, x, **waveform_kwargs):
+        """log-likelihood (B,) and gradient (B, 11) at sampling coordinates x
+        (B, 11) (see ``sampling_to_params``); numpy arrays."""
+        waveform_kwargs = self._merge_kwargs(dict(waveform_kwargs))
+        rows, t_start, t_end = self.fused_inputs(self.sampling_to_params(x), waveform_kwargs)
+        ll, grad = self.get_ll_grad_device(rows, t_start, t_end, waveform_kwargs.get("modes"))
+        return ll.cpu().numpy(), grad.T.cpu().numpy()
+
+    def loglike(self, x):
+        """Differentiable log-likelihood: x is a (B, 11) float64 torch tensor of
+        sampling coordinates; returns (B,) with a backward pass through the
+        adjoint kernel (``HeterodynedLogLike``)."""
+        return HeterodynedLogLike.apply(x, self)
+
+
+class HeterodynedLogLike(torch.autograd.Function):
+    """torch.autograd.Function over the fused heterodyned log-likelihood:
+    forward runs the gradient kernel (value and gradient in one pass) and keeps
+    the (B, 11) gradient for backward. Arguments: x (B, 11) sampling
+    coordinates (float64 tensor), the HeterodynedLikelihood."""
+
+    @staticmethod
+    def forward(ctx, x, like):
+        dev = x.device
+        ll, grad = like.get_ll_and_grad(x.detach().cpu().numpy())
+        g = torch.as_tensor(grad, dtype=x.dtype, device=dev)
+        ctx.save_for_backward(g)
+        return torch.as_tensor(ll, dtype=x.dtype, device=dev)
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        (g,) = ctx.saved_tensors
+        return grad_out[:, None] * g, None

@@ -152,6 +152,24 @@ def main():
     log(f"fused FP64: {ms_per_step:.3f} ms/step, {value:.4g} evals/s; setup {ms_setup / max(ncalls, 1):.3f} ms, "
         f"eval {ms_eval / max(ncalls, 1):.3f} ms per call")
 
+    # gradient (11 parameters) of the same log-likelihoods: adjoint kernel
+    for _ in range(max(args.warmup, 1)):
+        het.get_ll_grad_device(rows, t_start, t_end)
+    torch.cuda.synchronize()
+    timing_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ll_g, grad = het.get_ll_grad_device(rows, t_start, t_end)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    gms_setup, gms_eval, gn = timing_read()
+    timing_enable(False)
+    grad_rate = B / ((t1 - t0) / args.steps)
+    grad_ll_diff = float(torch.max(torch.abs(ll_g - ll_dev)).item())
+    log(f"gradient: {(t1 - t0) * 1e3 / args.steps:.3f} ms/step, {grad_rate:.4g} grad evals/s "
+        f"({value / grad_rate:.2f}x the forward time); setup {gms_setup / max(gn, 1):.3f} ms, "
+        f"eval {gms_eval / max(gn, 1):.3f} ms; max |logL(grad kernel) - logL(forward)| = {grad_ll_diff:.2e}")
+
     # the kernel-by-kernel FP64 path (the reference's structure) on the same templates
     t0 = time.perf_counter()
     n_plain = 3
@@ -205,7 +223,8 @@ def main():
         "config": {"workload": "heterodyned logL, fused FP64", "templates": B, "sparse_nodes": args.nodes,
                    "dense_bins": 2**args.dense_bits, "modes": 6, "tdi": "AET tdi2",
                    "plain_fp64_path_evals_per_s": plain_rate, "max_abs_dlogL_vs_cpu_oracle": dll,
-                   "grad_evals_per_s": None},
+                   "grad_evals_per_s": grad_rate, "grad_cost_vs_forward": value / grad_rate,
+                   "grad_params": "lnM, q, chi1, chi2, dist, phi_ref, cos inc, lam, sin beta, psi, t_ref"},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

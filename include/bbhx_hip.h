@@ -167,8 +167,58 @@ int bbhx_hip_hdyn_fused(double* d_h, double* h_h, const double* params, const do
                         const bbhx_response* orbits, int TDItag, int rescaled, int tdi2,
                         int include_last_segment, int precision, void* stream);
 
-/* Event timing of bbhx_hip_hdyn_fused's two launches (setup, evaluation) on the
- * caller's stream: enable(1) starts recording (and clears earlier records);
+/* Gradient of the heterodyned log-likelihood (no counterpart in the reference,
+ * which has no derivatives). Same inputs as bbhx_hip_hdyn_fused (precision is
+ * FP64); computes d_h, h_h as it does and grad (11, B): the derivatives of
+ * logL = Re d_h - 1/2 Re h_h (- 1/2 d_d) with respect to ln(m1 + m2), q = m1/m2,
+ * chi1z, chi2z, distance [m], phi_ref, cos(inc), lam, sin(beta), psi, t_ref [s]
+ * (row order). The observation window t_start/t_end is held fixed. K <= 512. */
+int bbhx_hip_hdyn_fused_grad(double* d_h, double* h_h, double* grad, const double* params, const double* t_start,
+                             const double* t_end, int numBinAll, const int* ells_host, const int* mms_host,
+                             int numModes, const double* freqs, const double* h0, const double* dataConstants, int K,
+                             const double* qnm, int table_len, double dspin, double initial_t_val,
+                             const bbhx_response* orbits, int TDItag, int rescaled, int tdi2,
+                             int include_last_segment, void* stream);
+
+/* ---- fused direct likelihood of a ragged batch (no counterpart in the reference) ---- */
+
+/* Likelihood.get_ll (likelihood.py:140-225) with BBHWaveformFD(direct=False)
+ * (waveformbuild.py:338-700) for templates whose sparse grids have different
+ * lengths L_b, in one sequence of six launches with no host synchronisation
+ * and no allocation, so that the call can be captured in a hipGraph. A plan
+ * holds the sizes and the device buffers (sparse grids, the nine spline
+ * quantities and their coefficients: 296 B per (knot, mode), plus per-binary
+ * tables); plan_bytes returns their total. */
+typedef struct bbhx_direct_plan bbhx_direct_plan;
+
+/* lengths_host (B,): sparse grid length of each binary (>= 4, <= 6144).
+ * data_length: N dense bins. chunk: dense bins per work item of the persistent
+ * kernel; block: 64, 128 or 256 threads; grid: persistent workgroups. */
+int bbhx_hip_direct_plan_create(bbhx_direct_plan** out, int numBinAll, const int* lengths_host, int numModes,
+                                const double* dataFreqs_host_unused, int data_length, int chunk, int block, int grid);
+int bbhx_hip_direct_plan_destroy(bbhx_direct_plan* p);
+long bbhx_hip_direct_plan_bytes(const bbhx_direct_plan* p);
+
+/* params (11, B) and t_start, t_end (B,) as for bbhx_hip_hdyn_fused; dataFreqs
+ * (N,) sorted; dataChannels (3, N) complex = noise weight x data, as
+ * Likelihood keeps them (likelihood.py:100-112); noiseFactors (3, N) the
+ * weights sqrt(df / S). Outputs (overwritten, not accumulated): d_h, h_h (B,)
+ * complex with the reference's factor 4 (Likelihood.cu:357-375); flags (B,)
+ * int, which the caller zeroes: bit 0 / bit 1 set when the observation-window
+ * search at the low / high end failed for some mode (the reference raises
+ * ValueError there, phenomhm.py:530-600). mf_min, mf_max: the default grid
+ * Mf range (1e-4, 0.6 in the reference). */
+int bbhx_hip_direct_fused(bbhx_direct_plan* p, double* d_h, double* h_h, int* flags, const double* params,
+                          const double* t_start, const double* t_end, const int* ells_host, const int* mms_host,
+                          int numModes, const double* dataFreqs, const double* dataChannels,
+                          const double* noiseFactors, const double* qnm, int table_len, double dspin,
+                          double initial_t_val, double mf_min, double mf_max, const bbhx_response* orbits,
+                          int TDItag, int rescaled, int tdi2, void* stream);
+
+/* Event timing of the fused entry points on the caller's stream, split in two:
+ * set-up (records; for the direct form also grids, splines and bands) and
+ * evaluation. bbhx_hip_hdyn_fused, bbhx_hip_hdyn_fused_grad and
+ * bbhx_hip_direct_fused record it: enable(1) starts recording (and clears earlier records);
  * read() waits for the recorded events and returns the summed milliseconds and
  * the number of calls, then clears. For benchmarks; off by default. */
 int bbhx_hip_timing_enable(int on);

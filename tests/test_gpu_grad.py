@@ -1,0 +1,126 @@
+"""Gradient of the heterodyned log-likelihood (adjoint kernel) against central
+finite differences of the FP64 CPU oracle's heterodyned log-likelihood.
+
+The oracle is evaluated at x +- h e_k and x +- 2h e_k (fourth-order central
+difference) in the sampling coordinates (ln M, q, chi1z, chi2z, distance,
+phi_ref, cos inc, lam, sin beta, psi, t_ref). Steps are fixed per coordinate so
+that truncation and round-off of the difference are both far below the
+tolerance: |g_gpu - g_fd| <= 1e-5 |g_fd| per component, or, for components
+that are small against the template's own gradient scale (where the
+difference itself is only known to round-off), <= 1e-5 times the
+difference's round-off bound.
+"""
+
+import numpy as np
+import pytest
+
+from oracle.pipeline import OracleHeterodyne, OracleWaveform
+from tests.params import MODES, YRSID_SI, draw
+
+pytestmark = pytest.mark.gpu
+
+# finite-difference steps per sampling coordinate
+STEPS = np.array([1e-8, 1e-7, 1e-7, 1e-7, 0.0, 1e-6, 1e-6, 1e-6, 1e-6, 1e-6, 1e-2])
+REL_DIST_STEP = 1e-7
+
+
+def to_sampling(p):
+    """(12, B) parameter rows -> (B, 11) sampling coordinates."""
+    m1, m2 = p[0], p[1]
+    return np.stack([np.log(m1 + m2), m1 / m2, p[2], p[3], p[4], p[5], np.cos(p[7]), p[8], np.sin(p[9]), p[10],
+                     p[11]], axis=1)
+
+
+@pytest.fixture(scope="module")
+def grad_problem():
+    from bbhx_amd import get_backend
+    from bbhx_amd.likelihood import HeterodynedLikelihood
+    from bbhx_amd.utils.orbits import EqualArmlengthOrbits
+    from bbhx_amd.utils.sensitivity import AET1SensitivityMatrix
+    from bbhx_amd.waveformbuild import BBHWaveformFD
+
+    backend = get_backend("hip")
+    orbits = EqualArmlengthOrbits(dt=1e4, t_max=3.0 * YRSID_SI)
+    ref = draw(1, 1001)[:, 0]
+    ref[0], ref[1] = 1.3e6, 4.1e5
+    ref[11] = 0.62 * YRSID_SI
+    f = np.arange(1, 2**15 + 1) * (4.0 / YRSID_SI)
+    owave = OracleWaveform(orbits.as_host_dict(), tdi2=True)
+    inj = owave(*ref, freqs=f, length=1024, modes=MODES, direct=False, fill=True, squeeze=True)[0]
+    sens = AET1SensitivityMatrix(f)
+    psd = np.array([sens[0], sens[1], sens[2]])
+    rng = np.random.default_rng(1001)
+    df = f[1] - f[0]
+    data = inj + np.sqrt(psd / (4 * df)) * (rng.standard_normal(psd.shape) + 1j * rng.standard_normal(psd.shape))
+    K = 96
+    wave = BBHWaveformFD(response_kwargs=dict(orbits=orbits, tdi2=True), force_backend=backend)
+    het = HeterodynedLikelihood(wave, f, data, ref, K, sens_mat=sens, template_gen_kwargs=dict(modes=MODES),
+                                reference_gen_kwargs=dict(length=1024, modes=MODES), force_backend=backend)
+    psd_fn = lambda x: [AET1SensitivityMatrix(x)[i] for i in range(3)]  # noqa: E731
+    ohet = OracleHeterodyne(owave, f, data, ref, K, psd_fn, reference_length=1024, modes=MODES)
+    # templates near the reference, far enough that every gradient component is sizable
+    x0 = to_sampling(ref[:, None])[0]
+    rng = np.random.default_rng(5)
+    dx = np.array([3e-5, 1e-3, 3e-3, 3e-3, 0.0, 0.05, 0.02, 0.01, 0.01, 0.05, 3.0])
+    X = x0[None, :] + dx[None, :] * rng.standard_normal((3, 11))
+    X[:, 4] = x0[4] * (1 + 0.01 * rng.standard_normal(3))
+    return dict(het=het, ohet=ohet, X=X)
+
+
+def oracle_ll(ohet, X):
+    from bbhx_amd.likelihood import HeterodynedLikelihood
+
+    return ohet.get_ll(HeterodynedLikelihood.sampling_to_params(X))[0]
+
+
+def test_grad_value_matches_forward(grad_problem):
+    het, ohet, X = grad_problem["het"], grad_problem["ohet"], grad_problem["X"]
+    ll_g, grad = het.get_ll_and_grad(X)
+    ll_f = het.get_ll(het.sampling_to_params(X), fused=True)
+    ll_o = oracle_ll(ohet, X)
+    print("logL gradient kernel", ll_g, "forward", ll_f, "oracle", ll_o, flush=True)
+    assert np.all(np.isfinite(grad))
+    assert np.max(np.abs(ll_g - ll_f)) < 1e-7
+    assert np.max(np.abs(ll_g - ll_o)) < 1e-6
+
+
+def test_grad_against_oracle_finite_differences(grad_problem):
+    het, ohet, X = grad_problem["het"], grad_problem["ohet"], grad_problem["X"]
+    B = X.shape[0]
+    _, grad = het.get_ll_and_grad(X)
+    steps = np.tile(STEPS, (B, 1))
+    steps[:, 4] = REL_DIST_STEP * X[:, 4]
+    pts = []
+    for k in range(11):
+        for s in (2, 1, -1, -2):
+            Y = X.copy()
+            Y[:, k] += s * steps[:, k]
+            pts.append(Y)
+    ll = oracle_ll(ohet, np.concatenate(pts)).reshape(11, 4, B)
+    llx = oracle_ll(ohet, X)
+    fd = (-ll[:, 0] + 8 * ll[:, 1] - 8 * ll[:, 2] + ll[:, 3]) / (12 * steps.T)  # (11, B)
+    fd = fd.T
+    # round-off bound of the difference: |logL| eps_sum / h, eps_sum ~ 1e-13 for these sums
+    roundoff = 1e-13 * np.maximum(np.abs(llx), 1.0)[:, None] / steps
+    err = np.abs(grad - fd)
+    rel = err / np.maximum(np.abs(fd), 1e-300)
+    names = ("lnM", "q", "chi1", "chi2", "dist", "phi", "cosi", "lam", "sinb", "psi", "tref")
+    for b in range(B):
+        print(f"template {b}: logL = {llx[b]:.6e}")
+        for k in range(11):
+            print(f"  {names[k]:5s} gpu {grad[b, k]: .9e} fd {fd[b, k]: .9e} rel {rel[b, k]:.2e} "
+                  f"roundoff {roundoff[b, k]:.1e}", flush=True)
+    ok = (err <= 1e-5 * np.abs(fd)) | (err <= 10 * roundoff)
+    assert np.all(ok), np.argwhere(~ok)
+
+
+def test_autograd_function_backward(grad_problem):
+    import torch
+
+    het, X = grad_problem["het"], grad_problem["X"]
+    x = torch.tensor(X, dtype=torch.float64, requires_grad=True)
+    ll = het.loglike(x)
+    w = torch.tensor([1.0, -2.0, 0.5], dtype=torch.float64)
+    (ll * w).sum().backward()
+    _, grad = het.get_ll_and_grad(X)
+    np.testing.assert_allclose(x.grad.numpy(), w.numpy()[:, None] * grad, rtol=1e-14, atol=0)
