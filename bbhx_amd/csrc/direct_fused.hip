@@ -41,7 +41,7 @@ bool response_orbit_tab(const bbhx_response* r, resp::OrbitTab& o);
 struct bbhx_direct_plan {
     int device = -1;
     int B = 0, M = 0, N = 0, Lmax = 0;
-    int chunk = 0, block = 0, grid = 0;
+    int min_chunk = 0;   // smallest chunk the partial-sum buffer is sized for
     long S = 0;          // total sparse points
     long max_items = 0;  // bound on (binary, chunk) work items
     // device buffers
@@ -502,12 +502,10 @@ void plan_free(bbhx_direct_plan* p)
 }  // namespace
 
 extern "C" int bbhx_hip_direct_plan_create(bbhx_direct_plan** out, int numBinAll, const int* lengths_host,
-                                           int numModes, const double* dataFreqs_host_unused, int data_length,
-                                           int chunk, int block, int grid)
+                                           int numModes, int data_length, int min_chunk)
 {
-    (void)dataFreqs_host_unused;
     if (!out || numBinAll < 1 || !lengths_host || numModes < 1 || numModes > MAX_MODES || data_length < 1 ||
-        chunk < 1 || (block != 64 && block != 128 && block != 256) || grid < 1)
+        min_chunk < 64)
         return ERR_ARGS;
     std::vector<long> off(numBinAll);
     long S = 0;
@@ -526,11 +524,9 @@ extern "C" int bbhx_hip_direct_plan_create(bbhx_direct_plan** out, int numBinAll
     p->M = numModes;
     p->N = data_length;
     p->Lmax = Lmax;
-    p->chunk = chunk;
-    p->block = block;
-    p->grid = grid;
+    p->min_chunk = min_chunk;
     p->S = S;
-    p->max_items = (long)numBinAll * ((data_length + chunk - 1) / chunk) + numBinAll;
+    p->max_items = (long)numBinAll * ((data_length + min_chunk - 1) / min_chunk) + numBinAll;
     if (p->max_items > 2147483647L) {
         delete p;
         return ERR_ARGS;
@@ -586,9 +582,11 @@ extern "C" int bbhx_hip_direct_fused(bbhx_direct_plan* p, double* d_h, double* h
                                      const int* mms_host, int numModes, const double* dataFreqs,
                                      const double* dataChannels, const double* noiseFactors, const double* qnm,
                                      int table_len, double dspin, double initial_t_val, double mf_min, double mf_max,
-                                     const bbhx_response* orbits, int TDItag, int rescaled, int tdi2, void* stream)
+                                     const bbhx_response* orbits, int TDItag, int rescaled, int tdi2, int chunk,
+                                     int block, int grid, void* stream)
 {
     if (!p) return ERR_ARGS;
+    if (chunk < p->min_chunk || (block != 64 && block != 128 && block != 256) || grid < 1) return ERR_ARGS;
     int dev = -1;
     (void)hipGetDevice(&dev);
     if (dev != p->device || numModes != p->M) return ERR_ARGS;
@@ -612,11 +610,11 @@ extern "C" int bbhx_hip_direct_fused(bbhx_direct_plan* p, double* d_h, double* h
                        orb, TDItag, rescaled, tdi2, initial_t_val, mf_min, mf_max);
     hipLaunchKernelGGL(k_dir_spline, dim3(B), dim3(64), 0, st, p->X, p->Y, p->C1, p->C2, p->C3, p->off, p->len, M);
     hipLaunchKernelGGL(k_dir_band, dim3(1), dim3(1024), 0, st, p->band, p->total, p->X, p->off, p->len, dataFreqs,
-                       p->N, B, M, p->chunk);
+                       p->N, B, M, chunk);
     if (timed) (void)hipEventRecord(ev[1], st);
-    hipLaunchKernelGGL(k_dir_like, dim3(p->grid), dim3(p->block), 0, st, p->partial, p->band, p->total, p->X, p->Y,
+    hipLaunchKernelGGL(k_dir_like, dim3(grid), dim3(block), 0, st, p->partial, p->band, p->total, p->X, p->Y,
                        p->C1, p->C2, p->C3, p->grid_lo, p->grid_step, p->off, p->len, t_start, t_end, dataFreqs,
-                       reinterpret_cast<const cd*>(dataChannels), noiseFactors, p->N, B, M, p->chunk);
+                       reinterpret_cast<const cd*>(dataChannels), noiseFactors, p->N, B, M, chunk);
     hipLaunchKernelGGL(k_dir_reduce, dim3(ceil_div(B, 256)), dim3(256), 0, st, d_h, h_h, p->partial, p->band,
                        p->total, B);
     if (timed) (void)hipEventRecord(ev[2], st);

@@ -75,13 +75,12 @@ def load_library():
         "bbhx_hip_hdyn_fused_grad": [_vp, _vp, _vp, _vp, _vp, _vp, C.c_int, _ip, _ip, C.c_int, _vp, _vp, _vp,
                                      C.c_int, _vp, C.c_int, C.c_double, C.c_double, _vp, C.c_int, C.c_int, C.c_int,
                                      C.c_int, _vp],
-        "bbhx_hip_direct_plan_create": [C.POINTER(_vp), C.c_int, _ip, C.c_int, _vp, C.c_int, C.c_int, C.c_int,
-                                        C.c_int],
+        "bbhx_hip_direct_plan_create": [C.POINTER(_vp), C.c_int, _ip, C.c_int, C.c_int, C.c_int],
         "bbhx_hip_direct_plan_destroy": [_vp],
         "bbhx_hip_direct_plan_bytes": [_vp],
         "bbhx_hip_direct_fused": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _ip, _ip, C.c_int, _vp, _vp, _vp, _vp, C.c_int,
                                   C.c_double, C.c_double, C.c_double, C.c_double, _vp, C.c_int, C.c_int, C.c_int,
-                                  _vp],
+                                  C.c_int, C.c_int, C.c_int, _vp],
         "bbhx_hip_release_workspace": [],
         "bbhx_hip_version": [],
         "bbhx_hip_timing_enable": [C.c_int],
@@ -346,9 +345,11 @@ def hdyn_fused_grad(d_h, h_h, grad, params, t_start, t_end, ells, mms, freqs, h0
 
 class DirectPlan:
     """Sizes and device buffers of the fused direct likelihood of a ragged batch
-    (bbhx_hip_direct_plan_*). lengths: (B,) sparse grid lengths."""
+    (bbhx_hip_direct_plan_*). lengths: (B,) sparse grid lengths. The launch
+    shape (chunk, block, grid) of the dense-band kernel can change between
+    calls (set_launch; chunk >= min_chunk)."""
 
-    def __init__(self, lengths, num_modes, data_length, chunk=16384, block=256, grid=None):
+    def __init__(self, lengths, num_modes, data_length, chunk=16384, block=256, grid=None, min_chunk=1024):
         torch = _torch()
         lib = load_library()
         lens, lp = _host_i32(lengths)
@@ -356,12 +357,17 @@ class DirectPlan:
             grid = 8 * torch.cuda.get_device_properties(torch.cuda.current_device()).multi_processor_count
         self.lengths = lens.copy()
         self.num_modes, self.data_length = int(num_modes), int(data_length)
-        self.chunk, self.block, self.grid = int(chunk), int(block), int(grid)
+        self.min_chunk = int(min(min_chunk, chunk))
+        self.set_launch(chunk, block, grid)
         h = _vp()
-        _check(lib.bbhx_hip_direct_plan_create(C.byref(h), int(lens.size), lp, self.num_modes, None,
-                                               self.data_length, self.chunk, self.block, self.grid),
-               "direct_plan_create")
+        _check(lib.bbhx_hip_direct_plan_create(C.byref(h), int(lens.size), lp, self.num_modes, self.data_length,
+                                               self.min_chunk), "direct_plan_create")
         self._h = h
+
+    def set_launch(self, chunk, block, grid):
+        if int(chunk) < self.min_chunk:
+            raise ValueError(f"chunk must be >= {self.min_chunk} for this plan")
+        self.chunk, self.block, self.grid = int(chunk), int(block), int(grid)
 
     @property
     def handle(self):
@@ -399,7 +405,7 @@ def direct_fused(plan, d_h, h_h, flags, params, t_start, t_end, ells, mms, data_
         _dev(t_start, f64), _dev(t_end, f64), elp, mmp, int(el.size), _dev(data_freqs, f64),
         _dev(data_channels, c128), _dev(noise_factors, f64), _dev(qnm, f64), int(qnm.numel()) // 8, float(dspin),
         float(initial_t_val), float(mf_min), float(mf_max), response.handle, int(TDItag), int(bool(rescaled)),
-        int(bool(tdi2)), _stream())
+        int(bool(tdi2)), plan.chunk, plan.block, plan.grid, _stream())
     _check(st, "direct_fused")
 
 

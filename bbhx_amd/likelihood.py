@@ -70,12 +70,13 @@ class Likelihood(BBHxParallelModule):
 
     # ---------------------------------------------------------- fused ragged path
 
-    def direct_plan(self, lengths, modes=None, chunk=16384, block=256, grid=None):
+    def direct_plan(self, lengths, modes=None, chunk=16384, block=256, grid=None, min_chunk=1024):
         """Plan (sizes + device buffers) of the fused direct likelihood for a
         batch whose binary b has a sparse grid of lengths[b] points."""
         gen = self.waveform_gen
         M = len(modes) if modes is not None else len(gen.amp_phase_gen.allowable_modes)
-        return self.backend.DirectPlan(lengths, M, self.data_stream_length, chunk=chunk, block=block, grid=grid)
+        return self.backend.DirectPlan(lengths, M, self.data_stream_length, chunk=chunk, block=block, grid=grid,
+                                       min_chunk=min_chunk)
 
     def get_ll_fused_device(self, plan, rows, t_start, t_end, flags, modes=None):
         """Fused direct log-likelihood of device inputs (``fused_inputs``): a

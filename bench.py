@@ -91,6 +91,109 @@ def cpu_baseline(prob, sample, nodes, min_seconds=10.0):
     return dict(value=sample / dt, ll=ll, cores=cores, seconds_per_batch=dt, reps=reps)
 
 
+def ragged_lengths(p, t_data):
+    """Sparse grid length of each template of the uneven batch, proportional to
+    the log-span of its observable band: f_lo = the leading-order PN frequency
+    a time t_ref before merger (or 1/T_data if higher), f_hi = min(0.1 / M_sec,
+    Nyquist 0.05 Hz); L = clip(2400 (log10(f_hi / f_lo) - 1), 256, 4096), which
+    spreads the seeded batch over the whole [256, 4096] range."""
+    MTSUN = 4.925491025873693e-06
+    m1, m2, tref = p[0], p[1], p[11]
+    Mc = (m1 * m2) ** 0.6 / (m1 + m2) ** 0.2 * MTSUN
+    f_pn = (5.0 * Mc / np.maximum(tref, 1.0)) ** 0.375 / (8 * np.pi * Mc)
+    f_lo = np.maximum(f_pn, 1.0 / t_data)
+    f_hi = np.minimum(0.1 / ((m1 + m2) * MTSUN), 0.05)
+    span = np.log10(np.maximum(f_hi / f_lo, 1.0 + 1e-12))
+    return np.clip(np.round(2400 * (span - 1.0)), 256, 4096).astype(np.int32)
+
+
+def bench_direct_ragged(args):
+    """Uneven batching: direct likelihood of templates with per-template sparse
+    grid lengths, fused, in one persistent launch sequence captured in a
+    hipGraph, launch shape from the autotuner. Prints one JSON line."""
+    import torch
+
+    from bbhx_amd.autotune import tune_direct
+    from bbhx_amd.cutils import timing_enable, timing_read
+    from bbhx_amd.likelihood import Likelihood, fused_param_rows
+    from bbhx_amd.utils.constants import YRSID_SI
+    from bbhx_amd.utils.orbits import EqualArmlengthOrbits
+    from bbhx_amd.utils.sensitivity import AET1SensitivityMatrix
+    from bbhx_amd.waveformbuild import BBHWaveformFD
+    from tests.params import MODES, draw
+
+    n = int(YRSID_SI // 10)
+    f = np.fft.rfftfreq(n, 10.0)[1:]
+    if args.direct_bins:
+        f = f[:args.direct_bins]
+    p = draw(args.batch, 1002, mtot=(1e4, 1e8))
+    lengths = ragged_lengths(p, YRSID_SI)
+    log(f"uneven batch: {p.shape[1]} templates, L in [{lengths.min()}, {lengths.max()}], sum {lengths.sum()}, "
+        f"{f.size} dense bins")
+    orbits = EqualArmlengthOrbits(dt=1e4, t_max=3.0 * YRSID_SI)
+    wave = BBHWaveformFD(response_kwargs=dict(orbits=orbits, tdi2=True))
+    sens = AET1SensitivityMatrix(f)
+    psd = np.array([sens[0], sens[1], sens[2]])
+    inj = wave(*p[:, 0], freqs=f, length=1024, modes=MODES, direct=False, fill=True)[0].cpu().numpy()
+    rng = np.random.default_rng(1003)
+    df = f[1] - f[0]
+    data = inj + np.sqrt(psd / (4 * df)) * (rng.standard_normal(psd.shape) + 1j * rng.standard_normal(psd.shape))
+    like = Likelihood(wave, f, data, psd)
+    log("autotuning the launch shape")
+    tuned = tune_direct(like, p, lengths, MODES, reps=2)
+    log(f"autotuner: chunk {tuned['chunk']}, block {tuned['block']}, grid {tuned['grid']} ({tuned['ms']:.2f} ms)")
+    plan = like.direct_plan(lengths, MODES, chunk=tuned["chunk"], block=tuned["block"], grid=tuned["grid"])
+    rows, ts, te = fused_param_rows(like.waveform_gen, p, 0.0, 1.0, like.device)
+    flags = torch.zeros(rows.shape[1], dtype=torch.int32, device=like.device)
+    st = torch.cuda.Stream()
+    st.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(st):
+        for _ in range(max(args.warmup, 1)):
+            flags.zero_()
+            like.get_ll_fused_device(plan, rows, ts, te, flags, MODES)
+    torch.cuda.current_stream().wait_stream(st)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        flags.zero_()
+        out = like.get_ll_fused_device(plan, rows, ts, te, flags, MODES)
+    torch.cuda.synchronize()
+    eager = like.get_ll_fused(p, lengths, modes=MODES, plan=plan)
+    g.replay()
+    torch.cuda.synchronize()
+    diff = float(np.max(np.abs(out.cpu().numpy() - eager)))
+    timing_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        g.replay()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    timing_enable(False)
+    ms = (t1 - t0) * 1e3 / args.steps
+    B = rows.shape[1]
+    log(f"graph replay: {ms:.3f} ms/step, {B / (ms * 1e-3):.4g} evals/s; |graph - eager| = {diff:.2e}; "
+        f"plan {plan.nbytes() / 2**30:.2f} GiB")
+    out = {
+        "metric": "direct logL evals/sec, uneven batch, 1 GPU",
+        "value": B / (ms * 1e-3),
+        "unit": "logL evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded injection + Gaussian noise, analytic A/E/T PSD)",
+        "config": {"workload": "direct logL, uneven batch, fused, hipGraph replay", "templates": B,
+                   "lengths_min": int(lengths.min()), "lengths_max": int(lengths.max()),
+                   "lengths_sum": int(lengths.sum()), "dense_bins": int(f.size), "modes": 6, "tdi": "AET tdi2",
+                   "launch": {k: tuned[k] for k in ("chunk", "block", "grid")}, "graph_vs_eager": diff},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,6 +205,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=4096)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", default="heterodyned", choices=("heterodyned", "direct-ragged"))
+    ap.add_argument("--direct-bins", type=int, default=0, help="truncate the 1 yr / 10 s data grid (testing)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB device-to-device copy (reference kernel for HBM byte counters)")
     args = ap.parse_args()
@@ -113,6 +218,12 @@ def main():
     import torch
 
     from bbhx_amd.cutils import timing_enable, timing_read
+
+    if args.workload == "direct-ragged":
+        if args.batch == 4096:
+            args.batch = 8192
+        bench_direct_ragged(args)
+        return
 
     if args.copy_calibration:
         src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")

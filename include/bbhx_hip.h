@@ -192,10 +192,11 @@ int bbhx_hip_hdyn_fused_grad(double* d_h, double* h_h, double* grad, const doubl
 typedef struct bbhx_direct_plan bbhx_direct_plan;
 
 /* lengths_host (B,): sparse grid length of each binary (>= 4, <= 6144).
- * data_length: N dense bins. chunk: dense bins per work item of the persistent
- * kernel; block: 64, 128 or 256 threads; grid: persistent workgroups. */
+ * data_length: N dense bins. min_chunk (>= 64): the smallest number of dense
+ * bins per work item that bbhx_hip_direct_fused will be called with (sizes the
+ * partial-sum buffer). */
 int bbhx_hip_direct_plan_create(bbhx_direct_plan** out, int numBinAll, const int* lengths_host, int numModes,
-                                const double* dataFreqs_host_unused, int data_length, int chunk, int block, int grid);
+                                int data_length, int min_chunk);
 int bbhx_hip_direct_plan_destroy(bbhx_direct_plan* p);
 long bbhx_hip_direct_plan_bytes(const bbhx_direct_plan* p);
 
@@ -207,13 +208,16 @@ long bbhx_hip_direct_plan_bytes(const bbhx_direct_plan* p);
  * int, which the caller zeroes: bit 0 / bit 1 set when the observation-window
  * search at the low / high end failed for some mode (the reference raises
  * ValueError there, phenomhm.py:530-600). mf_min, mf_max: the default grid
- * Mf range (1e-4, 0.6 in the reference). */
+ * Mf range (1e-4, 0.6 in the reference). Launch shape of the persistent
+ * dense-band kernel: chunk (>= the plan's min_chunk) dense bins per work item,
+ * block = 64, 128 or 256 threads, grid persistent workgroups; the result does
+ * not depend on it beyond the order of the sums. */
 int bbhx_hip_direct_fused(bbhx_direct_plan* p, double* d_h, double* h_h, int* flags, const double* params,
                           const double* t_start, const double* t_end, const int* ells_host, const int* mms_host,
                           int numModes, const double* dataFreqs, const double* dataChannels,
                           const double* noiseFactors, const double* qnm, int table_len, double dspin,
                           double initial_t_val, double mf_min, double mf_max, const bbhx_response* orbits,
-                          int TDItag, int rescaled, int tdi2, void* stream);
+                          int TDItag, int rescaled, int tdi2, int chunk, int block, int grid, void* stream);
 
 /* Event timing of the fused entry points on the caller's stream, split in two:
  * set-up (records; for the direct form also grids, splines and bands) and

@@ -24,7 +24,8 @@ def test_header_declares_the_backend_methods():
               "bbhx_hip_get_phenomd_ringdown_frequencies", "bbhx_hip_response_create",
               "bbhx_hip_response_add_orbits", "bbhx_hip_response_compute", "bbhx_hip_response_destroy",
               "bbhx_hip_interpolate", "bbhx_hip_InterpTDI", "bbhx_hip_direct_sum", "bbhx_hip_hdyn",
-              "bbhx_hip_direct_like", "bbhx_hip_hdyn_fused"):
+              "bbhx_hip_direct_like", "bbhx_hip_hdyn_fused", "bbhx_hip_hdyn_fused_grad", "bbhx_hip_direct_plan_create",
+              "bbhx_hip_direct_plan_destroy", "bbhx_hip_direct_fused"):
         assert n in names, n
 
 
@@ -56,6 +57,19 @@ def test_version_and_argument_errors_need_no_gpu():
                                            None, None, 0, None) == 1
     assert lib.bbhx_hip_response_compute(None, None, None, None, None, None, None, None, None, None, 2, 0, 1, 0, 6,
                                          10, 1, 1, None) == 1
+    # ragged direct plan: lengths below 4, bad min_chunk, no lengths
+    import ctypes as C
+
+    h = C.c_void_p()
+    lens = (C.c_int * 3)(512, 3, 600)
+    assert lib.bbhx_hip_direct_plan_create(C.byref(h), 3, lens, 6, 1000, 1024) == 1
+    lens_ok = (C.c_int * 3)(512, 300, 600)
+    assert lib.bbhx_hip_direct_plan_create(C.byref(h), 3, lens_ok, 6, 1000, 16) == 1
+    assert lib.bbhx_hip_direct_plan_create(C.byref(h), 3, None, 6, 1000, 1024) == 1
+    assert lib.bbhx_hip_direct_plan_create(C.byref(h), 3, lens_ok, 7, 1000, 1024) == 1
+    assert lib.bbhx_hip_direct_fused(None, *([None] * 8), 6, *([None] * 4), 2, 2e-3, 0.0, 1e-4, 0.6, None, 2, 0, 1,
+                                     16384, 256, 64, None) == 1
+    assert lib.bbhx_hip_direct_plan_bytes(None) == -1
 
 
 def test_registry_without_gpu_fails_loudly(monkeypatch):
@@ -146,6 +160,35 @@ def test_sensitivity_positive_and_t_suppressed():
     low = f < 1e-3
     assert np.all(s[2][low] < 1e-2 * s[0][low])  # T is insensitive at low frequency
     np.testing.assert_array_equal(get_sensitivity(f, sens_fn="T1TDISens"), s[2])
+
+
+def test_sampling_coordinates_round_trip():
+    from bbhx_amd.likelihood import HeterodynedLikelihood
+
+    rng = np.random.default_rng(4)
+    B = 5
+    x = np.stack([rng.uniform(11, 16, B), rng.uniform(1, 10, B), rng.uniform(-0.9, 0.9, B), rng.uniform(-0.9, 0.9, B),
+                  rng.uniform(1e25, 1e26, B), rng.uniform(0, 6, B), rng.uniform(-0.99, 0.99, B), rng.uniform(0, 6, B),
+                  rng.uniform(-0.99, 0.99, B), rng.uniform(0, 3, B), rng.uniform(1e6, 3e7, B)], axis=1)
+    p = HeterodynedLikelihood.sampling_to_params(x)
+    assert p.shape == (12, B)
+    np.testing.assert_allclose(np.log(p[0] + p[1]), x[:, 0], rtol=1e-14)
+    np.testing.assert_allclose(p[0] / p[1], x[:, 1], rtol=1e-14)
+    np.testing.assert_allclose(np.cos(p[7]), x[:, 6], rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(np.sin(p[9]), x[:, 8], rtol=1e-12, atol=1e-15)
+    np.testing.assert_array_equal(p[6], 0.0)
+    assert np.all(p[0] >= p[1])
+
+
+def test_uneven_lengths_rule_spans_range():
+    import bench
+    from tests.params import draw
+
+    p = draw(8192, 1002, mtot=(1e4, 1e8))
+    L = bench.ragged_lengths(p, 31558149.763545603)
+    assert L.min() == 256 and L.max() <= 4096 and L.max() > 3000
+    # longer grids for wider bands: monotone in the log-span
+    assert np.corrcoef(L, np.log(p[0] + p[1]))[0, 1] < 0
 
 
 def test_qnm_table_file_matches_data_module():
