@@ -64,15 +64,13 @@ namespace {
 
 constexpr int NQ = 9;
 
-__global__ void k_dir_setup(FusedRec* __restrict__ recs, const double* __restrict__ params, int B, ModeList modes,
+__global__ __launch_bounds__(64) void k_dir_setup(FusedRec* __restrict__ recs, const double* __restrict__ params, int B, ModeList modes,
                             int M, const double* __restrict__ qnm, int table_len, double dspin)
 {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= B * M) return;
     const int b = idx / M, mi = idx % M;
-    FusedRec r;
-    setup_one(r, params, B, b, modes.ell[mi], modes.mm[mi], qnm, table_len, dspin);
-    recs[idx] = r;
+    setup_one(recs[idx], params, B, b, modes.ell[mi], modes.mm[mi], qnm, table_len, dspin);
 }
 
 // numpy.logspace point i of num between 10^lo and 10^hi (start + i step, the

@@ -33,15 +33,13 @@ bool response_orbit_tab(const bbhx_response* r, resp::OrbitTab& o);
 
 namespace {
 
-__global__ void k_fused_setup(FusedRec* __restrict__ recs, const double* __restrict__ params, int B, ModeList modes,
+__global__ __launch_bounds__(64) void k_fused_setup(FusedRec* __restrict__ recs, const double* __restrict__ params, int B, ModeList modes,
                               int M, const double* __restrict__ qnm, int table_len, double dspin)
 {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= B * M) return;
     const int b = idx / M, mi = idx % M;
-    FusedRec r;
-    setup_one(r, params, B, b, modes.ell[mi], modes.mm[mi], qnm, table_len, dspin);
-    recs[idx] = r;
+    setup_one(recs[idx], params, B, b, modes.ell[mi], modes.mm[mi], qnm, table_len, dspin);
 }
 
 // One 64-lane wavefront (one workgroup) per template. Dynamic LDS holds the

@@ -109,7 +109,7 @@ __device__ inline void grad_resp_setup(GradResp& r, int l, int m, double phi, do
     r.dYc = mul_i((dc * dci) * e);
 }
 
-__global__ void k_grad_setup(FusedRec* __restrict__ recs, GradRec* __restrict__ grecs,
+__global__ __launch_bounds__(64) void k_grad_setup(FusedRec* __restrict__ recs, GradRec* __restrict__ grecs,
                              const double* __restrict__ params, int B, ModeList modes, int M,
                              const double* __restrict__ qnm, int table_len, double dspin)
 {
@@ -117,11 +117,7 @@ __global__ void k_grad_setup(FusedRec* __restrict__ recs, GradRec* __restrict__ 
     if (idx >= B * M) return;
     const int b = idx / M, mi = idx % M;
     const int l = modes.ell[mi], m = modes.mm[mi];
-    {
-        FusedRec r;
-        setup_one(r, params, B, b, l, m, qnm, table_len, dspin);
-        recs[idx] = r;
-    }
+    setup_one(recs[idx], params, B, b, l, m, qnm, table_len, dspin);
     // masses from (ln M, q): m1 = M q / (1 + q), m2 = M / (1 + q)
     const double m1 = params[P_M1 * B + b], m2 = params[P_M2 * B + b];
     const double Mt = m1 + m2, q = m1 / m2;

@@ -21,7 +21,7 @@ namespace {
 
 constexpr int EVAL_THREADS = 256;
 
-__global__ void k_phenomd_ringdown(double* __restrict__ fringdown, double* __restrict__ fdamp,
+__global__ __launch_bounds__(256) void k_phenomd_ringdown(double* __restrict__ fringdown, double* __restrict__ fdamp,
                                    const double* __restrict__ m1, const double* __restrict__ m2,
                                    const double* __restrict__ chi1z, const double* __restrict__ chi2z,
                                    int numBinAll, const double* __restrict__ y_rd, const double* __restrict__ c1_rd,
@@ -62,7 +62,7 @@ __global__ void k_phenomhm_ringdown(double* __restrict__ fringdown, double* __re
     }
 }
 
-__global__ void k_phm_setup(PhmRec* __restrict__ recs, const int* __restrict__ ells, const int* __restrict__ mms,
+__global__ __launch_bounds__(64) void k_phm_setup(PhmRec* __restrict__ recs, const int* __restrict__ ells, const int* __restrict__ mms,
                             const double* __restrict__ m1_SI, const double* __restrict__ m2_SI,
                             const double* __restrict__ chi1z, const double* __restrict__ chi2z,
                             const double* __restrict__ distance, int numModes, int numBinAll,
@@ -77,10 +77,8 @@ __global__ void k_phm_setup(PhmRec* __restrict__ recs, const int* __restrict__ e
     const double* dm = Mf_DM_all + (long)b * ncol;
     const double rdD = run_phenomd ? rd[0] : rd[numModes];
     const double dmD = run_phenomd ? dm[0] : dm[numModes];
-    PhmRec r;
-    phm_setup(r, m1_SI[b], m2_SI[b], chi1z[b], chi2z[b], distance[b], ells[mi], mms[mi], rd[mi], dm[mi], rdD, dmD,
-              run_phenomd != 0);
-    recs[idx] = r;
+    phm_setup(recs[idx], m1_SI[b], m2_SI[b], chi1z[b], chi2z[b], distance[b], ells[mi], mms[mi], rd[mi], dm[mi], rdD,
+              dmD, run_phenomd != 0);
 }
 
 __global__ __launch_bounds__(EVAL_THREADS) void k_phm_eval(double* __restrict__ out, const PhmRec* __restrict__ recs,

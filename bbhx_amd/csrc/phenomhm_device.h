@@ -813,7 +813,7 @@ __device__ inline bool phm_setup(PhmRecT<S>& r, nd<S> m1_SI, nd<S> m2_SI, nd<S> 
 // Amplitude, phase and t(f) of one (binary, mode) at frequency f in Hz
 // (calculate_modes / calculate_modes_phenomd, PhenomHMWaveform.cu:2461-2662).
 template <class S>
-__device__ inline void phm_eval(const PhmRecT<S>& r, nd<S> f, S& amp, S& phase, S& tf)
+__device__ inline void phm_eval(const PhmRecT<S>& r, double f, S& amp, S& phase, S& tf)
 {
     // A PhenomD-only record (run_phenomd) carries identity maps (a = 1, b = 0,
     // breakpoints at 1e300) and l = m = 2, so it takes the same path: every map

@@ -30,7 +30,7 @@ namespace {
 
 constexpr int EVAL_THREADS = 256;
 
-__global__ void k_resp_setup(RespRec* __restrict__ recs, const int* __restrict__ ells, const int* __restrict__ mms,
+__global__ __launch_bounds__(64) void k_resp_setup(RespRec* __restrict__ recs, const int* __restrict__ ells, const int* __restrict__ mms,
                              const double* __restrict__ phi_ref, const double* __restrict__ inc,
                              const double* __restrict__ lam, const double* __restrict__ beta,
                              const double* __restrict__ psi, int numModes, int numBinAll)
