@@ -11,18 +11,22 @@
 //                (Dual<4> instantiation of the set-up code), plus the
 //                polarisation basis, wave vector and Y factors with their
 //                derivatives in lam, beta (analytic) and cos inc.
-// k_fused_grad   one 64-lane wavefront per template:
+// k_grad_weights one 64-lane wavefront per template:
 //   1. forward: residuals r = h / h0 on the K nodes (LDS) and d_h, h_h,
 //      exactly as the forward kernel;
 //   2. adjoint of the heterodyned sums: for each node and channel the complex
 //      weight g with  d logL = sum Re(g dh)  (from the segment formulas
-//      of Likelihood.cu:91-103 and r = h / h0);
+//      of Likelihood.cu:91-103 and r = h / h0), written out (48 K bytes);
+// k_grad_points  one 64-lane wavefront per (template, mode), lane per node:
 //   3. per (node, mode) tangents: PhenomHM amplitude, phase and t(f) with
 //      respect to the intrinsic parameters (Dual<4> evaluation of the same
 //      phm_eval code), the response by an adjoint of the (linear) TDI
 //      combination and forward tangents of the six single-link terms
 //      G_ij in t, lam, beta, psi and cos inc; contracted with g at once, so
-//      each lane keeps 11 accumulators; the wavefront reduces them.
+//      each lane keeps 11 accumulators; the wavefront reduces them;
+// k_grad_reduce  the modes' partial gradients, summed in mode order.
+// Splitting the forward pass from the per-point tangents keeps each kernel's
+// register footprint (and so its occupancy) to its own needs.
 // The spline and response are recomputed, nothing per node is stored.
 
 #include "bbhx_hip.h"
@@ -384,19 +388,16 @@ __device__ inline void grad_point(const GradRec& G, const resp::OrbitTab& o, dou
     acc[G_TREF] += Q[0].re - Av * ImS * (2. * PI * f + pRd_t);
 }
 
-__global__ __launch_bounds__(64) void k_fused_grad(double* __restrict__ d_h, double* __restrict__ h_h,
-                                                   double* __restrict__ grad, const FusedRec* __restrict__ recs,
-                                                   const GradRec* __restrict__ grecs,
-                                                   const double* __restrict__ params,
-                                                   const double* __restrict__ t_start,
-                                                   const double* __restrict__ t_end, int B, int M,
-                                                   const double* __restrict__ freqs, const cd* __restrict__ h0,
-                                                   const cd* __restrict__ dc, int K, resp::OrbitTab orb, int TDItag,
-                                                   int rescaled, int tdi2, int jend, double initial_t_val)
+// Forward pass and adjoint weights of one template (one wavefront): d_h, h_h
+// as the forward kernel computes them, and gw[(b * 3 + c) * K + j] = the
+// complex weight g of node j, channel c.
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_grad_weights(
+    double* __restrict__ d_h, double* __restrict__ h_h, cd* __restrict__ gw, const FusedRec* __restrict__ recs,
+    const double* __restrict__ params, const double* __restrict__ t_start, const double* __restrict__ t_end, int B,
+    int M, const double* __restrict__ freqs, const cd* __restrict__ h0, const cd* __restrict__ dc, int K,
+    resp::OrbitTab orb, int TDItag, int rescaled, int tdi2, int jend, double initial_t_val)
 {
-    extern __shared__ cd lds[];  // residuals [3][K], then adjoint weights [3][K]
-    cd* rs = lds;
-    cd* gs = lds + 3 * K;
+    extern __shared__ cd rs[];  // residuals [3][K]
     const int b = blockIdx.x;
     Window win;
     win.t_ref = params[P_TREF * B + b];
@@ -407,26 +408,55 @@ __global__ __launch_bounds__(64) void k_fused_grad(double* __restrict__ d_h, dou
     __syncthreads();
     double sums[4];
     segment_sums(rs, freqs, dc, K, jend, sums);
-    adjoint_weights(gs, rs, freqs, h0, dc, K, jend);
-    __syncthreads();
-
-    const double dist = params[P_DIST * B + b];
-    double acc[NGRAD];
-    for (int p = 0; p < NGRAD; p++) acc[p] = 0.0;
-    const GradRec* gb = grecs + (long)b * M;
-    for (int j = threadIdx.x; j < K; j += 64) {
-        const double f = freqs[j];
-        const cd g[3] = {gs[j], gs[K + j], gs[2 * K + j]};
-        for (int mi = 0; mi < M; mi++) grad_point(gb[mi], orb, f, win, g, TDItag, rescaled, tdi2, dist, acc);
-    }
-    for (int p = 0; p < NGRAD; p++) acc[p] = wsum(acc[p]);
+    adjoint_weights(gw + (long)b * 3 * K, rs, freqs, h0, dc, K, jend);
     if (threadIdx.x == 0) {
         d_h[2 * b] = sums[0];
         d_h[2 * b + 1] = sums[1];
         h_h[2 * b] = sums[2];
         h_h[2 * b + 1] = sums[3];
-        for (int p = 0; p < NGRAD; p++) grad[(long)p * B + b] = acc[p];
     }
+}
+
+// Gradient contributions of one (template, mode): one wavefront, lane l owns
+// nodes l, l+64, ... (the record is the same for the whole wavefront);
+// partial[(b * M + m) * 11 + p].
+__global__ __launch_bounds__(64) void k_grad_points(double* __restrict__ partial, const GradRec* __restrict__ grecs,
+                                                    const cd* __restrict__ gw, const double* __restrict__ params,
+                                                    const double* __restrict__ t_start,
+                                                    const double* __restrict__ t_end, int B, int M,
+                                                    const double* __restrict__ freqs, int K, resp::OrbitTab orb,
+                                                    int TDItag, int rescaled, int tdi2, double initial_t_val)
+{
+    const int bm = blockIdx.x;
+    const int b = bm / M;
+    Window win;
+    win.t_ref = params[P_TREF * B + b];
+    win.ts = t_start[b];
+    win.te = t_end[b];
+    win.dt_shift = win.t_ref - initial_t_val;
+    const double dist = params[P_DIST * B + b];
+    const GradRec& G = grecs[bm];
+    const cd* g_b = gw + (long)b * 3 * K;
+    double acc[NGRAD];
+    for (int p = 0; p < NGRAD; p++) acc[p] = 0.0;
+    for (int j = threadIdx.x; j < K; j += 64) {
+        const cd g[3] = {g_b[j], g_b[K + j], g_b[2 * K + j]};
+        grad_point(G, orb, freqs[j], win, g, TDItag, rescaled, tdi2, dist, acc);
+    }
+    for (int p = 0; p < NGRAD; p++) acc[p] = wsum(acc[p]);
+    if (threadIdx.x == 0)
+        for (int p = 0; p < NGRAD; p++) partial[(long)bm * NGRAD + p] = acc[p];
+}
+
+// grad[p * B + b] = sum over modes, in mode order
+__global__ void k_grad_reduce(double* __restrict__ grad, const double* __restrict__ partial, int B, int M)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= B * NGRAD) return;
+    const int p = t / B, b = t % B;
+    double s = 0.0;
+    for (int m = 0; m < M; m++) s += partial[((long)b * M + m) * NGRAD + p];
+    grad[(long)p * B + b] = s;
 }
 
 }  // namespace
@@ -441,8 +471,7 @@ extern "C" int bbhx_hip_hdyn_fused_grad(double* d_h, double* h_h, double* grad, 
 {
     resp::OrbitTab orb;
     if (!response_orbit_tab(orbits, orb)) return ERR_ARGS;
-    // K <= 512 keeps the two LDS arrays (96 K bytes) within 48 KiB per workgroup
-    if (numBinAll < 0 || K < 2 || K > 512 || table_len < 2) return ERR_ARGS;
+    if (numBinAll < 0 || K < 2 || K > 1024 || table_len < 2) return ERR_ARGS;
     ModeList ml;
     const int mst = check_modes(ells_host, mms_host, numModes, ml);
     if (mst != OK) return mst;
@@ -453,7 +482,13 @@ extern "C" int bbhx_hip_hdyn_fused_grad(double* d_h, double* h_h, double* grad, 
     const int nrec = numBinAll * numModes;
     FusedRec* recs = static_cast<FusedRec*>(workspace_get(WS_FUSED, sizeof(FusedRec) * (size_t)nrec));
     GradRec* grecs = static_cast<GradRec*>(workspace_get(WS_GRAD, sizeof(GradRec) * (size_t)nrec));
-    if (!recs || !grecs) return ERR_ALLOC;
+    // adjoint weights (B, 3, K) complex, then partial gradients (B, M, 11)
+    const size_t gw_bytes = sizeof(cd) * 3 * (size_t)K * numBinAll;
+    char* scratch = static_cast<char*>(
+        workspace_get(WS_GRAD_AUX, gw_bytes + sizeof(double) * NGRAD * (size_t)nrec));
+    if (!recs || !grecs || !scratch) return ERR_ALLOC;
+    cd* gw = reinterpret_cast<cd*>(scratch);
+    double* partial = reinterpret_cast<double*>(scratch + gw_bytes);
     hipEvent_t ev[3];
     const bool timed = timer_take3(ev);
     if (timed) (void)hipEventRecord(ev[0], st);
@@ -462,11 +497,14 @@ extern "C" int bbhx_hip_hdyn_fused_grad(double* d_h, double* h_h, double* grad, 
     if (hipGetLastError() != hipSuccess) return ERR_LAUNCH;
     if (timed) (void)hipEventRecord(ev[1], st);
     const int jend = include_last_segment ? K : K - 1;
-    const size_t lds = sizeof(cd) * 6 * (size_t)K;
-    hipLaunchKernelGGL(k_fused_grad, dim3(numBinAll), dim3(64), lds, st, d_h, h_h, grad, recs, grecs, params, t_start,
-                       t_end, numBinAll, numModes, freqs, reinterpret_cast<const cd*>(h0),
+    hipLaunchKernelGGL(k_grad_weights, dim3(numBinAll), dim3(64), sizeof(cd) * 3 * (size_t)K, st, d_h, h_h, gw, recs,
+                       params, t_start, t_end, numBinAll, numModes, freqs, reinterpret_cast<const cd*>(h0),
                        reinterpret_cast<const cd*>(dataConstants), K, orb, TDItag, rescaled, tdi2, jend,
                        initial_t_val);
+    hipLaunchKernelGGL(k_grad_points, dim3(nrec), dim3(64), 0, st, partial, grecs, gw, params, t_start, t_end,
+                       numBinAll, numModes, freqs, K, orb, TDItag, rescaled, tdi2, initial_t_val);
+    hipLaunchKernelGGL(k_grad_reduce, dim3(ceil_div((long)numBinAll * NGRAD, 256)), dim3(256), 0, st, grad, partial,
+                       numBinAll, numModes);
     if (timed) (void)hipEventRecord(ev[2], st);
     return hipGetLastError() == hipSuccess ? OK : ERR_LAUNCH;
 }

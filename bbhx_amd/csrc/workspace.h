@@ -20,7 +20,8 @@ enum WsSlot : int {
     WS_PARTIAL = 3,
     WS_FUSED = 4,
     WS_GRAD = 5,
-    WS_SLOTS = 6,
+    WS_GRAD_AUX = 6,
+    WS_SLOTS = 7,
 };
 
 // Returns a device pointer with at least 'bytes' bytes on the current device,

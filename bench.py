@@ -122,12 +122,17 @@ def bench_direct_ragged(args):
     from bbhx_amd.waveformbuild import BBHWaveformFD
     from tests.params import MODES, draw
 
-    n = int(YRSID_SI // 10)
-    f = np.fft.rfftfreq(n, 10.0)[1:]
+    T = args.direct_years * YRSID_SI
+    n = int(T // args.direct_cadence)
+    f = np.fft.rfftfreq(n, args.direct_cadence)[1:]
     if args.direct_bins:
         f = f[:args.direct_bins]
-    p = draw(args.batch, 1002, mtot=(1e4, 1e8))
-    lengths = ragged_lengths(p, YRSID_SI)
+    if args.uniform_length:
+        p = draw(args.batch, 1005)
+        lengths = np.full(p.shape[1], args.uniform_length, dtype=np.int32)
+    else:
+        p = draw(args.batch, 1002, mtot=(1e4, 1e8))
+        lengths = ragged_lengths(p, YRSID_SI)
     log(f"uneven batch: {p.shape[1]} templates, L in [{lengths.min()}, {lengths.max()}], sum {lengths.sum()}, "
         f"{f.size} dense bins")
     orbits = EqualArmlengthOrbits(dt=1e4, t_max=3.0 * YRSID_SI)
@@ -174,7 +179,7 @@ def bench_direct_ragged(args):
     log(f"graph replay: {ms:.3f} ms/step, {B / (ms * 1e-3):.4g} evals/s; |graph - eager| = {diff:.2e}; "
         f"plan {plan.nbytes() / 2**30:.2f} GiB")
     out = {
-        "metric": "direct logL evals/sec, uneven batch, 1 GPU",
+        "metric": "direct logL evals/sec, 1 GPU",
         "value": B / (ms * 1e-3),
         "unit": "logL evals/s",
         "n_gpus": 1,
@@ -186,7 +191,9 @@ def bench_direct_ragged(args):
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded injection + Gaussian noise, analytic A/E/T PSD)",
-        "config": {"workload": "direct logL, uneven batch, fused, hipGraph replay", "templates": B,
+        "config": {"workload": ("direct logL, uneven batch" if not args.uniform_length else "direct logL, large")
+                   + ", fused, hipGraph replay", "templates": B, "data_years": args.direct_years,
+                   "cadence_s": args.direct_cadence,
                    "lengths_min": int(lengths.min()), "lengths_max": int(lengths.max()),
                    "lengths_sum": int(lengths.sum()), "dense_bins": int(f.size), "modes": 6, "tdi": "AET tdi2",
                    "launch": {k: tuned[k] for k in ("chunk", "block", "grid")}, "graph_vs_eager": diff},
@@ -206,7 +213,12 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="heterodyned", choices=("heterodyned", "direct-ragged"))
-    ap.add_argument("--direct-bins", type=int, default=0, help="truncate the 1 yr / 10 s data grid (testing)")
+    ap.add_argument("--direct-bins", type=int, default=0, help="truncate the dense data grid (testing)")
+    ap.add_argument("--direct-years", type=float, default=1.0, help="direct workloads: data span [yr]")
+    ap.add_argument("--direct-cadence", type=float, default=10.0, help="direct workloads: sampling cadence [s]")
+    ap.add_argument("--uniform-length", type=int, default=0,
+                    help="direct workloads: same sparse length for every template (large direct: 4096, with "
+                         "--batch 512 --direct-years 4 --direct-cadence 5)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB device-to-device copy (reference kernel for HBM byte counters)")
     args = ap.parse_args()

@@ -3,12 +3,12 @@ finite differences of the FP64 CPU oracle's heterodyned log-likelihood.
 
 The oracle is evaluated at x +- h e_k and x +- 2h e_k (fourth-order central
 difference) in the sampling coordinates (ln M, q, chi1z, chi2z, distance,
-phi_ref, cos inc, lam, sin beta, psi, t_ref). Steps are fixed per coordinate so
-that truncation and round-off of the difference are both far below the
-tolerance: |g_gpu - g_fd| <= 1e-5 |g_fd| per component, or, for components
-that are small against the template's own gradient scale (where the
-difference itself is only known to round-off), <= 1e-5 times the
-difference's round-off bound.
+phi_ref, cos inc, lam, sin beta, psi, t_ref). The oracle's logL carries a
+round-off of about 3e-13 |logL| (sums of terms up to 1e6 at |logL| ~ 1e5), so
+the steps are chosen large enough that this noise, amplified by 1/h, stays
+well below the tolerance, and small enough that the O(h^4) truncation does too
+(checked by varying h over four decades): |g_gpu - g_fd| <= 1e-5 |g_fd| per
+component.
 """
 
 import numpy as np
@@ -20,8 +20,8 @@ from tests.params import MODES, YRSID_SI, draw
 pytestmark = pytest.mark.gpu
 
 # finite-difference steps per sampling coordinate
-STEPS = np.array([1e-8, 1e-7, 1e-7, 1e-7, 0.0, 1e-6, 1e-6, 1e-6, 1e-6, 1e-6, 1e-2])
-REL_DIST_STEP = 1e-7
+STEPS = np.array([1e-6, 1e-5, 1e-5, 1e-5, 0.0, 1e-5, 1e-5, 1e-5, 1e-5, 1e-5, 1e-1])
+REL_DIST_STEP = 1e-5
 
 
 def to_sampling(p):
@@ -100,8 +100,8 @@ def test_grad_against_oracle_finite_differences(grad_problem):
     llx = oracle_ll(ohet, X)
     fd = (-ll[:, 0] + 8 * ll[:, 1] - 8 * ll[:, 2] + ll[:, 3]) / (12 * steps.T)  # (11, B)
     fd = fd.T
-    # round-off bound of the difference: |logL| eps_sum / h, eps_sum ~ 1e-13 for these sums
-    roundoff = 1e-13 * np.maximum(np.abs(llx), 1.0)[:, None] / steps
+    # round-off of the difference: the oracle's logL noise (3e-13 |logL|) amplified by ~1.5 / h
+    roundoff = 1.5 * 3e-13 * np.maximum(np.abs(llx), 1.0)[:, None] / steps
     err = np.abs(grad - fd)
     rel = err / np.maximum(np.abs(fd), 1e-300)
     names = ("lnM", "q", "chi1", "chi2", "dist", "phi", "cosi", "lam", "sinb", "psi", "tref")
@@ -110,7 +110,8 @@ def test_grad_against_oracle_finite_differences(grad_problem):
         for k in range(11):
             print(f"  {names[k]:5s} gpu {grad[b, k]: .9e} fd {fd[b, k]: .9e} rel {rel[b, k]:.2e} "
                   f"roundoff {roundoff[b, k]:.1e}", flush=True)
-    ok = (err <= 1e-5 * np.abs(fd)) | (err <= 10 * roundoff)
+    assert np.all(roundoff <= 1e-6 * np.abs(fd)), "step too small for this template"
+    ok = err <= 1e-5 * np.abs(fd)
     assert np.all(ok), np.argwhere(~ok)
 
 
