@@ -114,14 +114,21 @@ __device__ inline void grad_resp_setup(GradResp& r, int l, int m, double phi, do
 }
 
 __global__ __launch_bounds__(64) void k_grad_setup(FusedRec* __restrict__ recs, GradRec* __restrict__ grecs,
-                             const double* __restrict__ params, int B, ModeList modes, int M,
-                             const double* __restrict__ qnm, int table_len, double dspin)
+                                                   const double* __restrict__ params, int B, ModeList modes, int M,
+                                                   const double* __restrict__ qnm, int table_len, double dspin)
 {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= B * M) return;
+    // lanes [0, B M) build the double records, lanes [B M, 2 B M) the records
+    // with tangents: the two set-ups run side by side
+    const int gidx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nrec = B * M;
+    if (gidx >= 2 * nrec) return;
+    const int idx = gidx < nrec ? gidx : gidx - nrec;
     const int b = idx / M, mi = idx % M;
     const int l = modes.ell[mi], m = modes.mm[mi];
-    setup_one(recs[idx], params, B, b, l, m, qnm, table_len, dspin);
+    if (gidx < nrec) {
+        setup_one(recs[idx], params, B, b, l, m, qnm, table_len, dspin);
+        return;
+    }
     // masses from (ln M, q): m1 = M q / (1 + q), m2 = M / (1 + q)
     const double m1 = params[P_M1 * B + b], m2 = params[P_M2 * B + b];
     const double Mt = m1 + m2, q = m1 / m2;
@@ -492,7 +499,7 @@ extern "C" int bbhx_hip_hdyn_fused_grad(double* d_h, double* h_h, double* grad, 
     hipEvent_t ev[3];
     const bool timed = timer_take3(ev);
     if (timed) (void)hipEventRecord(ev[0], st);
-    hipLaunchKernelGGL(k_grad_setup, dim3(ceil_div(nrec, 64)), dim3(64), 0, st, recs, grecs, params, numBinAll, ml,
+    hipLaunchKernelGGL(k_grad_setup, dim3(ceil_div(2 * (long)nrec, 64)), dim3(64), 0, st, recs, grecs, params, numBinAll, ml,
                        numModes, qnm, table_len, dspin);
     if (hipGetLastError() != hipSuccess) return ERR_LAUNCH;
     if (timed) (void)hipEventRecord(ev[1], st);
