@@ -6,7 +6,8 @@
 //
 // Per binary b with sparse length L_b (layout: points of all binaries
 // concatenated, offset off_b; within a binary [i][mode] for the grid and
-// [i][mode][quantity] for the nine spline quantities):
+// [i][mode][quantity][y, c1, c2, c3] for the nine spline quantities, so that
+// one segment of one quantity is 32 contiguous bytes):
 //   k_dir_setup   one lane per (binary, mode): coefficient records
 //   k_dir_grid    one wavefront per (binary, mode): default grid
 //                 Mf in [mf_min, mf_max] * m / 2, t(f) there, the crossings of
@@ -49,12 +50,9 @@ struct bbhx_direct_plan {
     int* len = nullptr;    // (B,)
     FusedRec* recs = nullptr;
     double* X = nullptr;   // (S, M)
-    double* Y = nullptr;   // (S, M, 9)
-    double* C1 = nullptr;
-    double* C2 = nullptr;
-    double* C3 = nullptr;
+    double* Cf = nullptr;  // (S, M, 9, 4): value and cubic coefficients
     double* grid_lo = nullptr;    // (B, M) log10 of the first refitted grid point
-    double* grid_step = nullptr;  // (B, M) log10 step
+    double* grid_istep = nullptr; // (B, M) 1 / log10 step (0 for a degenerate grid)
     int* band = nullptr;          // (B, 3): st, et, chunk_begin
     int* total = nullptr;         // (1,) number of work items
     double* partial = nullptr;    // (max_items, 4)
@@ -63,6 +61,11 @@ struct bbhx_direct_plan {
 namespace {
 
 constexpr int NQ = 9;
+
+// one spline segment of one quantity: value at the knot and c1, c2, c3
+struct alignas(32) Cubic {
+    double y, c1, c2, c3;
+};
 
 __global__ __launch_bounds__(64) void k_dir_setup(FusedRec* __restrict__ recs, const double* __restrict__ params, int B, ModeList modes,
                             int M, const double* __restrict__ qnm, int table_len, double dspin)
@@ -105,8 +108,8 @@ __device__ inline double wmax_d(double v)
 }
 
 // One 64-lane wavefront per (binary, mode). LDS: t(f) of the default grid (L_b doubles).
-__global__ __launch_bounds__(64) void k_dir_grid(double* __restrict__ X, double* __restrict__ Y,
-                                                 double* __restrict__ grid_lo, double* __restrict__ grid_step,
+__global__ __launch_bounds__(64) void k_dir_grid(double* __restrict__ X, double* __restrict__ Cf,
+                                                 double* __restrict__ grid_lo, double* __restrict__ grid_istep,
                                                  int* __restrict__ flags, const FusedRec* __restrict__ recs,
                                                  const long* __restrict__ off, const int* __restrict__ len,
                                                  const double* __restrict__ params,
@@ -189,7 +192,7 @@ __global__ __launch_bounds__(64) void k_dir_grid(double* __restrict__ X, double*
     const double lo2 = log10(f_min), hi2 = log10(f_max), step2 = (hi2 - lo2) / (L - 1);
     if (lane == 0) {
         grid_lo[bm] = lo2;
-        grid_step[bm] = step2;
+        grid_istep[bm] = step2 > 0.0 ? 1.0 / step2 : 0.0;
     }
     const double dt_shift = t_ref - initial_t_val;
     for (int i = lane; i < L; i += 64) {
@@ -204,26 +207,24 @@ __global__ __launch_bounds__(64) void k_dir_grid(double* __restrict__ X, double*
         phase = phase + dphi;
         const long p = (base + i) * M + m;
         X[p] = f;
-        double* y = Y + p * NQ;
-        y[0] = amp;
-        y[1] = phase;
-        y[2] = tf;
-        y[3] = T[0].re;
-        y[4] = T[0].im;
-        y[5] = T[1].re;
-        y[6] = T[1].im;
-        y[7] = T[2].re;
-        y[8] = T[2].im;
+        double* y = Cf + p * NQ * 4;
+        y[0 * 4] = amp;
+        y[1 * 4] = phase;
+        y[2 * 4] = tf;
+        y[3 * 4] = T[0].re;
+        y[4 * 4] = T[0].im;
+        y[5 * 4] = T[1].re;
+        y[6 * 4] = T[1].im;
+        y[7 * 4] = T[2].re;
+        y[8 * 4] = T[2].im;
     }
 }
 
 // Not-a-knot spline of one binary: lane s = mode * 9 + quantity runs the
 // Thomas algorithm over the binary's L knots (the system of spline.hip);
 // consecutive lanes read consecutive quantities of the same knot.
-__global__ __launch_bounds__(64) void k_dir_spline(const double* __restrict__ X, const double* __restrict__ Y,
-                                                   double* __restrict__ C1, double* __restrict__ C2,
-                                                   double* __restrict__ C3, const long* __restrict__ off,
-                                                   const int* __restrict__ len, int M)
+__global__ __launch_bounds__(64) void k_dir_spline(const double* __restrict__ X, double* __restrict__ Cf,
+                                                   const long* __restrict__ off, const int* __restrict__ len, int M)
 {
     const int b = blockIdx.x;
     const int s = threadIdx.x;
@@ -233,10 +234,11 @@ __global__ __launch_bounds__(64) void k_dir_spline(const double* __restrict__ X,
     const long base = off[b];
     // element i of this system
     auto xi = [&](int i) { return X[(base + i) * M + m]; };
-    auto yi = [&](int i) { return Y[(base + i) * M * NQ + s]; };
-    auto at = [&](int i) { return (base + i) * M * NQ + s; };
-    double* dp = C1;  // d' then the slopes (= c1)
-    double* cp = C3;  // c' then c3
+    auto yi = [&](int i) { return Cf[((base + i) * M * NQ + s) * 4]; };
+    auto at = [&](int i) { return ((base + i) * M * NQ + s) * 4; };
+    double* dp = Cf + 1;  // d' then the slopes (= c1)
+    double* C2 = Cf + 2;
+    double* cp = Cf + 3;  // c' then c3
 
     double x0 = xi(0), x1 = xi(1), x2 = xi(2);
     double y0 = yi(0), y1 = yi(1), y2 = yi(2);
@@ -369,10 +371,9 @@ __device__ inline int owner(const int* __restrict__ band, int B, int it)
 // over the dense bins of work item it, where h~ = w h (Likelihood.cu:357-375).
 __global__ __launch_bounds__(256) void k_dir_like(double* __restrict__ partial, const int* __restrict__ band,
                                                   const int* __restrict__ total, const double* __restrict__ X,
-                                                  const double* __restrict__ Y, const double* __restrict__ C1,
-                                                  const double* __restrict__ C2, const double* __restrict__ C3,
+                                                  const Cubic* __restrict__ Cf,
                                                   const double* __restrict__ grid_lo,
-                                                  const double* __restrict__ grid_step,
+                                                  const double* __restrict__ grid_istep,
                                                   const long* __restrict__ off, const int* __restrict__ len,
                                                   const double* __restrict__ t_start,
                                                   const double* __restrict__ t_end, const double* __restrict__ freqs,
@@ -399,8 +400,8 @@ __global__ __launch_bounds__(256) void k_dir_like(double* __restrict__ partial, 
                 const double* xm = X + base * M + m;
                 // segment: searchsorted(x, f, 'right') - 1, first estimated from the
                 // log-spaced grid, then made exact against the stored knots
-                const double gst = grid_step[b * M + m];
-                const double u = gst > 0.0 ? (lf - grid_lo[b * M + m]) / gst : -2.0;
+                const double gis = grid_istep[b * M + m];
+                const double u = gis > 0.0 ? (lf - grid_lo[b * M + m]) * gis : -2.0;
                 int i = !(u >= -1.0) ? -1 : (u >= (double)(L - 1) ? L - 1 : (int)floor(u));
                 while (i + 1 < L && xm[(long)(i + 1) * M] <= f) i++;
                 while (i >= 0 && xm[(long)i * M] > f) i--;
@@ -411,8 +412,8 @@ __global__ __launch_bounds__(256) void k_dir_like(double* __restrict__ partial, 
                 double v[NQ];
 #pragma unroll
                 for (int q = 0; q < NQ; q++) {
-                    const long a = p * NQ + q;
-                    v[q] = Y[a] + C1[a] * x + C2[a] * x2 + C3[a] * x3;
+                    const Cubic c = Cf[p * NQ + q];
+                    v[q] = c.y + c.c1 * x + c.c2 * x2 + c.c3 * x3;
                 }
                 const double tf = v[2];
                 if ((tf >= ts) && ((tf <= te) || (te <= 0.0))) {
@@ -491,7 +492,7 @@ T* dalloc(size_t n, bool& ok)
 
 void plan_free(bbhx_direct_plan* p)
 {
-    void* bufs[] = {p->off, p->len, p->recs, p->X, p->Y, p->C1, p->C2, p->C3, p->grid_lo, p->grid_step, p->band,
+    void* bufs[] = {p->off, p->len, p->recs, p->X, p->Cf, p->grid_lo, p->grid_istep, p->band,
                     p->total, p->partial};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
@@ -535,12 +536,9 @@ extern "C" int bbhx_hip_direct_plan_create(bbhx_direct_plan** out, int numBinAll
     p->len = dalloc<int>(numBinAll, ok);
     p->recs = dalloc<FusedRec>((size_t)numBinAll * numModes, ok);
     p->X = dalloc<double>(SM, ok);
-    p->Y = dalloc<double>(SM * NQ, ok);
-    p->C1 = dalloc<double>(SM * NQ, ok);
-    p->C2 = dalloc<double>(SM * NQ, ok);
-    p->C3 = dalloc<double>(SM * NQ, ok);
+    p->Cf = dalloc<double>(SM * NQ * 4, ok);
     p->grid_lo = dalloc<double>((size_t)numBinAll * numModes, ok);
-    p->grid_step = dalloc<double>((size_t)numBinAll * numModes, ok);
+    p->grid_istep = dalloc<double>((size_t)numBinAll * numModes, ok);
     p->band = dalloc<int>((size_t)numBinAll * 3, ok);
     p->total = dalloc<int>(1, ok);
     p->partial = dalloc<double>((size_t)p->max_items * 4, ok);
@@ -603,15 +601,15 @@ extern "C" int bbhx_hip_direct_fused(bbhx_direct_plan* p, double* d_h, double* h
     if (timed) (void)hipEventRecord(ev[0], st);
     hipLaunchKernelGGL(k_dir_setup, dim3(ceil_div((long)B * M, 64)), dim3(64), 0, st, p->recs, params, B, ml, M, qnm,
                        table_len, dspin);
-    hipLaunchKernelGGL(k_dir_grid, dim3(B * M), dim3(64), sizeof(double) * (size_t)p->Lmax, st, p->X, p->Y,
-                       p->grid_lo, p->grid_step, flags, p->recs, p->off, p->len, params, t_start, t_end, B, M, ml,
+    hipLaunchKernelGGL(k_dir_grid, dim3(B * M), dim3(64), sizeof(double) * (size_t)p->Lmax, st, p->X, p->Cf,
+                       p->grid_lo, p->grid_istep, flags, p->recs, p->off, p->len, params, t_start, t_end, B, M, ml,
                        orb, TDItag, rescaled, tdi2, initial_t_val, mf_min, mf_max);
-    hipLaunchKernelGGL(k_dir_spline, dim3(B), dim3(64), 0, st, p->X, p->Y, p->C1, p->C2, p->C3, p->off, p->len, M);
+    hipLaunchKernelGGL(k_dir_spline, dim3(B), dim3(64), 0, st, p->X, p->Cf, p->off, p->len, M);
     hipLaunchKernelGGL(k_dir_band, dim3(1), dim3(1024), 0, st, p->band, p->total, p->X, p->off, p->len, dataFreqs,
                        p->N, B, M, chunk);
     if (timed) (void)hipEventRecord(ev[1], st);
-    hipLaunchKernelGGL(k_dir_like, dim3(grid), dim3(block), 0, st, p->partial, p->band, p->total, p->X, p->Y,
-                       p->C1, p->C2, p->C3, p->grid_lo, p->grid_step, p->off, p->len, t_start, t_end, dataFreqs,
+    hipLaunchKernelGGL(k_dir_like, dim3(grid), dim3(block), 0, st, p->partial, p->band, p->total, p->X,
+                       reinterpret_cast<const Cubic*>(p->Cf), p->grid_lo, p->grid_istep, p->off, p->len, t_start, t_end, dataFreqs,
                        reinterpret_cast<const cd*>(dataChannels), noiseFactors, p->N, B, M, chunk);
     hipLaunchKernelGGL(k_dir_reduce, dim3(ceil_div(B, 256)), dim3(256), 0, st, d_h, h_h, p->partial, p->band,
                        p->total, B);
