@@ -32,10 +32,11 @@ def _signature(lengths, num_modes, data_length, device_name):
             f"Lsum={int(lengths.sum())}|Lmax={int(lengths.max())}")
 
 
-def tune_direct(like, params, lengths, modes=None, reps=3, cache_path=None, candidates=None):
+def tune_direct(like, params, lengths, modes=None, reps=3, cache_path=None, candidates=None, progress=None):
     """Choose (chunk, block, grid) for ``like.get_ll_fused`` on this batch.
 
-    like: a bbhx_amd.likelihood.Likelihood; params (12, B); lengths (B,).
+    like: a bbhx_amd.likelihood.Likelihood; params (12, B); lengths (B,);
+    progress: optional callable(chunk, block, grid, ms) called after each candidate.
     Returns dict(chunk=, block=, grid=, ms=, table=[(chunk, block, grid, ms), ...]).
     """
     import torch
@@ -77,6 +78,8 @@ def tune_direct(like, params, lengths, modes=None, reps=3, cache_path=None, cand
         e1.record()
         torch.cuda.synchronize()
         table.append((chunk, block, grid, e0.elapsed_time(e1) / reps))
+        if progress is not None:
+            progress(*table[-1])
     del plan
     best = min(table, key=lambda t: t[3])
     res = dict(chunk=best[0], block=best[1], grid=best[2], ms=best[3], table=table)

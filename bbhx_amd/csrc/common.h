@@ -18,6 +18,7 @@ constexpr double PI = 3.141592653589793238462643383279502884;
 constexpr double PI_2 = 1.570796326794896619231321691639751442;
 constexpr double PI_4 = 0.785398163397448309615660845819875721;
 constexpr double TWO_PI = 6.283185307179586476925286766559005768;
+constexpr double LOG2_10 = 3.321928094887362347870319429489390176;
 constexpr double GAMMA_E = 0.577215664901532860606512090082402431;
 constexpr double SQRT2 = 1.414213562373095048801688724209698079;
 constexpr double INVSQRT3 = 0.577350269189625764509148780501957455;

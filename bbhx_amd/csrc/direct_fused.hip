@@ -6,8 +6,9 @@
 //
 // Per binary b with sparse length L_b (layout: points of all binaries
 // concatenated, offset off_b; within a binary [i][mode] for the grid and
-// [i][mode][quantity][y, c1, c2, c3] for the nine spline quantities, so that
-// one segment of one quantity is 32 contiguous bytes):
+// [i][mode] segment records: the knots x_i, x_{i+1} that bound segment i, then
+// per quantity [y, c1, c2, c3], so that one segment of one quantity is 32
+// contiguous bytes and one record read both checks and evaluates the segment):
 //   k_dir_setup   one lane per (binary, mode): coefficient records
 //   k_dir_grid    one wavefront per (binary, mode): default grid
 //                 Mf in [mf_min, mf_max] * m / 2, t(f) there, the crossings of
@@ -50,9 +51,9 @@ struct bbhx_direct_plan {
     int* len = nullptr;    // (B,)
     FusedRec* recs = nullptr;
     double* X = nullptr;   // (S, M)
-    double* Cf = nullptr;  // (S, M, 9, 4): value and cubic coefficients
-    double* grid_lo = nullptr;    // (B, M) log10 of the first refitted grid point
-    double* grid_istep = nullptr; // (B, M) 1 / log10 step (0 for a degenerate grid)
+    double* Cf = nullptr;  // (S, M, RS): segment records (bounds, value and cubic coefficients)
+    double* grid_lo = nullptr;    // (B, M) log2 of the first refitted grid point
+    double* grid_istep = nullptr; // (B, M) 1 / log2 step (0 for a degenerate grid)
     int* band = nullptr;          // (B, 3): st, et, chunk_begin
     int* total = nullptr;         // (1,) number of work items
     double* partial = nullptr;    // (max_items, 4)
@@ -66,6 +67,19 @@ constexpr int NQ = 9;
 struct alignas(32) Cubic {
     double y, c1, c2, c3;
 };
+
+// segment i of one (binary, mode): it covers x_lo <= f < x_hi (x_hi = x_lo for
+// the last knot, which starts no segment)
+struct alignas(32) SegRec {
+    double x_lo, x_hi, pad0, pad1;
+    Cubic q[NQ];
+};
+constexpr int RS = sizeof(SegRec) / sizeof(double);  // 40 doubles
+constexpr int RQ = 4;                                 // offset of q[0] in doubles
+static_assert(sizeof(SegRec) == 320, "segment record layout");
+
+__device__ inline const Cubic& cubic_of(const SegRec& r, int k) { return r.q[k]; }
+__device__ inline const Cubic& cubic_of(const Cubic (&c)[NQ], int k) { return c[k]; }
 
 __global__ __launch_bounds__(64) void k_dir_setup(FusedRec* __restrict__ recs, const double* __restrict__ params, int B, ModeList modes,
                             int M, const double* __restrict__ qnm, int table_len, double dspin)
@@ -191,8 +205,9 @@ __global__ __launch_bounds__(64) void k_dir_grid(double* __restrict__ X, double*
     // refitted grid and the nine quantities on it
     const double lo2 = log10(f_min), hi2 = log10(f_max), step2 = (hi2 - lo2) / (L - 1);
     if (lane == 0) {
-        grid_lo[bm] = lo2;
-        grid_istep[bm] = step2 > 0.0 ? 1.0 / step2 : 0.0;
+        // log2 units: the dense kernel estimates the segment from an FP32 log2
+        grid_lo[bm] = lo2 * LOG2_10;
+        grid_istep[bm] = step2 > 0.0 ? 1.0 / (step2 * LOG2_10) : 0.0;
     }
     const double dt_shift = t_ref - initial_t_val;
     for (int i = lane; i < L; i += 64) {
@@ -207,7 +222,7 @@ __global__ __launch_bounds__(64) void k_dir_grid(double* __restrict__ X, double*
         phase = phase + dphi;
         const long p = (base + i) * M + m;
         X[p] = f;
-        double* y = Cf + p * NQ * 4;
+        double* y = Cf + p * RS + RQ;
         y[0 * 4] = amp;
         y[1 * 4] = phase;
         y[2 * 4] = tf;
@@ -234,8 +249,9 @@ __global__ __launch_bounds__(64) void k_dir_spline(const double* __restrict__ X,
     const long base = off[b];
     // element i of this system
     auto xi = [&](int i) { return X[(base + i) * M + m]; };
-    auto yi = [&](int i) { return Cf[((base + i) * M * NQ + s) * 4]; };
-    auto at = [&](int i) { return ((base + i) * M * NQ + s) * 4; };
+    const int q = s - m * NQ;
+    auto at = [&](int i) { return ((base + i) * M + m) * RS + RQ + q * 4; };
+    auto yi = [&](int i) { return Cf[at(i)]; };
     double* dp = Cf + 1;  // d' then the slopes (= c1)
     double* C2 = Cf + 2;
     double* cp = Cf + 3;  // c' then c3
@@ -283,6 +299,10 @@ __global__ __launch_bounds__(64) void k_dir_spline(const double* __restrict__ X,
     dp[at(n - 1)] = 0.0;
     C2[at(n - 1)] = 0.0;
     cp[at(n - 1)] = 0.0;
+    if (q == 0) {
+        Cf[((base + n - 1) * M + m) * RS] = xn;
+        Cf[((base + n - 1) * M + m) * RS + 1] = xn;
+    }
     for (int i = n - 2; i >= 0; i--) {
         const double si = dp[at(i)] - cp[at(i)] * snext;
         const double x_i = xi(i), y_i = yi(i);
@@ -292,6 +312,10 @@ __global__ __launch_bounds__(64) void k_dir_spline(const double* __restrict__ X,
         dp[at(i)] = si;
         C2[at(i)] = (slope - si) / dx - t;
         cp[at(i)] = t / dx;
+        if (q == 0) {
+            Cf[((base + i) * M + m) * RS] = x_i;
+            Cf[((base + i) * M + m) * RS + 1] = xn;
+        }
         snext = si;
         xn = x_i;
         yn = y_i;
@@ -367,11 +391,73 @@ __device__ inline int owner(const int* __restrict__ band, int B, int it)
     return lo;
 }
 
-// Persistent dense-band kernel: partial[it] = (Re, Im of sum conj(d~) h~, Re, Im of sum |h~|^2)
+// h[ch] += conj(T_ch A e^{i phase}) of one mode at x = f - x_i into its segment, if
+// t_start <= tf <= t_end (t_end <= 0: no upper limit) (WaveformBuild.cu:48-60, :80-143)
+template <class Q>
+__device__ inline void add_mode(cd h[3], const Q& q, double x, double ts, double te)
+{
+    const double x2 = x * x, x3 = x * x2;
+    auto ev = [&](int k) {
+        const Cubic c = cubic_of(q, k);
+        return c.y + c.c1 * x + c.c2 * x2 + c.c3 * x3;
+    };
+    const double tf = ev(2);
+    if ((tf >= ts) && ((tf <= te) || (te <= 0.0))) {
+        const cd ap = ev(0) * cexpi_red(ev(1));
+        h[0] += conj(mk(ev(3), ev(4)) * ap);
+        h[1] += conj(mk(ev(5), ev(6)) * ap);
+        h[2] += conj(mk(ev(7), ev(8)) * ap);
+    }
+}
+
+// Segment of one lane's dense frequency f in the grid of one mode, searched by
+// that lane, and its contribution (lanes of a wavefront in many different
+// segments). The record at the FP32 log2 estimate is read with its bounds and
+// all nine cubics in one round of loads; the index then moves until
+// x_lo <= f < x_hi or leaves [0, L - 2] (no contribution, WaveformBuild.cu:92-93).
+__device__ inline void mode_per_lane(cd h[3], const SegRec* __restrict__ rm, int M, int L, double f, int i, double ts,
+                                     double te)
+{
+    const SegRec* r = rm + (long)i * M;
+    double2 bnd = *reinterpret_cast<const double2*>(r);
+    Cubic cq[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) cq[q] = r->q[q];
+    // keeps the compiler from sinking these loads below the bounds check
+    // (which would serialise two rounds of memory latency)
+#pragma unroll
+    for (int q = 0; q < NQ; q++) asm volatile("" ::"v"(cq[q].y), "v"(cq[q].c1), "v"(cq[q].c2), "v"(cq[q].c3));
+    if (!(bnd.x <= f && f < bnd.y)) {
+        // the estimate missed (rounding near a knot): walk to the segment holding f
+        while (f < bnd.x && i > 0) bnd = *reinterpret_cast<const double2*>(rm + (long)--i * M);
+        while (f >= bnd.y && i < L - 2) bnd = *reinterpret_cast<const double2*>(rm + (long)++i * M);
+        if (!(bnd.x <= f && f < bnd.y)) return;
+        r = rm + (long)i * M;
+#pragma unroll
+        for (int q = 0; q < NQ; q++) cq[q] = r->q[q];
+    }
+    add_mode(h, cq, f - bnd.x, ts, te);
+}
+
+constexpr int RB = 2;  // iterations of 64 dense bins per block of a wavefront
+
+// Persistent dense-band kernel: partial[it] = (Re, Im of sum conj(d~) h~, sum |h~|^2, 0)
 // over the dense bins of work item it, where h~ = w h (Likelihood.cu:357-375).
+//
+// Each wavefront takes a contiguous share of the work item's bins, in blocks of
+// RB x 64 consecutive bins (lane l: bins kb + 64 r + l), and sums the modes of a
+// block mode by mode. The segment of bin f in a mode's grid is
+// searchsorted(x, f, 'right') - 1 (waveformbuild.py:139-156). Dense bins are
+// much finer than the sparse grid over most of a band, so the lanes of an
+// iteration usually share one or two segments: a wavefront-uniform walker then
+// holds the current segment, reads its record once with scalar loads (bounds and
+// the nine cubics) and evaluates every lane whose f lies in [x_lo, x_hi), moving
+// to the next segment until all lanes are placed. The comparisons are the exact
+// FP64 ones against the stored knots, so the segment of every bin is the one
+// searchsorted gives. Where an iteration spans more than three segments (low
+// frequencies, short segments) each lane searches its own (mode_per_lane).
 __global__ __launch_bounds__(256) void k_dir_like(double* __restrict__ partial, const int* __restrict__ band,
-                                                  const int* __restrict__ total, const double* __restrict__ X,
-                                                  const Cubic* __restrict__ Cf,
+                                                  const int* __restrict__ total, const SegRec* __restrict__ R,
                                                   const double* __restrict__ grid_lo,
                                                   const double* __restrict__ grid_istep,
                                                   const long* __restrict__ off, const int* __restrict__ len,
@@ -380,9 +466,9 @@ __global__ __launch_bounds__(256) void k_dir_like(double* __restrict__ partial, 
                                                   const cd* __restrict__ dch, const double* __restrict__ wts, int N,
                                                   int B, int M, int chunk)
 {
-    __shared__ double red[4][256 / 64];
+    __shared__ double red[3][256 / 64];
     const int nitems = total[0];
-    const int tid = threadIdx.x, nth = blockDim.x;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nw = blockDim.x >> 6;
     for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
         const int b = owner(band, B, it);
         const int c = it - band[3 * b + 2];
@@ -391,54 +477,84 @@ __global__ __launch_bounds__(256) void k_dir_like(double* __restrict__ partial, 
         const long base = off[b];
         const int L = len[b];
         const double ts = t_start[b], te = t_end[b];
-        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-        for (int k = k0 + tid; k < k1; k += nth) {
-            const double f = freqs[k];
-            const double lf = log10(f);
-            cd h[3] = {mk(0.0, 0.0), mk(0.0, 0.0), mk(0.0, 0.0)};
-            for (int m = 0; m < M; m++) {
-                const double* xm = X + base * M + m;
-                // segment: searchsorted(x, f, 'right') - 1, first estimated from the
-                // log-spaced grid, then made exact against the stored knots
-                const double gis = grid_istep[b * M + m];
-                const double u = gis > 0.0 ? (lf - grid_lo[b * M + m]) * gis : -2.0;
-                int i = !(u >= -1.0) ? -1 : (u >= (double)(L - 1) ? L - 1 : (int)floor(u));
-                while (i + 1 < L && xm[(long)(i + 1) * M] <= f) i++;
-                while (i >= 0 && xm[(long)i * M] > f) i--;
-                if (i <= -1 || i >= L - 1) continue;  // WaveformBuild.cu:92-93
-                const long p = (base + i) * M + m;
-                const double x = f - X[p];
-                const double x2 = x * x, x3 = x * x2;
-                double v[NQ];
+        // this wavefront's share: a multiple of 64 bins
+        const int per = (((k1 - k0 + nw - 1) / nw) + 63) & ~63;
+        const int kw0 = k0 + wv * per, kw1 = min(k1, kw0 + per);
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+        for (int kb = kw0; kb < kw1; kb += RB * 64) {
+            double f[RB], lf[RB];
+            cd h[RB][3];
 #pragma unroll
-                for (int q = 0; q < NQ; q++) {
-                    const Cubic c = Cf[p * NQ + q];
-                    v[q] = c.y + c.c1 * x + c.c2 * x2 + c.c3 * x3;
-                }
-                const double tf = v[2];
-                if ((tf >= ts) && ((tf <= te) || (te <= 0.0))) {
-                    const cd ap = v[0] * cexpi_red(v[1]);
-                    h[0] += conj(mk(v[3], v[4]) * ap);
-                    h[1] += conj(mk(v[5], v[6]) * ap);
-                    h[2] += conj(mk(v[7], v[8]) * ap);
+            for (int r = 0; r < RB; r++) {
+                const int k = kb + 64 * r + lane;
+                f[r] = k < kw1 ? freqs[k] : 0.0;
+                lf[r] = (double)__log2f((float)f[r]);
+#pragma unroll
+                for (int ch = 0; ch < 3; ch++) h[r][ch] = mk(0.0, 0.0);
+            }
+            for (int m = 0; m < M; m++) {
+                const double gis = grid_istep[b * M + m];
+                if (!(gis > 0.0)) continue;  // degenerate grid: no segment holds any f
+                const double glo = grid_lo[b * M + m];
+                const SegRec* rm = R + base * M + m;
+                int cur = -1;  // the walker's segment (wavefront-uniform); -1: not placed
+#pragma unroll
+                for (int r = 0; r < RB; r++) {
+                    bool pending = kb + 64 * r + lane < kw1;
+                    const unsigned long long pm = __ballot(pending);
+                    if (pm == 0) break;  // later iterations are past the share too
+                    const int est = (int)fmin(fmax((lf[r] - glo) * gis, 0.0), (double)(L - 2));
+                    const int e_first = __builtin_amdgcn_readlane(est, __builtin_ctzll(pm));
+                    const int e_last = __builtin_amdgcn_readlane(est, 63 - __builtin_clzll(pm));
+                    if (e_last - e_first > 2) {
+                        if (pending) mode_per_lane(h[r], rm, M, L, f[r], est, ts, te);
+                        cur = -1;
+                        continue;
+                    }
+                    if (cur < 0) cur = e_first;
+                    // the walker moves down only while a pending lane lies below the current
+                    // segment, up only past lanes it has placed: in a sorted grid every step
+                    // makes progress; the bound only guards against an unsorted one
+                    for (int guard = 2 * L + 8; guard > 0; guard--) {
+                        if (__ballot(pending) == 0) break;
+                        const SegRec* rc = rm + (long)cur * M;
+                        const double xlo = rc->x_lo, xhi = rc->x_hi;
+                        const bool below = pending && f[r] < xlo;
+                        if (__ballot(below) != 0) {
+                            if (cur == 0) pending = pending && !below;  // left of the grid
+                            else cur--;
+                            continue;
+                        }
+                        const bool in = pending && f[r] < xhi;
+                        if (in) add_mode(h[r], *rc, f[r] - xlo, ts, te);
+                        pending = pending && !in;
+                        if (__ballot(pending) != 0) {
+                            if (cur == L - 2) pending = false;  // right of the grid
+                            else cur++;
+                        }
+                    }
                 }
             }
-            for (int ch = 0; ch < 3; ch++) {
-                const double w = wts[(long)ch * N + k];
-                const cd ht = h[ch] * w;
-                const cd d = dch[(long)ch * N + k];
-                const cd dh = conj(d) * ht;
-                a0 += dh.re;
-                a1 += dh.im;
-                a2 += ht.re * ht.re + ht.im * ht.im;
+#pragma unroll
+            for (int r = 0; r < RB; r++) {
+                const int k = kb + 64 * r + lane;
+                if (k < kw1) {
+#pragma unroll
+                    for (int ch = 0; ch < 3; ch++) {
+                        const cd ht = h[r][ch] * wts[(long)ch * N + k];
+                        const cd dh = conj(dch[(long)ch * N + k]) * ht;
+                        a0 += dh.re;
+                        a1 += dh.im;
+                        a2 += ht.re * ht.re + ht.im * ht.im;
+                    }
+                }
             }
         }
         // block reduction in a fixed order
         a0 = wsum(a0);
         a1 = wsum(a1);
         a2 = wsum(a2);
-        const int wv = tid >> 6;
-        if ((tid & 63) == 0) {
+        if (lane == 0) {
             red[0][wv] = a0;
             red[1][wv] = a1;
             red[2][wv] = a2;
@@ -446,7 +562,7 @@ __global__ __launch_bounds__(256) void k_dir_like(double* __restrict__ partial, 
         __syncthreads();
         if (tid == 0) {
             double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-            for (int w = 0; w < (nth >> 6); w++) {
+            for (int w = 0; w < nw; w++) {
                 s0 += red[0][w];
                 s1 += red[1][w];
                 s2 += red[2][w];
@@ -536,7 +652,7 @@ extern "C" int bbhx_hip_direct_plan_create(bbhx_direct_plan** out, int numBinAll
     p->len = dalloc<int>(numBinAll, ok);
     p->recs = dalloc<FusedRec>((size_t)numBinAll * numModes, ok);
     p->X = dalloc<double>(SM, ok);
-    p->Cf = dalloc<double>(SM * NQ * 4, ok);
+    p->Cf = dalloc<double>(SM * RS, ok);
     p->grid_lo = dalloc<double>((size_t)numBinAll * numModes, ok);
     p->grid_istep = dalloc<double>((size_t)numBinAll * numModes, ok);
     p->band = dalloc<int>((size_t)numBinAll * 3, ok);
@@ -570,7 +686,21 @@ extern "C" long bbhx_hip_direct_plan_bytes(const bbhx_direct_plan* p)
 {
     if (!p) return -1;
     const long SM = p->S * p->M;
-    return SM * 8 * (1 + 4 * NQ) + (long)p->B * p->M * (sizeof(FusedRec) + 16) + p->max_items * 32 + p->B * 24;
+    return SM * 8 * (1 + RS) + (long)p->B * p->M * (sizeof(FusedRec) + 16) + p->max_items * 32 + p->B * 24;
+}
+
+extern "C" int bbhx_hip_direct_plan_band(const bbhx_direct_plan* p, int* band_host)
+{
+    if (!p || !band_host) return ERR_ARGS;
+    std::vector<int> b3((size_t)p->B * 3);
+    if (hipDeviceSynchronize() != hipSuccess ||
+        hipMemcpy(b3.data(), p->band, sizeof(int) * b3.size(), hipMemcpyDeviceToHost) != hipSuccess)
+        return ERR_LAUNCH;
+    for (int b = 0; b < p->B; b++) {
+        band_host[2 * b] = b3[3 * b];
+        band_host[2 * b + 1] = b3[3 * b + 1];
+    }
+    return OK;
 }
 
 extern "C" int bbhx_hip_direct_fused(bbhx_direct_plan* p, double* d_h, double* h_h, int* flags, const double* params,
@@ -608,8 +738,9 @@ extern "C" int bbhx_hip_direct_fused(bbhx_direct_plan* p, double* d_h, double* h
     hipLaunchKernelGGL(k_dir_band, dim3(1), dim3(1024), 0, st, p->band, p->total, p->X, p->off, p->len, dataFreqs,
                        p->N, B, M, chunk);
     if (timed) (void)hipEventRecord(ev[1], st);
-    hipLaunchKernelGGL(k_dir_like, dim3(grid), dim3(block), 0, st, p->partial, p->band, p->total, p->X,
-                       reinterpret_cast<const Cubic*>(p->Cf), p->grid_lo, p->grid_istep, p->off, p->len, t_start, t_end, dataFreqs,
+    hipLaunchKernelGGL(k_dir_like, dim3(grid), dim3(block), 0, st, p->partial, p->band, p->total,
+                       reinterpret_cast<const SegRec*>(p->Cf), p->grid_lo, p->grid_istep, p->off, p->len, t_start,
+                       t_end, dataFreqs,
                        reinterpret_cast<const cd*>(dataChannels), noiseFactors, p->N, B, M, chunk);
     hipLaunchKernelGGL(k_dir_reduce, dim3(ceil_div(B, 256)), dim3(256), 0, st, d_h, h_h, p->partial, p->band,
                        p->total, B);

@@ -78,6 +78,7 @@ def load_library():
         "bbhx_hip_direct_plan_create": [C.POINTER(_vp), C.c_int, _ip, C.c_int, C.c_int, C.c_int],
         "bbhx_hip_direct_plan_destroy": [_vp],
         "bbhx_hip_direct_plan_bytes": [_vp],
+        "bbhx_hip_direct_plan_band": [_vp, _ip],
         "bbhx_hip_direct_fused": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _ip, _ip, C.c_int, _vp, _vp, _vp, _vp, C.c_int,
                                   C.c_double, C.c_double, C.c_double, C.c_double, _vp, C.c_int, C.c_int, C.c_int,
                                   C.c_int, C.c_int, C.c_int, _vp],
@@ -102,7 +103,7 @@ def exported_symbols():
         "bbhx_hip_response_add_orbits", "bbhx_hip_response_compute", "bbhx_hip_interpolate", "bbhx_hip_InterpTDI",
         "bbhx_hip_direct_sum", "bbhx_hip_hdyn", "bbhx_hip_direct_like", "bbhx_hip_hdyn_fused",
         "bbhx_hip_hdyn_fused_grad", "bbhx_hip_direct_plan_create", "bbhx_hip_direct_plan_destroy",
-        "bbhx_hip_direct_plan_bytes", "bbhx_hip_direct_fused",
+        "bbhx_hip_direct_plan_bytes", "bbhx_hip_direct_plan_band", "bbhx_hip_direct_fused",
         "bbhx_hip_release_workspace", "bbhx_hip_version", "bbhx_hip_timing_enable", "bbhx_hip_timing_read",
     ]
 
@@ -379,6 +380,12 @@ class DirectPlan:
 
     def nbytes(self):
         return int(load_library().bbhx_hip_direct_plan_bytes(self._h))
+
+    def band(self):
+        """(B, 2) int32 dense band [st, et) of each binary from the last call."""
+        out = np.zeros((self.num_bin_all, 2), dtype=np.int32)
+        _check(load_library().bbhx_hip_direct_plan_band(self._h, out.ctypes.data_as(_ip)), "direct_plan_band")
+        return out
 
     def __del__(self):
         h = getattr(self, "_h", None)

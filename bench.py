@@ -145,7 +145,8 @@ def bench_direct_ragged(args):
     data = inj + np.sqrt(psd / (4 * df)) * (rng.standard_normal(psd.shape) + 1j * rng.standard_normal(psd.shape))
     like = Likelihood(wave, f, data, psd)
     log("autotuning the launch shape")
-    tuned = tune_direct(like, p, lengths, MODES, reps=2)
+    tuned = tune_direct(like, p, lengths, MODES, reps=2,
+                        progress=lambda c, b, g, ms: log(f"  chunk {c} block {b} grid {g}: {ms:.2f} ms"))
     log(f"autotuner: chunk {tuned['chunk']}, block {tuned['block']}, grid {tuned['grid']} ({tuned['ms']:.2f} ms)")
     plan = like.direct_plan(lengths, MODES, chunk=tuned["chunk"], block=tuned["block"], grid=tuned["grid"])
     rows, ts, te = fused_param_rows(like.waveform_gen, p, 0.0, 1.0, like.device)
@@ -167,17 +168,28 @@ def bench_direct_ragged(args):
     g.replay()
     torch.cuda.synchronize()
     diff = float(np.max(np.abs(out.cpu().numpy() - eager)))
-    timing_enable(True)
+    band = plan.band()
+    dense = int(np.sum(np.maximum(band[:, 1] - band[:, 0], 0)))
     t0 = time.perf_counter()
     for _ in range(args.steps):
         g.replay()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    timing_enable(False)
     ms = (t1 - t0) * 1e3 / args.steps
+    # set-up / dense-band split from eager calls (the captured graph carries no timing events)
+    timing_enable(True)
+    for _ in range(2):
+        flags.zero_()
+        like.get_ll_fused_device(plan, rows, ts, te, flags, MODES)
+    ms_setup, ms_eval, ncalls = timing_read()
+    timing_enable(False)
     B = rows.shape[1]
+    split = {}
+    if ncalls:
+        split = {"setup_ms": ms_setup / ncalls, "dense_ms": ms_eval / ncalls}
     log(f"graph replay: {ms:.3f} ms/step, {B / (ms * 1e-3):.4g} evals/s; |graph - eager| = {diff:.2e}; "
-        f"plan {plan.nbytes() / 2**30:.2f} GiB")
+        f"plan {plan.nbytes() / 2**30:.2f} GiB; {dense} dense (bin, template) pairs "
+        f"({dense / (ms * 1e-3):.4g}/s); split {split}")
     out = {
         "metric": "direct logL evals/sec, 1 GPU",
         "value": B / (ms * 1e-3),
@@ -196,7 +208,8 @@ def bench_direct_ragged(args):
                    "cadence_s": args.direct_cadence,
                    "lengths_min": int(lengths.min()), "lengths_max": int(lengths.max()),
                    "lengths_sum": int(lengths.sum()), "dense_bins": int(f.size), "modes": 6, "tdi": "AET tdi2",
-                   "launch": {k: tuned[k] for k in ("chunk", "block", "grid")}, "graph_vs_eager": diff},
+                   "launch": {k: tuned[k] for k in ("chunk", "block", "grid")}, "graph_vs_eager": diff,
+                   "dense_bin_templates": dense, **split},
     }
     print(json.dumps(out), flush=True)
 
@@ -212,6 +225,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=4096)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-grad-1024", action="store_true",
+                    help="skip the gradient workload of 1024 templates at K = 256")
     ap.add_argument("--workload", default="heterodyned", choices=("heterodyned", "direct-ragged"))
     ap.add_argument("--direct-bins", type=int, default=0, help="truncate the dense data grid (testing)")
     ap.add_argument("--direct-years", type=float, default=1.0, help="direct workloads: data span [yr]")
@@ -293,6 +308,33 @@ def main():
         f"({value / grad_rate:.2f}x the forward time); setup {gms_setup / max(gn, 1):.3f} ms, "
         f"eval {gms_eval / max(gn, 1):.3f} ms; max |logL(grad kernel) - logL(forward)| = {grad_ll_diff:.2e}")
 
+    # gradient workload of 1024 templates at K = 256 sparse nodes (same reference and data model)
+    grad_1024 = None
+    if not args.no_grad_1024:
+        log("gradient workload: 1024 templates, K = 256")
+        prob_g = build_problem(1024, 256, args.dense_bits, seed=1003)
+        het_g = prob_g["het"]
+        rg, tsg, teg = het_g.fused_inputs(prob_g["templates"], {})
+        for _ in range(max(args.warmup, 1)):
+            het_g.get_ll_device(rg, tsg, teg)
+            het_g.get_ll_grad_device(rg, tsg, teg)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            het_g.get_ll_device(rg, tsg, teg)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            het_g.get_ll_grad_device(rg, tsg, teg)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        fwd_ms, grd_ms = (t1 - t0) * 1e3 / args.steps, (t2 - t1) * 1e3 / args.steps
+        grad_1024 = {"templates": 1024, "sparse_nodes": 256, "forward_ms": fwd_ms, "grad_ms": grd_ms,
+                     "grad_evals_per_s": 1024 / (grd_ms * 1e-3), "grad_cost_vs_forward": grd_ms / fwd_ms}
+        log(f"gradient (1024 x K=256): {grd_ms:.3f} ms/step, {grad_1024['grad_evals_per_s']:.4g} grad evals/s, "
+            f"{grd_ms / fwd_ms:.2f}x the forward ({fwd_ms:.3f} ms)")
+        del het_g, prob_g
+
     # the kernel-by-kernel FP64 path (the reference's structure) on the same templates
     t0 = time.perf_counter()
     n_plain = 3
@@ -347,7 +389,8 @@ def main():
                    "dense_bins": 2**args.dense_bits, "modes": 6, "tdi": "AET tdi2",
                    "plain_fp64_path_evals_per_s": plain_rate, "max_abs_dlogL_vs_cpu_oracle": dll,
                    "grad_evals_per_s": grad_rate, "grad_cost_vs_forward": value / grad_rate,
-                   "grad_params": "lnM, q, chi1, chi2, dist, phi_ref, cos inc, lam, sin beta, psi, t_ref"},
+                   "grad_params": "lnM, q, chi1, chi2, dist, phi_ref, cos inc, lam, sin beta, psi, t_ref",
+                   "grad_1024x256": grad_1024},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -187,7 +187,7 @@ int bbhx_hip_hdyn_fused_grad(double* d_h, double* h_h, double* grad, const doubl
  * lengths L_b, in one sequence of six launches with no host synchronisation
  * and no allocation, so that the call can be captured in a hipGraph. A plan
  * holds the sizes and the device buffers (sparse grids, the nine spline
- * quantities and their coefficients: 296 B per (knot, mode), plus per-binary
+ * quantities and their coefficients: 328 B per (knot, mode), plus per-binary
  * tables); plan_bytes returns their total. */
 typedef struct bbhx_direct_plan bbhx_direct_plan;
 
@@ -199,6 +199,10 @@ int bbhx_hip_direct_plan_create(bbhx_direct_plan** out, int numBinAll, const int
                                 int data_length, int min_chunk);
 int bbhx_hip_direct_plan_destroy(bbhx_direct_plan* p);
 long bbhx_hip_direct_plan_bytes(const bbhx_direct_plan* p);
+/* Dense band of each binary from the last bbhx_hip_direct_fused call on this
+ * plan: band_host (B, 2) receives [st, et), the dense bins the binary's
+ * template covers (waveformbuild.py:139-156). Waits for the device. */
+int bbhx_hip_direct_plan_band(const bbhx_direct_plan* p, int* band_host);
 
 /* params (11, B) and t_start, t_end (B,) as for bbhx_hip_hdyn_fused; dataFreqs
  * (N,) sorted; dataChannels (3, N) complex = noise weight x data, as

@@ -216,6 +216,33 @@ class OracleLikelihood:
         return -1 / 2 * (self.d_d + h_h - 2 * d_h).real, d_h, h_h
 
 
+def hdyn_constants(f_dense, d, h0, S_n, freqs):
+    """Heterodyne constants of a reference template (likelihood.py:499-527).
+
+    f_dense (N,) dense frequencies already cut to [freqs[0], freqs[-1]]; d, h0
+    (3, N) complex data and reference template there; S_n (3, N) PSD; freqs (K,)
+    sparse nodes. Returns the flat (4, 3, K) complex constants A0, A1, B0, B1
+    (segment j = [freqs[j], freqs[j+1]) summed into slot j + 1, the last dense
+    point left out, as the reference's loop over unique(bins[:-1]) does) and the
+    totals (d_d, h_h, Re d_h) over the whole band (likelihood.py:529-533)."""
+    K = freqs.size
+    bins = np.searchsorted(freqs, f_dense, "right") - 1
+    f_m = (freqs[1:] + freqs[:-1]) / 2
+    df = f_dense[1] - f_dense[0]
+    A0 = 4 * (h0.conj() * d) / S_n * df
+    A1 = A0 * (f_dense - f_m[bins])
+    B0 = 4 * (h0.conj() * h0) / S_n * df
+    B1 = B0 * (f_dense - f_m[bins])
+    C = [np.zeros((3, K), dtype=np.complex128) for _ in range(4)]
+    for ind in np.unique(bins[:-1]):
+        keep = bins == ind
+        keep[-1] = False
+        for c, flat in zip(C, (A0, A1, B0, B1)):
+            c[:, ind + 1] = np.sum(flat[:, keep], axis=1)
+    totals = (np.sum(4 * (d.conj() * d) / S_n * df).real, np.sum(B0).real, np.sum(A0).real)
+    return np.concatenate([c.flatten() for c in C]), totals
+
+
 class OracleHeterodyne:
     """likelihood.py:380-648, with the constants built as the reference's Python loop does."""
 
@@ -238,24 +265,9 @@ class OracleHeterodyne:
         self.h0_sparse = wave(*ref_params, freqs=freqs, direct=True, compress=True, squeeze=True, **wkw)[None, :, :]
         inds = (f_dense >= freqs[0]) & (f_dense <= freqs[-1])
         f_dense, d, h0 = f_dense[inds], d[:, inds], h0[:, inds]
-        bins = np.searchsorted(freqs, f_dense, "right") - 1
-        f_m = (freqs[1:] + freqs[:-1]) / 2
-        df = f_dense[1] - f_dense[0]
         S_n = np.asarray(psd_fn(f_dense))
-        A0 = 4 * (h0.conj() * d) / S_n * df
-        A1 = A0 * (f_dense - f_m[bins])
-        B0 = 4 * (h0.conj() * h0) / S_n * df
-        B1 = B0 * (f_dense - f_m[bins])
-        C = [np.zeros((3, K), dtype=np.complex128) for _ in range(4)]
-        for ind in np.unique(bins[:-1]):
-            keep = bins == ind
-            keep[-1] = False
-            for c, flat in zip(C, (A0, A1, B0, B1)):
-                c[:, ind + 1] = np.sum(flat[:, keep], axis=1)
-        self.data_constants = np.concatenate([c.flatten() for c in C])
-        self.reference_d_d = np.sum(4 * (d.conj() * d) / S_n * df).real
-        self.reference_h_h = np.sum(B0).real
-        self.reference_d_h = np.sum(A0).real
+        self.data_constants, (self.reference_d_d, self.reference_h_h, self.reference_d_h) = hdyn_constants(
+            f_dense, d, h0, S_n, freqs)
         self.reference_ll = -0.5 * (self.reference_d_d + self.reference_h_h - 2 * self.reference_d_h)
         self.freqs = freqs
         self.wkw = wkw

@@ -25,7 +25,7 @@ def test_header_declares_the_backend_methods():
               "bbhx_hip_response_add_orbits", "bbhx_hip_response_compute", "bbhx_hip_response_destroy",
               "bbhx_hip_interpolate", "bbhx_hip_InterpTDI", "bbhx_hip_direct_sum", "bbhx_hip_hdyn",
               "bbhx_hip_direct_like", "bbhx_hip_hdyn_fused", "bbhx_hip_hdyn_fused_grad", "bbhx_hip_direct_plan_create",
-              "bbhx_hip_direct_plan_destroy", "bbhx_hip_direct_fused"):
+              "bbhx_hip_direct_plan_destroy", "bbhx_hip_direct_plan_band", "bbhx_hip_direct_fused"):
         assert n in names, n
 
 
@@ -70,6 +70,7 @@ def test_version_and_argument_errors_need_no_gpu():
     assert lib.bbhx_hip_direct_fused(None, *([None] * 8), 6, *([None] * 4), 2, 2e-3, 0.0, 1e-4, 0.6, None, 2, 0, 1,
                                      16384, 256, 64, None) == 1
     assert lib.bbhx_hip_direct_plan_bytes(None) == -1
+    assert lib.bbhx_hip_direct_plan_band(None, None) == 1
 
 
 def test_registry_without_gpu_fails_loudly(monkeypatch):

@@ -403,3 +403,68 @@ def test_direct_like_brute_force_and_invariants():
     dh2, hh2 = oracle.direct_like(d_w.ravel(), w, [2 * data], np.array([0], np.int32), np.array([N], np.int32), N)
     np.testing.assert_allclose(dh2, 2 * dh1, rtol=1e-12)
     np.testing.assert_allclose(hh2, 4 * hh1, rtol=1e-12)
+
+
+def _hdyn_constants_problem(seed, N=700, K=23):
+    rng = np.random.default_rng(seed)
+    f = np.arange(1, N + 1) * 1e-5
+    freqs = np.logspace(np.log10(f[0]), np.log10(f[-1] * 1.0000001), K)
+    d = rng.standard_normal((3, N)) + 1j * rng.standard_normal((3, N))
+    h0 = rng.standard_normal((3, N)) + 1j * rng.standard_normal((3, N))
+    S = rng.uniform(0.5, 2.0, (3, N))
+    return f, freqs, d, h0, S
+
+
+def test_hdyn_constants_brute_force():
+    """Per-point loop with bisect (no searchsorted, no masks) on a small band."""
+    from bisect import bisect_right
+
+    from oracle.pipeline import hdyn_constants
+
+    f, freqs, d, h0, S = _hdyn_constants_problem(11)
+    N, K = f.size, freqs.size
+    dc, tot = hdyn_constants(f, d, h0, S, freqs)
+    want = np.zeros((4, 3, K), dtype=np.complex128)
+    df = f[1] - f[0]
+    for k in range(N - 1):
+        j = bisect_right(list(freqs), f[k]) - 1
+        fm = 0.5 * (freqs[j] + freqs[j + 1])
+        for c in range(3):
+            a0 = 4 * np.conj(h0[c, k]) * d[c, k] / S[c, k] * df
+            b0 = 4 * abs(h0[c, k]) ** 2 / S[c, k] * df
+            want[0, c, j + 1] += a0
+            want[1, c, j + 1] += a0 * (f[k] - fm)
+            want[2, c, j + 1] += b0
+            want[3, c, j + 1] += b0 * (f[k] - fm)
+    np.testing.assert_allclose(dc.reshape(4, 3, K), want, rtol=1e-12, atol=1e-13 * np.abs(want).max())
+    assert np.all(dc.reshape(4, 3, K)[:, :, 0] == 0)
+    np.testing.assert_allclose(tot[0], np.sum(4 * np.abs(d) ** 2 / S * df), rtol=1e-12)
+
+
+def test_hdyn_constants_exact_for_segmentwise_linear_ratio():
+    """h = h0 r with r linear in f on each segment: the heterodyned d_h built
+    from the constants equals the direct sum conj(h) d over the band (the
+    reference leaves out the last dense point); for r constant on each
+    segment, h_h too."""
+    from oracle.pipeline import hdyn_constants
+
+    f, freqs, d, h0, S = _hdyn_constants_problem(12)
+    N, K = f.size, freqs.size
+    dc, _ = hdyn_constants(f, d, h0, S, freqs)
+    A0, A1, B0, B1 = dc.reshape(4, 3, K)
+    rng = np.random.default_rng(13)
+    r0 = rng.standard_normal((3, K)) + 1j * rng.standard_normal((3, K))  # value at the segment midpoint
+    r1 = (rng.standard_normal((3, K)) + 1j * rng.standard_normal((3, K))) * 1e3
+    bins = np.searchsorted(freqs, f, "right") - 1
+    fm = 0.5 * (freqs[1:] + freqs[:-1])
+    df = f[1] - f[0]
+    r = r0[:, bins + 1] + r1[:, bins + 1] * (f - fm[bins])
+    h = h0 * r
+    direct_dh = np.sum((4 * np.conj(h) * d / S * df)[:, :-1])
+    het_dh = np.sum(A0 * np.conj(r0) + A1 * np.conj(r1))
+    np.testing.assert_allclose(het_dh, direct_dh, rtol=1e-12)
+    h = h0 * r0[:, bins + 1]
+    direct_hh = np.sum((4 * np.abs(h) ** 2 / S * df)[:, :-1])
+    het_hh = np.sum(B0 * np.abs(r0) ** 2)
+    np.testing.assert_allclose(het_hh, direct_hh, rtol=1e-12)
+    assert np.abs(np.sum(B1)) > 0  # B1 carries the first moment
