@@ -21,7 +21,8 @@ enum WsSlot : int {
     WS_FUSED = 4,
     WS_GRAD = 5,
     WS_GRAD_AUX = 6,
-    WS_SLOTS = 7,
+    WS_PREP = 7,
+    WS_SLOTS = 8,
 };
 
 // Returns a device pointer with at least 'bytes' bytes on the current device,

@@ -79,6 +79,7 @@ def load_library():
         "bbhx_hip_direct_plan_destroy": [_vp],
         "bbhx_hip_direct_plan_bytes": [_vp],
         "bbhx_hip_direct_plan_band": [_vp, _ip],
+        "bbhx_hip_hdyn_prep": [_vp, _vp, _vp, _vp, _vp, _vp, C.c_int, _vp, C.c_int, C.c_double, _vp],
         "bbhx_hip_direct_fused": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _ip, _ip, C.c_int, _vp, _vp, _vp, _vp, C.c_int,
                                   C.c_double, C.c_double, C.c_double, C.c_double, _vp, C.c_int, C.c_int, C.c_int,
                                   C.c_int, C.c_int, C.c_int, _vp],
@@ -103,7 +104,7 @@ def exported_symbols():
         "bbhx_hip_response_add_orbits", "bbhx_hip_response_compute", "bbhx_hip_interpolate", "bbhx_hip_InterpTDI",
         "bbhx_hip_direct_sum", "bbhx_hip_hdyn", "bbhx_hip_direct_like", "bbhx_hip_hdyn_fused",
         "bbhx_hip_hdyn_fused_grad", "bbhx_hip_direct_plan_create", "bbhx_hip_direct_plan_destroy",
-        "bbhx_hip_direct_plan_bytes", "bbhx_hip_direct_plan_band", "bbhx_hip_direct_fused",
+        "bbhx_hip_direct_plan_bytes", "bbhx_hip_direct_plan_band", "bbhx_hip_direct_fused", "bbhx_hip_hdyn_prep",
         "bbhx_hip_release_workspace", "bbhx_hip_version", "bbhx_hip_timing_enable", "bbhx_hip_timing_read",
     ]
 
@@ -416,6 +417,21 @@ def direct_fused(plan, d_h, h_h, flags, params, t_start, t_end, ells, mms, data_
     _check(st, "direct_fused")
 
 
+def hdyn_prep(data_constants, totals, data_freqs, data_channels, h0, psd, nodes, df):
+    """Heterodyne constants on the device (no reference counterpart in BBHxBackendMethods; see include/bbhx_hip.h)."""
+    torch = _torch()
+    f64, c128 = torch.float64, torch.complex128
+    N, K = int(data_freqs.numel()), int(nodes.numel())
+    if tuple(data_channels.shape) != (3, N) or tuple(h0.shape) != (3, N) or tuple(psd.shape) != (3, N):
+        raise ValueError("data_channels, h0 and psd must be (3, N)")
+    if data_constants.numel() != 12 * K or totals.numel() != 3:
+        raise ValueError("data_constants must hold 4 x 3 x K complex, totals 3 doubles")
+    st = load_library().bbhx_hip_hdyn_prep(_dev(data_constants, c128), _dev(totals, f64), _dev(data_freqs, f64),
+                                           _dev(data_channels, c128), _dev(h0, c128), _dev(psd, f64), N,
+                                           _dev(nodes, f64), K, float(df), _stream())
+    _check(st, "hdyn_prep")
+
+
 def timing_enable(on=True):
     """Start (or stop) event timing of the fused heterodyne launches."""
     _check(load_library().bbhx_hip_timing_enable(int(bool(on))), "timing_enable")
@@ -477,6 +493,7 @@ class BBHxHipBackend:
         self.hdyn_fused_grad = hdyn_fused_grad
         self.direct_fused = direct_fused
         self.DirectPlan = DirectPlan
+        self.hdyn_prep = hdyn_prep
         self.device = torch.device("cuda", torch.cuda.current_device())
 
     @property

@@ -227,34 +227,20 @@ class HeterodynedLikelihood(BBHxParallelModule):
         f_n_host = f_dense_h[inds]
 
         freqs_d = to_dev(freqs, self.device)
-        bins = torch.searchsorted(freqs_d, self.f_dense, right=True) - 1
         f_m = (freqs_d[1:] + freqs_d[:-1]) / 2
-        df = self.f_dense[1] - self.f_dense[0]
+        df = float(f_n_host[1] - f_n_host[0])
 
         self.sens_mat.update_frequency_arr(f_n_host)
         S_n = to_dev(np.asarray([self.sens_mat[0], self.sens_mat[1], self.sens_mat[2]]), self.device)
 
-        A0_flat = 4 * (h0.conj() * self.d) / S_n * df
-        A1_flat = 4 * (h0.conj() * self.d) / S_n * df * (self.f_dense - f_m[bins.clamp(max=f_m.numel() - 1)])
-        B0_flat = 4 * (h0.conj() * h0) / S_n * df
-        B1_flat = 4 * (h0.conj() * h0) / S_n * df * (self.f_dense - f_m[bins.clamp(max=f_m.numel() - 1)])
-
-        # Segmented sums into slot bin + 1. The reference loops over unique(bins[:-1]) and
-        # drops the last dense point of the band (likelihood.py:512-522): same as summing
-        # every point but the last.
+        # A0, A1, B0, B1 per segment and channel, and the band totals: one
+        # deterministic segmented reduction on the device (likelihood.py:499-533)
         K = self.length_f_het
-        keep_n = self.f_dense.numel() - 1
-        slot = (bins[:keep_n] + 1).to(torch.int64)
-        consts = []
-        for flat in (A0_flat, A1_flat, B0_flat, B1_flat):
-            acc = torch.zeros((3, K + 1), dtype=torch.complex128, device=self.device)
-            acc.index_add_(1, slot, flat[:, :keep_n])
-            consts.append(acc[:, :K])
-        self.data_constants = torch.cat([c.flatten() for c in consts]).contiguous()
-
-        self.reference_d_d = float(torch.sum(4 * (self.d.conj() * self.d) / S_n * df).real)
-        self.reference_h_h = float(torch.sum(B0_flat).real)
-        self.reference_d_h = float(torch.sum(A0_flat).real)
+        self.data_constants = torch.zeros(12 * K, dtype=torch.complex128, device=self.device)
+        totals = torch.zeros(3, dtype=torch.float64, device=self.device)
+        self.backend.hdyn_prep(self.data_constants, totals, self.f_dense, self.d, h0.contiguous(),
+                               S_n.contiguous(), freqs_d, df)
+        self.reference_d_d, self.reference_h_h, self.reference_d_h = (float(v) for v in totals.cpu().numpy())
         self.reference_ll = -1 / 2 * (self.reference_d_d + self.reference_h_h - 2 * self.reference_d_h)
 
         self.freqs = freqs_d

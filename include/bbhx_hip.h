@@ -223,6 +223,20 @@ int bbhx_hip_direct_fused(bbhx_direct_plan* p, double* d_h, double* h_h, int* fl
                           double initial_t_val, double mf_min, double mf_max, const bbhx_response* orbits,
                           int TDItag, int rescaled, int tdi2, int chunk, int block, int grid, void* stream);
 
+/* Heterodyne constants of a reference template (HeterodynedLikelihood.
+ * init_heterodyne_info, likelihood.py:499-533; the reference's unused kernel
+ * for it is prep_hdyn, Likelihood.cu:238-352). dataFreqs (N,) sorted, already
+ * cut to [nodes[0], nodes[K-1]]; dataChannels, h0 (3, N) complex data and
+ * reference template there; psd (3, N); nodes (K,) sparse frequencies; df the
+ * dense spacing. Writes dataConstants (4, 3, K) complex in the layout hdyn
+ * reads (A0, A1, B0, B1; slot j + 1 = segment [nodes[j], nodes[j+1]), the last
+ * dense point left out as the reference's loop does, slot 0 zero) and
+ * totals (3,) = d_d, h_h, Re d_h over all N points. Deterministic (fixed
+ * summation order). */
+int bbhx_hip_hdyn_prep(double* dataConstants, double* totals, const double* dataFreqs, const double* dataChannels,
+                       const double* h0, const double* psd, int data_length, const double* nodes, int K, double df,
+                       void* stream);
+
 /* Event timing of the fused entry points on the caller's stream, split in two:
  * set-up (records; for the direct form also grids, splines and bands) and
  * evaluation. bbhx_hip_hdyn_fused, bbhx_hip_hdyn_fused_grad and
