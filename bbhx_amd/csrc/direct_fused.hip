@@ -19,9 +19,12 @@
 //   k_dir_spline  one wavefront per binary, one lane per (mode, quantity):
 //                 not-a-knot spline coefficients (Interpolate.cu:15-359)
 //   k_dir_band    dense band [st, et) of each binary (waveformbuild.py:139-156)
-//                 and the exclusive prefix of its chunks of dense bins
+//                 and the exclusive prefix of its chunks of dense bins (chunks
+//                 aligned to absolute multiples of the chunk size)
+//   k_dir_cnt_zero, k_dir_cnt, k_dir_cnt_scan, k_dir_perm
+//                 chunk-major execution order of the work items (counting sort)
 //   k_dir_like    persistent: workgroups loop over the (binary, chunk) work
-//                 items; each thread evaluates dense bins (spline segment per
+//                 items in that order; each thread evaluates dense bins (spline segment per
 //                 mode, WaveformBuild.cu:80-143, window and mode sum :48-60)
 //                 and accumulates the noise-weighted inner products
 //                 (Likelihood.cu:357-471); one partial sum per work item
@@ -57,6 +60,10 @@ struct bbhx_direct_plan {
     int* band = nullptr;          // (B, 3): st, et, chunk_begin
     int* total = nullptr;         // (1,) number of work items
     double* partial = nullptr;    // (max_items, 4)
+    int* perm = nullptr;          // (max_items,) execution order of the work items
+    int* cnt = nullptr;           // (max_chunks + 1,) items per absolute chunk, then offsets
+    int* next = nullptr;          // (1,) next entry of perm for the dense-band kernel
+    int max_chunks = 0;           // absolute chunks of min_chunk bins
 };
 
 namespace {
@@ -335,7 +342,9 @@ __device__ inline int search_right(const double* __restrict__ a, int n, double v
 }
 
 // Dense band of each binary and the exclusive prefix of its chunks; one
-// workgroup of 1024 threads, each owning a contiguous run of binaries.
+// workgroup of 1024 threads, each owning a contiguous run of binaries. Chunks
+// are aligned to absolute multiples of 'chunk' dense bins, so that the work
+// items of different binaries over the same dense bins read the same data.
 __global__ __launch_bounds__(1024) void k_dir_band(int* __restrict__ band, int* __restrict__ total,
                                                    const double* __restrict__ X, const long* __restrict__ off,
                                                    const int* __restrict__ len, const double* __restrict__ freqs,
@@ -355,7 +364,7 @@ __global__ __launch_bounds__(1024) void k_dir_band(int* __restrict__ band, int* 
             hi = fmax(hi, X[(base + L - 1) * M + m]);
         }
         const int st = search_right(freqs, N, lo), et = search_right(freqs, N, hi);
-        const int nch = et > st ? (et - st + chunk - 1) / chunk : 0;
+        const int nch = et > st ? (et - 1) / chunk - st / chunk + 1 : 0;
         band[3 * b] = st;
         band[3 * b + 1] = et;
         band[3 * b + 2] = nch;  // count for now
@@ -379,6 +388,20 @@ __global__ __launch_bounds__(1024) void k_dir_band(int* __restrict__ band, int* 
     if (t == 1023) total[0] = part[1023];
 }
 
+// Execution order of the work items, chunk-major: perm lists the items of
+// absolute chunk 0, then of chunk 1, ... (a counting sort; the order inside a
+// chunk is whatever the atomics give, which does not affect any result: each
+// item writes its own partial, reduced per binary in item order). Concurrent
+// workgroups then work on the same dense bins for different binaries, so the
+// data and noise weights of a chunk come from L2 instead of being fetched again
+// for every binary. cnt has nchunks + 1 entries.
+__global__ void k_dir_cnt_zero(int* __restrict__ cnt, int n, int* __restrict__ next)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) cnt[i] = 0;
+    if (i == 0) next[0] = 0;
+}
+
 // binary owning work item 'it' (largest b with chunk_begin <= it)
 __device__ inline int owner(const int* __restrict__ band, int B, int it)
 {
@@ -389,6 +412,57 @@ __device__ inline int owner(const int* __restrict__ band, int B, int it)
         else hi = mid - 1;
     }
     return lo;
+}
+
+// absolute chunk of work item it
+__device__ inline int item_chunk(const int* __restrict__ band, int B, int it, int chunk, int& b)
+{
+    b = owner(band, B, it);
+    return band[3 * b] / chunk + (it - band[3 * b + 2]);
+}
+
+// one lane per work item
+__global__ void k_dir_cnt(int* __restrict__ cnt, const int* __restrict__ band, const int* __restrict__ total, int B,
+                          int chunk)
+{
+    const int it = blockIdx.x * blockDim.x + threadIdx.x;
+    if (it >= total[0]) return;
+    int b;
+    atomicAdd(&cnt[item_chunk(band, B, it, chunk, b)], 1);
+}
+
+// exclusive scan of cnt[0, n) in place; one workgroup of 1024 threads
+__global__ __launch_bounds__(1024) void k_dir_cnt_scan(int* __restrict__ cnt, int n)
+{
+    __shared__ int part[1024];
+    const int t = threadIdx.x;
+    const int per = (n + 1023) / 1024;
+    const int g0 = t * per, g1 = min(n, g0 + per);
+    int sum = 0;
+    for (int g = g0; g < g1; g++) sum += cnt[g];
+    part[t] = sum;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        const int v = t >= o ? part[t - o] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int run = t > 0 ? part[t - 1] : 0;
+    for (int g = g0; g < g1; g++) {
+        const int c = cnt[g];
+        cnt[g] = run;
+        run += c;
+    }
+}
+
+__global__ void k_dir_perm(int* __restrict__ perm, int* __restrict__ cnt, const int* __restrict__ band,
+                           const int* __restrict__ total, int B, int chunk)
+{
+    const int it = blockIdx.x * blockDim.x + threadIdx.x;
+    if (it >= total[0]) return;
+    int b;
+    perm[atomicAdd(&cnt[item_chunk(band, B, it, chunk, b)], 1)] = it;
 }
 
 // h[ch] += conj(T_ch A e^{i phase}) of one mode at x = f - x_i into its segment, if
@@ -457,7 +531,8 @@ constexpr int RB = 2;  // iterations of 64 dense bins per block of a wavefront
 // searchsorted gives. Where an iteration spans more than three segments (low
 // frequencies, short segments) each lane searches its own (mode_per_lane).
 __global__ __launch_bounds__(256) void k_dir_like(double* __restrict__ partial, const int* __restrict__ band,
-                                                  const int* __restrict__ total, const SegRec* __restrict__ R,
+                                                  const int* __restrict__ total, const int* __restrict__ perm,
+                                                  int* __restrict__ next, const SegRec* __restrict__ R,
                                                   const double* __restrict__ grid_lo,
                                                   const double* __restrict__ grid_istep,
                                                   const long* __restrict__ off, const int* __restrict__ len,
@@ -467,13 +542,23 @@ __global__ __launch_bounds__(256) void k_dir_like(double* __restrict__ partial, 
                                                   int B, int M, int chunk)
 {
     __shared__ double red[3][256 / 64];
+    __shared__ int s_next;
     const int nitems = total[0];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nw = blockDim.x >> 6;
-    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
-        const int b = owner(band, B, it);
-        const int c = it - band[3 * b + 2];
-        const int k0 = band[3 * b] + c * chunk;
-        const int k1 = min(band[3 * b + 1], k0 + chunk);
+    for (;;) {
+        // the next item in the chunk-major order (dynamic: the items running at
+        // any time stay a window of consecutive entries, i.e. the same chunks)
+        if (tid == 0) s_next = atomicAdd(next, 1);
+        __syncthreads();
+        const int pi = s_next;
+        __syncthreads();
+        if (pi >= nitems) break;
+        const int it = perm[pi];
+        int b;
+        const int g = item_chunk(band, B, it, chunk, b);
+        const int st = band[3 * b];
+        const int k0 = max(st, g * chunk);
+        const int k1 = min(band[3 * b + 1], (g + 1) * chunk);
         const long base = off[b];
         const int L = len[b];
         const double ts = t_start[b], te = t_end[b];
@@ -609,7 +694,7 @@ T* dalloc(size_t n, bool& ok)
 void plan_free(bbhx_direct_plan* p)
 {
     void* bufs[] = {p->off, p->len, p->recs, p->X, p->Cf, p->grid_lo, p->grid_istep, p->band,
-                    p->total, p->partial};
+                    p->total, p->partial, p->perm, p->cnt, p->next};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
 }
@@ -658,6 +743,10 @@ extern "C" int bbhx_hip_direct_plan_create(bbhx_direct_plan** out, int numBinAll
     p->band = dalloc<int>((size_t)numBinAll * 3, ok);
     p->total = dalloc<int>(1, ok);
     p->partial = dalloc<double>((size_t)p->max_items * 4, ok);
+    p->max_chunks = (data_length + min_chunk - 1) / min_chunk;
+    p->perm = dalloc<int>((size_t)p->max_items, ok);
+    p->cnt = dalloc<int>((size_t)p->max_chunks + 1, ok);
+    p->next = dalloc<int>(1, ok);
     if (!ok) {
         plan_free(p);
         delete p;
@@ -686,7 +775,8 @@ extern "C" long bbhx_hip_direct_plan_bytes(const bbhx_direct_plan* p)
 {
     if (!p) return -1;
     const long SM = p->S * p->M;
-    return SM * 8 * (1 + RS) + (long)p->B * p->M * (sizeof(FusedRec) + 16) + p->max_items * 32 + p->B * 24;
+    return SM * 8 * (1 + RS) + (long)p->B * p->M * (sizeof(FusedRec) + 16) + p->max_items * 36 + p->B * 24 +
+           4L * (p->max_chunks + 1);
 }
 
 extern "C" int bbhx_hip_direct_plan_band(const bbhx_direct_plan* p, int* band_host)
@@ -737,8 +827,16 @@ extern "C" int bbhx_hip_direct_fused(bbhx_direct_plan* p, double* d_h, double* h
     hipLaunchKernelGGL(k_dir_spline, dim3(B), dim3(64), 0, st, p->X, p->Cf, p->off, p->len, M);
     hipLaunchKernelGGL(k_dir_band, dim3(1), dim3(1024), 0, st, p->band, p->total, p->X, p->off, p->len, dataFreqs,
                        p->N, B, M, chunk);
+    const int nchunks = (p->N + chunk - 1) / chunk;  // <= max_chunks since chunk >= min_chunk
+    hipLaunchKernelGGL(k_dir_cnt_zero, dim3(ceil_div(nchunks + 1, 256)), dim3(256), 0, st, p->cnt, nchunks + 1,
+                       p->next);
+    // one lane per possible work item (those past the batch's count exit)
+    const int item_blocks = ceil_div(p->max_items, 256);
+    hipLaunchKernelGGL(k_dir_cnt, dim3(item_blocks), dim3(256), 0, st, p->cnt, p->band, p->total, B, chunk);
+    hipLaunchKernelGGL(k_dir_cnt_scan, dim3(1), dim3(1024), 0, st, p->cnt, nchunks);
+    hipLaunchKernelGGL(k_dir_perm, dim3(item_blocks), dim3(256), 0, st, p->perm, p->cnt, p->band, p->total, B, chunk);
     if (timed) (void)hipEventRecord(ev[1], st);
-    hipLaunchKernelGGL(k_dir_like, dim3(grid), dim3(block), 0, st, p->partial, p->band, p->total,
+    hipLaunchKernelGGL(k_dir_like, dim3(grid), dim3(block), 0, st, p->partial, p->band, p->total, p->perm, p->next,
                        reinterpret_cast<const SegRec*>(p->Cf), p->grid_lo, p->grid_istep, p->off, p->len, t_start,
                        t_end, dataFreqs,
                        reinterpret_cast<const cd*>(dataChannels), noiseFactors, p->N, B, M, chunk);

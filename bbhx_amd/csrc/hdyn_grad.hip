@@ -5,12 +5,14 @@
 // differentiated is the one HeterodynedLikelihood.get_ll returns
 // (likelihood.py:550-648): logL = Re d_h - 1/2 Re h_h - 1/2 d_d.
 //
-// k_grad_setup   one lane per (template, mode): the double record of the
-//                forward pass and a PhenomHM record whose every coefficient
-//                carries its derivatives with respect to (ln M, q, chi1, chi2)
-//                (Dual<4> instantiation of the set-up code), plus the
-//                polarisation basis, wave vector and Y factors with their
-//                derivatives in lam, beta (analytic) and cos inc.
+// k_grad_setup_fwd one lane per (template, mode): the double record of the
+//                forward pass;
+// k_grad_setup   (on a side stream, concurrently with the two kernels that
+//                follow) one lane per (template, mode): a PhenomHM record whose
+//                every coefficient carries its derivatives with respect to
+//                (ln M, q, chi1, chi2) (Dual<4> instantiation of the set-up
+//                code), plus the polarisation basis, wave vector and Y factors
+//                with their derivatives in lam, beta (analytic) and cos inc.
 // k_grad_weights one 64-lane wavefront per template:
 //   1. forward: residuals r = h / h0 on the K nodes (LDS) and d_h, h_h,
 //      exactly as the forward kernel;
@@ -113,22 +115,26 @@ __device__ inline void grad_resp_setup(GradResp& r, int l, int m, double phi, do
     r.dYc = mul_i((dc * dci) * e);
 }
 
-__global__ __launch_bounds__(64) void k_grad_setup(FusedRec* __restrict__ recs, GradRec* __restrict__ grecs,
-                                                   const double* __restrict__ params, int B, ModeList modes, int M,
-                                                   const double* __restrict__ qnm, int table_len, double dspin)
+// the double records of the forward pass, one lane per (template, mode)
+__global__ __launch_bounds__(64) void k_grad_setup_fwd(FusedRec* __restrict__ recs, const double* __restrict__ params,
+                                                       int B, ModeList modes, int M, const double* __restrict__ qnm,
+                                                       int table_len, double dspin)
 {
-    // lanes [0, B M) build the double records, lanes [B M, 2 B M) the records
-    // with tangents: the two set-ups run side by side
-    const int gidx = blockIdx.x * blockDim.x + threadIdx.x;
-    const int nrec = B * M;
-    if (gidx >= 2 * nrec) return;
-    const int idx = gidx < nrec ? gidx : gidx - nrec;
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= B * M) return;
+    const int b = idx / M, mi = idx % M;
+    setup_one(recs[idx], params, B, b, modes.ell[mi], modes.mm[mi], qnm, table_len, dspin);
+}
+
+// the records with tangents, one lane per (template, mode)
+__global__ __launch_bounds__(64) void k_grad_setup(GradRec* __restrict__ grecs, const double* __restrict__ params,
+                                                   int B, ModeList modes, int M, const double* __restrict__ qnm,
+                                                   int table_len, double dspin)
+{
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= B * M) return;
     const int b = idx / M, mi = idx % M;
     const int l = modes.ell[mi], m = modes.mm[mi];
-    if (gidx < nrec) {
-        setup_one(recs[idx], params, B, b, l, m, qnm, table_len, dspin);
-        return;
-    }
     // masses from (ln M, q): m1 = M q / (1 + q), m2 = M / (1 + q)
     const double m1 = params[P_M1 * B + b], m2 = params[P_M2 * B + b];
     const double Mt = m1 + m2, q = m1 / m2;
@@ -204,54 +210,42 @@ __device__ inline void grad_point(const GradRec& G, const resp::OrbitTab& o, dou
     if (!in_window(win, t)) return;
     const GradResp& R = G.s;
 
-    // orbit projections and their derivatives in t, lam, beta (index 0, 1, 2)
+    // orbit projections (values), their time derivatives, and the spacecraft
+    // projections' derivatives in lam, beta; the links' derivatives in lam,
+    // beta are formed later, link by link, to keep fewer values live
     int w = int(floor(t * o.inv_dt));
     w = w < 0 ? 0 : (w > o.N - 2 ? o.N - 2 : w);
     const double a = (t - o.dt * w) * o.inv_dt;
+    const double* x0 = o.x + (long)w * 9;
+    const double* n0 = o.n + (long)w * 18;
+    const int li[3] = {o.li12, o.li23, o.li31};
     double kp[3], dkp[3][3];
-    {
-        const double* x0 = o.x + (long)w * 9;
-        for (int i = 0; i < 3; i++) {
-            const double* lo = x0 + 3 * i;
-            const double* hi = lo + 9;
-            const double l0 = resp::dot3(R.k, lo), h0v = resp::dot3(R.k, hi);
-            kp[i] = fma(h0v - l0, a, l0);
-            dkp[0][i] = (h0v - l0) * o.inv_dt;
-            for (int q = 0; q < 2; q++) {
-                const double lq = resp::dot3(R.dk[q], lo), hq = resp::dot3(R.dk[q], hi);
-                dkp[1 + q][i] = fma(hq - lq, a, lq);
-            }
+    for (int i = 0; i < 3; i++) {
+        const double* lo = x0 + 3 * i;
+        const double* hi = lo + 9;
+        const double l0 = resp::dot3(R.k, lo), h0v = resp::dot3(R.k, hi);
+        kp[i] = fma(h0v - l0, a, l0);
+        dkp[0][i] = (h0v - l0) * o.inv_dt;
+        for (int q = 0; q < 2; q++) {
+            const double lq = resp::dot3(R.dk[q], lo), hq = resp::dot3(R.dk[q], hi);
+            dkp[1 + q][i] = fma(hq - lq, a, lq);
         }
     }
-    double kn[3], un[3], vn[3], dkn[3][3], dun[3][3], dvn[3][3];
-    {
-        const double* n0 = o.n + (long)w * 18;
-        const int li[3] = {o.li12, o.li23, o.li31};
-        for (int j = 0; j < 3; j++) {
-            const double* lo = n0 + 3 * li[j];
-            const double* hi = lo + 18;
-            double l0 = resp::dot3(R.k, lo), h0v = resp::dot3(R.k, hi);
-            kn[j] = fma(h0v - l0, a, l0);
-            dkn[0][j] = (h0v - l0) * o.inv_dt;
-            l0 = resp::dot3(R.u, lo);
-            h0v = resp::dot3(R.u, hi);
-            un[j] = fma(h0v - l0, a, l0);
-            dun[0][j] = (h0v - l0) * o.inv_dt;
-            l0 = resp::dot3(R.v, lo);
-            h0v = resp::dot3(R.v, hi);
-            vn[j] = fma(h0v - l0, a, l0);
-            dvn[0][j] = (h0v - l0) * o.inv_dt;
-            for (int q = 0; q < 2; q++) {
-                double lq = resp::dot3(R.dk[q], lo), hq = resp::dot3(R.dk[q], hi);
-                dkn[1 + q][j] = fma(hq - lq, a, lq);
-                lq = resp::dot3(R.du[q], lo);
-                hq = resp::dot3(R.du[q], hi);
-                dun[1 + q][j] = fma(hq - lq, a, lq);
-                lq = resp::dot3(R.dv[q], lo);
-                hq = resp::dot3(R.dv[q], hi);
-                dvn[1 + q][j] = fma(hq - lq, a, lq);
-            }
-        }
+    double kn[3], un[3], vn[3], dkn_t[3], dun_t[3], dvn_t[3];
+    for (int j = 0; j < 3; j++) {
+        const double* lo = n0 + 3 * li[j];
+        const double* hi = lo + 18;
+        double l0 = resp::dot3(R.k, lo), h0v = resp::dot3(R.k, hi);
+        kn[j] = fma(h0v - l0, a, l0);
+        dkn_t[j] = (h0v - l0) * o.inv_dt;
+        l0 = resp::dot3(R.u, lo);
+        h0v = resp::dot3(R.u, hi);
+        un[j] = fma(h0v - l0, a, l0);
+        dun_t[j] = (h0v - l0) * o.inv_dt;
+        l0 = resp::dot3(R.v, lo);
+        h0v = resp::dot3(R.v, hi);
+        vn[j] = fma(h0v - l0, a, l0);
+        dvn_t[j] = (h0v - l0) * o.inv_dt;
     }
 
     // orbital Doppler phase and its derivatives (Response.cu:278-296)
@@ -360,6 +354,25 @@ __device__ inline void grad_point(const GradRec& G, const resp::OrbitTab& o, dou
     cd Q[5] = {mk(0.0, 0.0), mk(0.0, 0.0), mk(0.0, 0.0), mk(0.0, 0.0), mk(0.0, 0.0)};
     const cd ix = mk(0.0, x);
     for (int L = 0; L < 3; L++) {
+        // this link's projections differentiated in t (0), lam (1), beta (2)
+        double dknL[3], dunL[3], dvnL[3];
+        dknL[0] = dkn_t[L];
+        dunL[0] = dun_t[L];
+        dvnL[0] = dvn_t[L];
+        {
+            const double* lo = n0 + 3 * li[L];
+            const double* hi = lo + 18;
+            for (int q = 0; q < 2; q++) {
+                double lq = resp::dot3(R.dk[q], lo), hq = resp::dot3(R.dk[q], hi);
+                dknL[1 + q] = fma(hq - lq, a, lq);
+                lq = resp::dot3(R.du[q], lo);
+                hq = resp::dot3(R.du[q], hi);
+                dunL[1 + q] = fma(hq - lq, a, lq);
+                lq = resp::dot3(R.dv[q], lo);
+                hq = resp::dot3(R.dv[q], hi);
+                dvnL[1 + q] = fma(hq - lq, a, lq);
+            }
+        }
         const cd BL = bm[L] * sm[L] + bp[L] * sp[L];
         const cd CL = bp[L] * dsp[L] - bm[L] * dsm[L];
         const cd eB = eL[L] * BL;
@@ -368,10 +381,10 @@ __device__ inline void grad_point(const GradRec& G, const resp::OrbitTab& o, dou
         const cd egB = eg * BL * ix * o.inv_L;
         const cd ixeB = ix * eB;
         for (int d = 0; d < 3; d++) {
-            const double dU = 2.0 * (un[L] * dun[d][L] - vn[L] * dvn[d][L]);
-            const double dV = 2.0 * (dun[d][L] * vn[L] + un[L] * dvn[d][L]);
+            const double dU = 2.0 * (un[L] * dunL[d] - vn[L] * dvnL[d]);
+            const double dV = 2.0 * (dunL[d] * vn[L] + un[L] * dvnL[d]);
             const cd dnHn = R.Yp * dU + R.Yc * dV;
-            Q[d] += ixeB * dnHn + egC * dkn[d][L] + egB * (dkp[d][ka[L]] + dkp[d][kb[L]]);
+            Q[d] += ixeB * dnHn + egC * dknL[d] + egB * (dkp[d][ka[L]] + dkp[d][kb[L]]);
         }
         // psi: du = v, dv = -u
         {
@@ -466,6 +479,28 @@ __global__ void k_grad_reduce(double* __restrict__ grad, const double* __restric
     grad[(long)p * B + b] = s;
 }
 
+// One side stream (default priority) and a fork / join event pair per device,
+// created on first use and kept for the life of the process.
+struct SideStream {
+    hipStream_t stream = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+};
+
+SideStream* side_stream()
+{
+    static SideStream per_dev[64];
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+    SideStream& s = per_dev[dev];
+    if (!s.stream) {
+        if (hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking) != hipSuccess) return nullptr;
+        if (hipEventCreateWithFlags(&s.fork, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&s.join, hipEventDisableTiming) != hipSuccess)
+            return nullptr;
+    }
+    return &s;
+}
+
 }  // namespace
 
 extern "C" int bbhx_hip_hdyn_fused_grad(double* d_h, double* h_h, double* grad, const double* params,
@@ -496,10 +531,21 @@ extern "C" int bbhx_hip_hdyn_fused_grad(double* d_h, double* h_h, double* grad, 
     if (!recs || !grecs || !scratch) return ERR_ALLOC;
     cd* gw = reinterpret_cast<cd*>(scratch);
     double* partial = reinterpret_cast<double*>(scratch + gw_bytes);
+    // The records with tangents are needed only by k_grad_points; their set-up
+    // (few lanes, long serial chains) runs on a side stream, beside the forward
+    // records and the weights pass on the caller's stream (event fork / join, so
+    // the sequence stays capturable in a graph).
+    SideStream* side = side_stream();
+    if (!side) return ERR_LAUNCH;
     hipEvent_t ev[3];
     const bool timed = timer_take3(ev);
     if (timed) (void)hipEventRecord(ev[0], st);
-    hipLaunchKernelGGL(k_grad_setup, dim3(ceil_div(2 * (long)nrec, 64)), dim3(64), 0, st, recs, grecs, params, numBinAll, ml,
+    if (hipEventRecord(side->fork, st) != hipSuccess || hipStreamWaitEvent(side->stream, side->fork, 0) != hipSuccess)
+        return ERR_LAUNCH;
+    hipLaunchKernelGGL(k_grad_setup, dim3(ceil_div((long)nrec, 64)), dim3(64), 0, side->stream, grecs, params,
+                       numBinAll, ml, numModes, qnm, table_len, dspin);
+    if (hipEventRecord(side->join, side->stream) != hipSuccess) return ERR_LAUNCH;
+    hipLaunchKernelGGL(k_grad_setup_fwd, dim3(ceil_div((long)nrec, 64)), dim3(64), 0, st, recs, params, numBinAll, ml,
                        numModes, qnm, table_len, dspin);
     if (hipGetLastError() != hipSuccess) return ERR_LAUNCH;
     if (timed) (void)hipEventRecord(ev[1], st);
@@ -508,6 +554,7 @@ extern "C" int bbhx_hip_hdyn_fused_grad(double* d_h, double* h_h, double* grad, 
                        params, t_start, t_end, numBinAll, numModes, freqs, reinterpret_cast<const cd*>(h0),
                        reinterpret_cast<const cd*>(dataConstants), K, orb, TDItag, rescaled, tdi2, jend,
                        initial_t_val);
+    if (hipStreamWaitEvent(st, side->join, 0) != hipSuccess) return ERR_LAUNCH;
     hipLaunchKernelGGL(k_grad_points, dim3(nrec), dim3(64), 0, st, partial, grecs, gw, params, t_start, t_end,
                        numBinAll, numModes, freqs, K, orb, TDItag, rescaled, tdi2, initial_t_val);
     hipLaunchKernelGGL(k_grad_reduce, dim3(ceil_div((long)numBinAll * NGRAD, 256)), dim3(256), 0, st, grad, partial,

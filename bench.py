@@ -190,6 +190,45 @@ def bench_direct_ragged(args):
     log(f"graph replay: {ms:.3f} ms/step, {B / (ms * 1e-3):.4g} evals/s; |graph - eager| = {diff:.2e}; "
         f"plan {plan.nbytes() / 2**30:.2f} GiB; {dense} dense (bin, template) pairs "
         f"({dense / (ms * 1e-3):.4g}/s); split {split}")
+
+    # roofline of the dense-band kernel: the oracle's counted FP64 work per dense
+    # (bin, template) x the pairs of this batch, over the kernel's event-timed duration
+    model = json.loads((ROOT / "bbhx_amd" / "flop_model.json").read_text())["direct"]
+    roofline = None
+    if split:
+        flops = model["flops_per_dense_bin_template"] * dense
+        achieved = flops / (split["dense_ms"] * 1e-3) / 1e12
+        # unique HBM bytes of the launch: data and noise weights of the bands once
+        # (24 B per channel and bin, + 8 B frequency) and every segment record once
+        unique = 80.0 * f.size + 320.0 * len(MODES) * float(lengths.sum())
+        roofline = {"bound": "mfma", "compute": "FP64 VALU (no dense contraction, no MFMA; peak = FP64 vector peak)",
+                    "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "kernel": "k_dir_like",
+                    "kernel_ms": split["dense_ms"], "flops_per_launch": flops,
+                    "flops_per_dense_bin_template": model["flops_per_dense_bin_template"],
+                    "unique_hbm_bytes": unique,
+                    "unique_hbm_frac": unique / (split["dense_ms"] * 1e-3) / (HBM_PEAK_GBS * 1e9)}
+
+    cpu = None
+    if not args.no_cpu:
+        # the FP64 CPU oracle template by template at each template's own length
+        # (its loops run over templates, so one template uses one core)
+        from oracle.pipeline import OracleLikelihood, OracleWaveform
+
+        log("timing the CPU oracle")
+        owave = OracleWaveform(orbits.as_host_dict(), tdi2=True)
+        olike = OracleLikelihood(owave, f, data, psd)
+        t0 = time.perf_counter()
+        done, worst = 0, 0.0
+        while done < B and (time.perf_counter() - t0 < args.cpu_seconds or done == 0):
+            llo = olike.get_ll(p[:, done:done + 1], length=int(lengths[done]), modes=MODES)[0][0]
+            worst = max(worst, abs(llo - float(eager[done])))
+            done += 1
+        sec = time.perf_counter() - t0
+        cpu = {"value": done / sec, "unit": "logL evals/s", "cores": 1, "kind": "port",
+               "sample": f"FP64 CPU oracle, the first {done} templates of the batch one by one at their own "
+                         f"sparse lengths, {sec:.1f} s; max |dlogL| vs GPU {worst:.2e}"}
+        log(f"cpu oracle: {done / sec:.4g} evals/s on 1 core over {done} templates; max |dlogL| vs GPU {worst:.2e}")
     out = {
         "metric": "direct logL evals/sec, 1 GPU",
         "value": B / (ms * 1e-3),
@@ -210,6 +249,8 @@ def bench_direct_ragged(args):
                    "lengths_sum": int(lengths.sum()), "dense_bins": int(f.size), "modes": 6, "tdi": "AET tdi2",
                    "launch": {k: tuned[k] for k in ("chunk", "block", "grid")}, "graph_vs_eager": diff,
                    "dense_bin_templates": dense, **split},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 

@@ -184,7 +184,7 @@ int bbhx_hip_hdyn_fused_grad(double* d_h, double* h_h, double* grad, const doubl
 
 /* Likelihood.get_ll (likelihood.py:140-225) with BBHWaveformFD(direct=False)
  * (waveformbuild.py:338-700) for templates whose sparse grids have different
- * lengths L_b, in one sequence of six launches with no host synchronisation
+ * lengths L_b, in one sequence of launches with no host synchronisation
  * and no allocation, so that the call can be captured in a hipGraph. A plan
  * holds the sizes and the device buffers (sparse grids, the nine spline
  * quantities and their coefficients: 328 B per (knot, mode), plus per-binary

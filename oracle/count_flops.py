@@ -110,11 +110,89 @@ def count(K, sample, dense_bits):
     return counts, err, B, names
 
 
+def count_direct(sample, cadence=10.0, years=1.0):
+    """Operation counts of the dense evaluation and direct inner products
+    (orc_interp_tdi + orc_direct_like, counted) for the first `sample`
+    templates of the uneven-batching workload (seed 1002, total mass
+    log-uniform 1e4-1e8 Msun, sparse lengths by the bench's rule, data over
+    `years` at `cadence` s), each at its own sparse length, as
+    Likelihood.get_ll runs them (likelihood.py:140-225). Returns the counts,
+    the number of dense (bin, template) pairs evaluated and the largest
+    relative difference of the counted d_h, h_h against the uncounted oracle."""
+    import bench
+    import oracle.pipeline as pl
+    from bbhx_amd.utils.constants import YRSID_SI
+    from bbhx_amd.utils.orbits import EqualArmlengthOrbits
+    from bbhx_amd.utils.sensitivity import AET1SensitivityMatrix
+    from oracle.pipeline import OracleLikelihood, OracleWaveform
+    from tests.params import MODES, draw
+
+    T = years * YRSID_SI
+    f = np.fft.rfftfreq(int(T // cadence), cadence)[1:]
+    p = draw(8192, 1002, mtot=(1e4, 1e8))[:, :sample]
+    lengths = bench.ragged_lengths(p, YRSID_SI)
+    owave = OracleWaveform(EqualArmlengthOrbits(dt=1e4, t_max=3.0 * YRSID_SI).as_host_dict(), tdi2=True)
+    sens = AET1SensitivityMatrix(f)
+    psd = np.array([sens[0], sens[1], sens[2]])
+    rng = np.random.default_rng(1003)
+    data = np.sqrt(psd / (4 * (f[1] - f[0]))) * (rng.standard_normal(psd.shape) + 1j * rng.standard_normal(psd.shape))
+    olike = OracleLikelihood(owave, f, data, psd)
+
+    lib = _lib()
+    nops = lib.opc_num_ops()
+    names = lib.opc_op_names().decode().split(",")
+    total = np.zeros(nops)
+    pairs = 0
+    worst = 0.0
+    captured = {}
+    orig_interp, orig_direct = pl.interp_tdi, pl.direct_like
+
+    def interp_hook(*a, **k):
+        captured["interp"] = a
+        return orig_interp(*a, **k)
+
+    def direct_hook(data_channels, noise_factors, templates, inds_start, ind_lengths, data_length, **k):
+        nonlocal total, pairs, worst
+        out = orig_direct(data_channels, noise_factors, templates, inds_start, ind_lengths, data_length, **k)
+        (data_freqs, freqs, y, c1, c2, c3, t_start, t_end, length, B, M, inds_list, i_st, i_len) = captured["interp"]
+        f64 = lambda x: np.ascontiguousarray(x, dtype=np.float64)  # noqa: E731
+        i_len = np.ascontiguousarray(i_len, dtype=np.int32)
+        ioff = np.zeros(B, dtype=np.int64)
+        ioff[1:] = np.cumsum(M * i_len.astype(np.int64))[:-1]
+        inds = np.ascontiguousarray(np.concatenate(inds_list), dtype=np.int32)
+        counts = np.zeros(nops)
+        dh = np.zeros(B, dtype=np.complex128)
+        hh = np.zeros(B, dtype=np.complex128)
+        P = lambda x: x.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+        args = [f64(data_freqs), f64(freqs), f64(y), f64(c1), f64(c2), f64(c3), f64(t_start), f64(t_end)]
+        dch = np.ascontiguousarray(data_channels, dtype=np.complex128)
+        nf = f64(noise_factors)
+        lib.opc_interp_direct(P(counts), P(dh), P(hh), P(args[0]), int(data_length), P(args[1]), P(args[2]),
+                              P(args[3]), P(args[4]), P(args[5]), P(args[6]), P(args[7]), int(length), int(B), int(M),
+                              P(inds), P(ioff), P(np.ascontiguousarray(i_st, dtype=np.int32)), P(i_len), P(dch),
+                              P(nf))
+        d_ref, h_ref = out
+        worst = max(worst, float(np.max(np.abs(dh - d_ref) / np.maximum(np.abs(d_ref), 1e-300))),
+                    float(np.max(np.abs(hh - h_ref) / np.maximum(np.abs(h_ref), 1e-300))))
+        total += counts
+        pairs += int(i_len.sum())
+        return out
+
+    pl.interp_tdi, pl.direct_like = interp_hook, direct_hook
+    try:
+        for b in range(p.shape[1]):
+            olike.get_ll(p[:, b:b + 1], length=int(lengths[b]), modes=MODES)
+    finally:
+        pl.interp_tdi, pl.direct_like = orig_interp, orig_direct
+    return total, pairs, worst, names, lengths
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--sample", type=int, default=64)
     ap.add_argument("--nodes", type=int, default=128)
     ap.add_argument("--dense-bits", type=int, default=20)
+    ap.add_argument("--direct-sample", type=int, default=8)
     ap.add_argument("--out", default=str(ROOT / "bbhx_amd" / "flop_model.json"))
     args = ap.parse_args()
 
@@ -147,6 +225,19 @@ def main():
         "nodes": K,
         "generated_by": "python oracle/count_flops.py",
         "max_rel_ll_diff_vs_oracle": max(err, res[2 * K][1]),
+    }
+    dcounts, pairs, dworst, dnames, dlens = count_direct(args.direct_sample)
+    if dworst > 1e-9:
+        raise RuntimeError(f"counted direct run differs from the oracle: {dworst}")
+    dflops = sum(dcounts[i] for i, n in enumerate(dnames) if n in ARITH + LIBM)
+    model["direct"] = {
+        "workload": f"dense evaluation + direct inner products (interp_tdi + direct_like), first "
+                    f"{args.direct_sample} templates of the uneven batch (seed 1002), 1 yr at 10 s, six modes, "
+                    f"TDI-2 AET, sparse lengths {dlens.tolist()}",
+        "flops_per_dense_bin_template": dflops / pairs,
+        "libm_calls_per_dense_bin_template": sum(dcounts[i] for i, n in enumerate(dnames) if n in LIBM) / pairs,
+        "dense_bin_templates_counted": pairs,
+        "max_rel_dh_hh_diff_vs_oracle": dworst,
     }
     Path(args.out).write_text(json.dumps(model, indent=1) + "\n")
     print(json.dumps(model, indent=1))

@@ -112,4 +112,43 @@ int opc_hdyn_forward(double* counts, double* ll, const double* params, int B, co
     return 0;
 }
 
+// Counted dense evaluation and direct inner products of one batch
+// (TemplateInterpFD + Likelihood.get_ll, waveformbuild.py:89-239,
+// likelihood.py:140-225): orc_interp_tdi then orc_direct_like on the same
+// arguments the uncounted oracle receives (see oracle/__init__.py). Returns
+// the counts and d_h, h_h (B,) complex for checking against the uncounted run.
+int opc_interp_direct(double* counts, double* d_h, double* h_h, const double* dataFreqs, int data_length,
+                      const double* freqs, const double* y, const double* c1, const double* c2, const double* c3,
+                      const double* t_start, const double* t_end, int length, int numBinAll, int numModes,
+                      const int* inds, const long* inds_offsets, const int* inds_start, const int* ind_lengths,
+                      const double* dataChannels, const double* noise)
+{
+    using opc::Real;
+    const long nper = (long)numBinAll * numModes * length;
+    auto R = [](const double* p, size_t n) { return std::vector<Real>(p, p + n); };
+    std::vector<Real> df = R(dataFreqs, data_length), fr = R(freqs, nper), Y = R(y, 9 * nper), C1 = R(c1, 9 * nper),
+                      C2 = R(c2, 9 * nper), C3 = R(c3, 9 * nper), ts = R(t_start, numBinAll),
+                      te = R(t_end, numBinAll), D = R(dataChannels, 6 * (size_t)data_length),
+                      W = R(noise, 3 * (size_t)data_length);
+    std::vector<long> toff(numBinAll);
+    long tot = 0;
+    for (int b = 0; b < numBinAll; b++) {
+        toff[b] = tot;
+        tot += 3L * ind_lengths[b];
+    }
+    std::vector<Real> tmpl(2 * (size_t)tot, Real(0.0)), dh(2 * (size_t)numBinAll, Real(0.0)),
+        hh(2 * (size_t)numBinAll, Real(0.0));
+    std::memset(&opc::g, 0, sizeof(opc::g));
+    orc_interp_tdi(tmpl.data(), toff.data(), df.data(), fr.data(), Y.data(), C1.data(), C2.data(), C3.data(),
+                   ts.data(), te.data(), length, numBinAll, numModes, inds, inds_offsets, inds_start, ind_lengths);
+    orc_direct_like(dh.data(), hh.data(), D.data(), W.data(), tmpl.data(), toff.data(), inds_start, ind_lengths,
+                    data_length, numBinAll);
+    for (int b = 0; b < 2 * numBinAll; b++) {
+        d_h[b] = dh[b].v;
+        h_h[b] = hh[b].v;
+    }
+    for (int o = 0; o < opc::NOPS; o++) counts[o] = double(opc::g.n[o]);
+    return 0;
+}
+
 }  // extern "C"
