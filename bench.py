@@ -201,9 +201,13 @@ def bench_direct_ragged(args):
         # unique HBM bytes of the launch: data and noise weights of the bands once
         # (24 B per channel and bin, + 8 B frequency) and every segment record once
         unique = 80.0 * f.size + 320.0 * len(MODES) * float(lengths.sum())
+        traffic = None
+        tfile = ROOT / "profiles" / "k_dir_like.direct_ragged_1024.traffic.json"
+        if tfile.exists() and B == 1024 and not args.uniform_length and args.direct_years == 1.0:
+            traffic = json.loads(tfile.read_text())["hbm_bytes_per_launch"]
         roofline = {"bound": "mfma", "compute": "FP64 VALU (no dense contraction, no MFMA; peak = FP64 vector peak)",
                     "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                    "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None, "kernel": "k_dir_like",
+                    "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic, "kernel": "k_dir_like",
                     "kernel_ms": split["dense_ms"], "flops_per_launch": flops,
                     "flops_per_dense_bin_template": model["flops_per_dense_bin_template"],
                     "unique_hbm_bytes": unique,
