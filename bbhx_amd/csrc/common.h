@@ -84,6 +84,16 @@ __device__ inline double cbrt_pos(double x)
     return y * (y3 + 2.0 * x) * rcp_pos(2.0 * y3 + x);
 }
 
+// A polynomial coefficient as a scalar-register operand at its point of use
+// (rebuilt there by two scalar moves): the FMA chains below then take it as
+// their addend directly, instead of copying a vector-register copy of the
+// constant into the accumulator before each fmac.
+__device__ inline double scoef(double c)
+{
+    asm volatile("" : "+s"(c));
+    return c;
+}
+
 // sin and cos for |x| <= ~4 (e.g. after reduce_2pi): quadrant reduction by
 // pi/2 = HI + LO, then the Taylor series of sin (to x^17) and cos (to x^18)
 // on |r| <= pi/4, whose truncation errors (< 1e-19) are below FP64 rounding.
@@ -99,23 +109,23 @@ __device__ inline void sincos_small(double x, double* s, double* c)
     const double r2 = r * r;
     // sin(r) = r (1 - r^2/3! + r^4/5! - ...), cos(r) = 1 - r^2/2! + r^4/4! - ...
     double ps = -1.0 / 355687428096000.0;            // -1/17!
-    ps = fma(ps, r2, 1.0 / 1307674368000.0);          // 1/15!
-    ps = fma(ps, r2, -1.0 / 6227020800.0);            // -1/13!
-    ps = fma(ps, r2, 1.0 / 39916800.0);               // 1/11!
-    ps = fma(ps, r2, -1.0 / 362880.0);                // -1/9!
-    ps = fma(ps, r2, 1.0 / 5040.0);                   // 1/7!
-    ps = fma(ps, r2, -1.0 / 120.0);                   // -1/5!
-    ps = fma(ps, r2, 1.0 / 6.0);                      // 1/3! (sign applied below)
+    ps = fma(ps, r2, scoef(1.0 / 1307674368000.0));          // 1/15!
+    ps = fma(ps, r2, scoef(-1.0 / 6227020800.0));            // -1/13!
+    ps = fma(ps, r2, scoef(1.0 / 39916800.0));               // 1/11!
+    ps = fma(ps, r2, scoef(-1.0 / 362880.0));                // -1/9!
+    ps = fma(ps, r2, scoef(1.0 / 5040.0));                   // 1/7!
+    ps = fma(ps, r2, scoef(-1.0 / 120.0));                   // -1/5!
+    ps = fma(ps, r2, scoef(1.0 / 6.0));                      // 1/3! (sign applied below)
     const double sr = fma(-r * r2, ps, r);
     double pc = 1.0 / 6402373705728000.0;             // 1/18!
-    pc = fma(pc, r2, -1.0 / 20922789888000.0);        // -1/16!
-    pc = fma(pc, r2, 1.0 / 87178291200.0);            // 1/14!
-    pc = fma(pc, r2, -1.0 / 479001600.0);             // -1/12!
-    pc = fma(pc, r2, 1.0 / 3628800.0);                // 1/10!
-    pc = fma(pc, r2, -1.0 / 40320.0);                 // -1/8!
-    pc = fma(pc, r2, 1.0 / 720.0);                    // 1/6!
-    pc = fma(pc, r2, -1.0 / 24.0);                    // -1/4!
-    pc = fma(pc, r2, 0.5);                            // 1/2! (sign applied below)
+    pc = fma(pc, r2, scoef(-1.0 / 20922789888000.0));        // -1/16!
+    pc = fma(pc, r2, scoef(1.0 / 87178291200.0));            // 1/14!
+    pc = fma(pc, r2, scoef(-1.0 / 479001600.0));             // -1/12!
+    pc = fma(pc, r2, scoef(1.0 / 3628800.0));                // 1/10!
+    pc = fma(pc, r2, scoef(-1.0 / 40320.0));                 // -1/8!
+    pc = fma(pc, r2, scoef(1.0 / 720.0));                    // 1/6!
+    pc = fma(pc, r2, scoef(-1.0 / 24.0));                    // -1/4!
+    pc = fma(pc, r2, 0.5);                                   // 1/2! (sign applied below)
     const double cr = fma(-r2, pc, 1.0);
     const int k = ((int)q) & 3;
     const double ss = (k & 1) ? cr : sr;

@@ -124,7 +124,7 @@ __device__ inline void segment_sums(const cd* rs, const double* __restrict__ fre
     for (int j = lane + 1; j < jend; j += 64) {
         const double fj = freqs[j], fprev = freqs[j - 1];
         if (fprev == 0.0) continue;
-        const double inv = 1.0 / (fj - fprev);
+        const double inv = rcp_pos(fj - fprev);  // nodes increase
         const double fm = (fj + fprev) / 2.0;
         for (int c = 0; c < 3; c++) {
             const cd cur = rs[c * K + j], prev = rs[c * K + j - 1];

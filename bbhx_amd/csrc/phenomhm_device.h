@@ -236,7 +236,8 @@ __device__ inline S hm_amp_factor(const PhmRecT<S>& r, nd<S> Mf_wf, nd<S> Mf_amp
     if (h1 == 0.0) return 0.0;
     const S v2 = v1 * r.pn_v2ratio;
     const S v3 = cbrt_pos(r.pn_vscale * Mf_amp);
-    return h1 * pn_habs(r, v3) / pn_habs(r, v2);
+    // |H(v2)| > 0 wherever |H(v1)| > 0 (the same factors of eta and delta vanish)
+    return h1 * pn_habs(r, v3) * rcp_pos(pn_habs(r, v2));
 }
 
 // ---------------------------------------------------------------- phase
