@@ -35,6 +35,7 @@
 #include "bbhx_hip.h"
 #include "common.h"
 #include "fused_device.h"
+#include "grad_device.h"
 
 using namespace bbhx;
 using namespace bbhx::fused;
@@ -63,6 +64,10 @@ struct bbhx_direct_plan {
     int* perm = nullptr;          // (max_items,) execution order of the work items
     int* cnt = nullptr;           // (max_chunks + 1,) items per absolute chunk, then offsets
     int* next = nullptr;          // (1,) next entry of perm for the dense-band kernel
+    // gradient buffers (bbhx_hip_direct_plan_enable_grad)
+    double* Yadj = nullptr;              // (S, M, 9, 4) adjoint records
+    bbhx::gradk::GradRec* grecs = nullptr;  // (B, M) records with tangents
+    double* gpart = nullptr;             // (B, M, 11) per-mode gradients
     int max_chunks = 0;           // absolute chunks of min_chunk bins
 };
 
@@ -515,6 +520,93 @@ __device__ inline void mode_per_lane(cd h[3], const SegRec* __restrict__ rm, int
 
 constexpr int RB = 2;  // iterations of 64 dense bins per block of a wavefront
 
+// ---------------------------------------------------------------- adjoint helpers
+
+// Adjoint record of one (knot, mode): for each of the nine quantities, the
+// adjoints of its segment's (y, c1, c2, c3); after k_dirg_spline_adj, slot 0
+// of each quantity holds the adjoint of the quantity's value at the knot.
+constexpr int NA = NQ * 4;  // 36 doubles
+
+// quantity of adjoint j (0..7): A, phase, then the six transfer-function parts
+// (t_f carries none: the observation window is held fixed)
+__device__ inline int adj_q(int j) { return j < 2 ? j : j + 1; }
+
+// d logL with respect to the 32 segment coefficients (A, phase, T1..T3 re/im;
+// y, c1, c2, c3 each) of one mode at one dense bin, given
+// Lam_c = 4 w_c (d~_c - h~_c) (d logL = Re sum_c conj(Lam_c) dh_c with
+// h_c = sum_m conj(T_c A e^{i phase}); Likelihood.cu:357-375 differentiated).
+template <class Q>
+__device__ inline void mode_adjoint(double cb[32], const Q& q, double x, double ts, double te, const cd Lam[3])
+{
+    const double x2 = x * x, x3 = x * x2;
+    auto ev = [&](int k) {
+        const Cubic c = cubic_of(q, k);
+        return c.y + c.c1 * x + c.c2 * x2 + c.c3 * x3;
+    };
+    const double tf = ev(2);
+    if (!((tf >= ts) && ((tf <= te) || (te <= 0.0)))) return;
+    const double A = ev(0);
+    const cd E = cexpi_red(ev(1));
+    const cd AE = A * E;
+    double a[8];
+    cd P = mk(0.0, 0.0);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const cd T = mk(ev(3 + 2 * c), ev(4 + 2 * c));
+        P += Lam[c] * T;
+        const cd LAE = Lam[c] * AE;
+        a[2 + 2 * c] = LAE.re;
+        a[3 + 2 * c] = -LAE.im;
+    }
+    const cd PE = P * E;
+    a[0] = PE.re;
+    a[1] = -A * PE.im;
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        cb[4 * j] = a[j];
+        cb[4 * j + 1] = a[j] * x;
+        cb[4 * j + 2] = a[j] * x2;
+        cb[4 * j + 3] = a[j] * x3;
+    }
+}
+
+// Sum of v[0..32) over the 64 lanes of a wavefront by recursive halving (each
+// step exchanges half of the remaining values with the partner lane): lane l
+// returns the total of value l >> 1 (31 exchanges instead of 32 x 6).
+__device__ inline double reduce32(double v[32], int lane)
+{
+#pragma unroll
+    for (int h = 16, bit = 32; h >= 1; h >>= 1, bit >>= 1) {
+        const bool up = (lane & bit) != 0;
+#pragma unroll
+        for (int j = 0; j < h; j++) {
+            const double send = up ? v[j] : v[j + h];
+            const double keep = up ? v[j + h] : v[j];
+            v[j] = keep + __shfl_xor(send, bit, 64);
+        }
+    }
+    return v[0] + __shfl_xor(v[0], 1, 64);
+}
+
+// adjoint of one lane's bin for one mode, segment searched by the lane (as
+// mode_per_lane), added to the segment's adjoint record
+__device__ inline void mode_adjoint_per_lane(double* __restrict__ Ym, const SegRec* __restrict__ rm, int M, int L,
+                                             double f, int i, double ts, double te, const cd Lam[3])
+{
+    double2 bnd = *reinterpret_cast<const double2*>(rm + (long)i * M);
+    while (f < bnd.x && i > 0) bnd = *reinterpret_cast<const double2*>(rm + (long)--i * M);
+    while (f >= bnd.y && i < L - 2) bnd = *reinterpret_cast<const double2*>(rm + (long)++i * M);
+    if (!(bnd.x <= f && f < bnd.y)) return;
+    double cb[32];
+#pragma unroll
+    for (int j = 0; j < 32; j++) cb[j] = 0.0;
+    mode_adjoint(cb, *(rm + (long)i * M), f - bnd.x, ts, te, Lam);
+    double* y = Ym + (long)i * M * NA;
+#pragma unroll
+    for (int j = 0; j < 32; j++)
+        if (cb[j] != 0.0) unsafeAtomicAdd(y + adj_q(j >> 2) * 4 + (j & 3), cb[j]);
+}
+
 // Persistent dense-band kernel: partial[it] = (Re, Im of sum conj(d~) h~, sum |h~|^2, 0)
 // over the dense bins of work item it, where h~ = w h (Likelihood.cu:357-375).
 //
@@ -530,7 +622,14 @@ constexpr int RB = 2;  // iterations of 64 dense bins per block of a wavefront
 // FP64 ones against the stored knots, so the segment of every bin is the one
 // searchsorted gives. Where an iteration spans more than three segments (low
 // frequencies, short segments) each lane searches its own (mode_per_lane).
-__global__ __launch_bounds__(256) void k_dir_like(double* __restrict__ partial, const int* __restrict__ band,
+//
+// ADJ: also the adjoint pass of the direct log-likelihood's gradient: after the
+// mode sum of a block, Lam = 4 w (d~ - h~) per bin and channel, then the modes
+// again, each bin's coefficient adjoints (mode_adjoint) summed over the lanes
+// that share a segment (reduce32) and added to the segment's adjoint record.
+template <bool ADJ>
+__global__ __launch_bounds__(256) void k_dir_like_t(double* __restrict__ partial, double* __restrict__ Yadj,
+                                                    const int* __restrict__ band,
                                                   const int* __restrict__ total, const int* __restrict__ perm,
                                                   int* __restrict__ next, const SegRec* __restrict__ R,
                                                   const double* __restrict__ grid_lo,
@@ -626,11 +725,65 @@ __global__ __launch_bounds__(256) void k_dir_like(double* __restrict__ partial, 
                 if (k < kw1) {
 #pragma unroll
                     for (int ch = 0; ch < 3; ch++) {
-                        const cd ht = h[r][ch] * wts[(long)ch * N + k];
-                        const cd dh = conj(dch[(long)ch * N + k]) * ht;
+                        const double w = wts[(long)ch * N + k];
+                        const cd ht = h[r][ch] * w;
+                        const cd d = dch[(long)ch * N + k];
+                        const cd dh = conj(d) * ht;
                         a0 += dh.re;
                         a1 += dh.im;
                         a2 += ht.re * ht.re + ht.im * ht.im;
+                        if (ADJ) h[r][ch] = (4.0 * w) * (d - ht);  // Lam
+                    }
+                }
+            }
+            if (ADJ) {
+                for (int m = 0; m < M; m++) {
+                    const double gis = grid_istep[b * M + m];
+                    if (!(gis > 0.0)) continue;
+                    const double glo = grid_lo[b * M + m];
+                    const SegRec* rm = R + base * M + m;
+                    double* Ym = Yadj + (base * M + m) * (long)NA;
+                    int cur = -1;
+#pragma unroll
+                    for (int r = 0; r < RB; r++) {
+                        bool pending = kb + 64 * r + lane < kw1;
+                        const unsigned long long pm = __ballot(pending);
+                        if (pm == 0) break;
+                        const int est = (int)fmin(fmax((lf[r] - glo) * gis, 0.0), (double)(L - 2));
+                        const int e_first = __builtin_amdgcn_readlane(est, __builtin_ctzll(pm));
+                        const int e_last = __builtin_amdgcn_readlane(est, 63 - __builtin_clzll(pm));
+                        if (e_last - e_first > 2) {
+                            if (pending) mode_adjoint_per_lane(Ym, rm, M, L, f[r], est, ts, te, h[r]);
+                            cur = -1;
+                            continue;
+                        }
+                        if (cur < 0) cur = e_first;
+                        for (int guard = 2 * L + 8; guard > 0; guard--) {
+                            if (__ballot(pending) == 0) break;
+                            const SegRec* rc = rm + (long)cur * M;
+                            const double xlo = rc->x_lo, xhi = rc->x_hi;
+                            const bool below = pending && f[r] < xlo;
+                            if (__ballot(below) != 0) {
+                                if (cur == 0) pending = pending && !below;
+                                else cur--;
+                                continue;
+                            }
+                            const bool in = pending && f[r] < xhi;
+                            double cb[32];
+#pragma unroll
+                            for (int j = 0; j < 32; j++) cb[j] = 0.0;
+                            if (in) mode_adjoint(cb, *rc, f[r] - xlo, ts, te, h[r]);
+                            const double tot = reduce32(cb, lane);
+                            if ((lane & 1) == 0) {
+                                const int j = lane >> 1;
+                                if (tot != 0.0) unsafeAtomicAdd(Ym + (long)cur * M * NA + adj_q(j >> 2) * 4 + (j & 3), tot);
+                            }
+                            pending = pending && !in;
+                            if (__ballot(pending) != 0) {
+                                if (cur == L - 2) pending = false;
+                                else cur++;
+                            }
+                        }
                     }
                 }
             }
@@ -680,6 +833,176 @@ __global__ void k_dir_reduce(double* __restrict__ d_h, double* __restrict__ h_h,
     h_h[2 * b + 1] = 0.0;
 }
 
+// ---------------------------------------------------------------- gradient kernels
+
+__global__ void k_dirg_zero(double* __restrict__ y, long n)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) y[i] = 0.0;
+}
+
+// records with tangents, one lane per (binary, mode)
+__global__ __launch_bounds__(64) void k_dirg_setup(gradk::GradRec* __restrict__ grecs,
+                                                   const double* __restrict__ params, int B, ModeList modes, int M,
+                                                   const double* __restrict__ qnm, int table_len, double dspin)
+{
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= B * M) return;
+    const int b = idx / M, mi = idx % M;
+    gradk::grad_setup_one(grecs[idx], params, B, b, modes.ell[mi], modes.mm[mi], qnm, table_len, dspin);
+}
+
+// Adjoint of the not-a-knot spline construction (the system of k_dir_spline):
+// segment-coefficient adjoints -> adjoints of the knot values. One wavefront
+// per binary, one lane per (mode, quantity) system (t_f excluded). In the
+// record of knot i, quantity q: slots 0..3 hold the coefficient adjoints of
+// segment i on entry; the passes reuse them (0: value adjoint, 1: slope
+// adjoint then z, 2, 3: the transposed solve's c', d'); slot 0 holds the
+// knot-value adjoint on exit.
+__global__ __launch_bounds__(64) void k_dirg_spline_adj(double* __restrict__ Yadj, const double* __restrict__ X,
+                                                        const long* __restrict__ off, const int* __restrict__ len,
+                                                        int M)
+{
+    const int b = blockIdx.x;
+    const int s = threadIdx.x;
+    if (s >= M * 8) return;
+    const int m = s / 8, q = adj_q(s % 8);
+    const int n = len[b];
+    const long base = off[b];
+    auto xi = [&](int i) { return X[(base + i) * M + m]; };
+    auto Y = [&](int i) { return Yadj + ((base + i) * M + m) * (long)NA + q * 4; };
+    // pass 1: segment coefficients -> value and slope adjoints (carry into knot i + 1)
+    double cy = 0.0, cs = 0.0;
+    for (int i = 0; i < n - 1; i++) {
+        double* y = Y(i);
+        const double c0 = y[0], c1 = y[1], c2 = y[2], c3 = y[3];
+        const double dx = xi(i + 1) - xi(i);
+        const double tb = -c2 + c3 / dx;
+        const double slb = c2 / dx - 2.0 * tb / dx;
+        y[0] = cy + c0 - slb / dx;
+        y[1] = cs + c1 - c2 / dx + tb / dx;
+        cy = slb / dx;
+        cs = tb / dx;
+    }
+    {
+        double* y = Y(n - 1);
+        y[0] = cy;
+        y[1] = cs;
+    }
+    // pass 2: A^T z = slope adjoints (A: the tridiagonal matrix of the slopes)
+    auto Aij = [&](int i, int j) -> double {
+        if (i == 0) return j == 0 ? xi(2) - xi(1) : (j == 1 ? xi(2) - xi(0) : 0.0);
+        if (i == n - 1) return j == n - 2 ? xi(n - 1) - xi(n - 3) : (j == n - 1 ? xi(n - 2) - xi(n - 3) : 0.0);
+        if (j == i - 1) return xi(i + 1) - xi(i);
+        if (j == i) return 2.0 * (xi(i + 1) - xi(i - 1));
+        if (j == i + 1) return xi(i) - xi(i - 1);
+        return 0.0;
+    };
+    double cprev = 0.0, dprev = 0.0;
+    for (int i = 0; i < n; i++) {
+        double* y = Y(i);
+        const double bdiag = Aij(i, i);
+        double den, d;
+        if (i == 0) {
+            den = bdiag;
+            d = y[1] / den;
+        } else {
+            const double a = Aij(i - 1, i);
+            den = bdiag - a * cprev;
+            d = (y[1] - a * dprev) / den;
+        }
+        const double c = i < n - 1 ? Aij(i + 1, i) / den : 0.0;
+        y[2] = c;
+        y[3] = d;
+        cprev = c;
+        dprev = d;
+    }
+    double znext = Y(n - 1)[3];
+    Y(n - 1)[1] = znext;
+    for (int i = n - 2; i >= 0; i--) {
+        double* y = Y(i);
+        znext = y[3] - y[2] * znext;
+        y[1] = znext;
+    }
+    // pass 3: value adjoints += R^T z (R: the right-hand side's dependence on the values)
+    {
+        const double dx1 = xi(1) - xi(0), dx2 = xi(2) - xi(1), dd = xi(2) - xi(0);
+        const double a1 = (dx1 + 2.0 * dd) * dx2 / (dd * dx1), a2 = dx1 * dx1 / (dd * dx2);
+        const double z0 = Y(0)[1];
+        Y(0)[0] += -a1 * z0;
+        Y(1)[0] += (a1 - a2) * z0;
+        Y(2)[0] += a2 * z0;
+    }
+    for (int i = 1; i < n - 1; i++) {
+        const double e1 = xi(i) - xi(i - 1), e2 = xi(i + 1) - xi(i);
+        const double b1 = 3.0 * e2 / e1, b2 = 3.0 * e1 / e2;
+        const double zi = Y(i)[1];
+        Y(i - 1)[0] += -b1 * zi;
+        Y(i)[0] += (b1 - b2) * zi;
+        Y(i + 1)[0] += b2 * zi;
+    }
+    {
+        const double e1 = xi(n - 2) - xi(n - 3), e2 = xi(n - 1) - xi(n - 2), ed = xi(n - 1) - xi(n - 3);
+        const double c1 = e2 * e2 / (ed * e1), c2 = (2.0 * ed + e2) * e1 / (ed * e2);
+        const double zn = Y(n - 1)[1];
+        Y(n - 3)[0] += -c1 * zn;
+        Y(n - 2)[0] += (c1 - c2) * zn;
+        Y(n - 1)[0] += c2 * zn;
+    }
+}
+
+// Knot-level contraction: one wavefront per (binary, mode), lane per knot; the
+// knot-value adjoints of A, phase and T weight the waveform's and response's
+// tangents at the knot's frequency (grad_point_core, no window rule: the
+// window acts on the dense bins). partial[(b M + m) 11 + p].
+__global__ __launch_bounds__(64) void k_dirg_knots(double* __restrict__ gpart, const gradk::GradRec* __restrict__ grecs,
+                                                   const double* __restrict__ Yadj, const double* __restrict__ X,
+                                                   const long* __restrict__ off, const int* __restrict__ len,
+                                                   const double* __restrict__ params, int B, int M,
+                                                   resp::OrbitTab orb, int TDItag, int rescaled, int tdi2,
+                                                   double initial_t_val)
+{
+    const int bm = blockIdx.x;
+    const int b = bm / M, m = bm % M;
+    Window win;
+    win.t_ref = params[P_TREF * B + b];
+    win.ts = 0.0;
+    win.te = 0.0;
+    win.dt_shift = win.t_ref - initial_t_val;
+    const double dist = params[P_DIST * B + b];
+    const gradk::GradRec& G = grecs[bm];
+    const int n = len[b];
+    const long base = off[b];
+    double acc[gradk::NGRAD];
+    for (int p = 0; p < gradk::NGRAD; p++) acc[p] = 0.0;
+    for (int i = threadIdx.x; i < n; i += 64) {
+        const long rec = (base + i) * M + m;
+        const double* y = Yadj + rec * NA;
+        const double aA = y[0 * 4], aPh = y[1 * 4];
+        cd Wt[3];
+        for (int c = 0; c < 3; c++) Wt[c] = mk(y[(3 + 2 * c) * 4], -y[(4 + 2 * c) * 4]);
+        auto weights = [&](const cd*, cd, double, cd W[3], double& a_A, double& a_Ph) {
+            for (int c = 0; c < 3; c++) W[c] = Wt[c];
+            a_A = aA;
+            a_Ph = aPh;
+        };
+        gradk::grad_point_core(G, orb, X[rec], win, false, weights, TDItag, rescaled, tdi2, dist, acc);
+    }
+    for (int p = 0; p < gradk::NGRAD; p++) acc[p] = wsum(acc[p]);
+    if (threadIdx.x == 0)
+        for (int p = 0; p < gradk::NGRAD; p++) gpart[(long)bm * gradk::NGRAD + p] = acc[p];
+}
+
+// grad[p B + b] = sum over modes, in mode order
+__global__ void k_dirg_reduce(double* __restrict__ grad, const double* __restrict__ gpart, int B, int M)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= B * gradk::NGRAD) return;
+    const int p = t / B, b = t % B;
+    double s = 0.0;
+    for (int m = 0; m < M; m++) s += gpart[((long)b * M + m) * gradk::NGRAD + p];
+    grad[(long)p * B + b] = s;
+}
+
 template <class T>
 T* dalloc(size_t n, bool& ok)
 {
@@ -694,7 +1017,7 @@ T* dalloc(size_t n, bool& ok)
 void plan_free(bbhx_direct_plan* p)
 {
     void* bufs[] = {p->off, p->len, p->recs, p->X, p->Cf, p->grid_lo, p->grid_istep, p->band,
-                    p->total, p->partial, p->perm, p->cnt, p->next};
+                    p->total, p->partial, p->perm, p->cnt, p->next, p->Yadj, p->grecs, p->gpart};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
 }
@@ -775,8 +1098,10 @@ extern "C" long bbhx_hip_direct_plan_bytes(const bbhx_direct_plan* p)
 {
     if (!p) return -1;
     const long SM = p->S * p->M;
-    return SM * 8 * (1 + RS) + (long)p->B * p->M * (sizeof(FusedRec) + 16) + p->max_items * 36 + p->B * 24 +
-           4L * (p->max_chunks + 1);
+    long bytes = SM * 8 * (1 + RS) + (long)p->B * p->M * (sizeof(FusedRec) + 16) + p->max_items * 36 + p->B * 24 +
+                 4L * (p->max_chunks + 1);
+    if (p->Yadj) bytes += SM * 8 * NA + (long)p->B * p->M * (sizeof(bbhx::gradk::GradRec) + 8 * bbhx::gradk::NGRAD);
+    return bytes;
 }
 
 extern "C" int bbhx_hip_direct_plan_band(const bbhx_direct_plan* p, int* band_host)
@@ -793,13 +1118,16 @@ extern "C" int bbhx_hip_direct_plan_band(const bbhx_direct_plan* p, int* band_ho
     return OK;
 }
 
-extern "C" int bbhx_hip_direct_fused(bbhx_direct_plan* p, double* d_h, double* h_h, int* flags, const double* params,
-                                     const double* t_start, const double* t_end, const int* ells_host,
-                                     const int* mms_host, int numModes, const double* dataFreqs,
-                                     const double* dataChannels, const double* noiseFactors, const double* qnm,
-                                     int table_len, double dspin, double initial_t_val, double mf_min, double mf_max,
-                                     const bbhx_response* orbits, int TDItag, int rescaled, int tdi2, int chunk,
-                                     int block, int grid, void* stream)
+namespace {
+
+// The launch sequence of the fused direct likelihood; with grad != nullptr
+// also its gradient (the plan's gradient buffers must exist).
+int direct_launch(bbhx_direct_plan* p, double* d_h, double* h_h, double* grad, int* flags, const double* params,
+                  const double* t_start, const double* t_end, const int* ells_host, const int* mms_host,
+                  int numModes, const double* dataFreqs, const double* dataChannels, const double* noiseFactors,
+                  const double* qnm, int table_len, double dspin, double initial_t_val, double mf_min, double mf_max,
+                  const bbhx_response* orbits, int TDItag, int rescaled, int tdi2, int chunk, int block, int grid,
+                  void* stream)
 {
     if (!p) return ERR_ARGS;
     if (chunk < p->min_chunk || (block != 64 && block != 128 && block != 256) || grid < 1) return ERR_ARGS;
@@ -814,6 +1142,7 @@ extern "C" int bbhx_hip_direct_fused(bbhx_direct_plan* p, double* d_h, double* h
     if (!d_h || !h_h || !flags || !params || !t_start || !t_end || !dataFreqs || !dataChannels || !noiseFactors ||
         !qnm || table_len < 2 || !(mf_min > 0.0) || !(mf_max > mf_min))
         return ERR_ARGS;
+    if (grad && !p->Yadj) return ERR_ARGS;  // bbhx_hip_direct_plan_enable_grad first
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     const int B = p->B, M = p->M;
     hipEvent_t ev[3];
@@ -835,13 +1164,80 @@ extern "C" int bbhx_hip_direct_fused(bbhx_direct_plan* p, double* d_h, double* h
     hipLaunchKernelGGL(k_dir_cnt, dim3(item_blocks), dim3(256), 0, st, p->cnt, p->band, p->total, B, chunk);
     hipLaunchKernelGGL(k_dir_cnt_scan, dim3(1), dim3(1024), 0, st, p->cnt, nchunks);
     hipLaunchKernelGGL(k_dir_perm, dim3(item_blocks), dim3(256), 0, st, p->perm, p->cnt, p->band, p->total, B, chunk);
+    if (grad) {
+        hipLaunchKernelGGL(k_dirg_zero, dim3(4096), dim3(256), 0, st, p->Yadj, p->S * (long)M * NA);
+        hipLaunchKernelGGL(k_dirg_setup, dim3(ceil_div((long)B * M, 64)), dim3(64), 0, st, p->grecs, params, B, ml, M,
+                           qnm, table_len, dspin);
+    }
     if (timed) (void)hipEventRecord(ev[1], st);
-    hipLaunchKernelGGL(k_dir_like, dim3(grid), dim3(block), 0, st, p->partial, p->band, p->total, p->perm, p->next,
-                       reinterpret_cast<const SegRec*>(p->Cf), p->grid_lo, p->grid_istep, p->off, p->len, t_start,
-                       t_end, dataFreqs,
-                       reinterpret_cast<const cd*>(dataChannels), noiseFactors, p->N, B, M, chunk);
+    if (grad)
+        hipLaunchKernelGGL(k_dir_like_t<true>, dim3(grid), dim3(block), 0, st, p->partial, p->Yadj, p->band, p->total,
+                           p->perm, p->next, reinterpret_cast<const SegRec*>(p->Cf), p->grid_lo, p->grid_istep,
+                           p->off, p->len, t_start, t_end, dataFreqs, reinterpret_cast<const cd*>(dataChannels),
+                           noiseFactors, p->N, B, M, chunk);
+    else
+        hipLaunchKernelGGL(k_dir_like_t<false>, dim3(grid), dim3(block), 0, st, p->partial, nullptr, p->band,
+                           p->total, p->perm, p->next, reinterpret_cast<const SegRec*>(p->Cf), p->grid_lo,
+                           p->grid_istep, p->off, p->len, t_start, t_end, dataFreqs,
+                           reinterpret_cast<const cd*>(dataChannels), noiseFactors, p->N, B, M, chunk);
     hipLaunchKernelGGL(k_dir_reduce, dim3(ceil_div(B, 256)), dim3(256), 0, st, d_h, h_h, p->partial, p->band,
                        p->total, B);
+    if (grad) {
+        hipLaunchKernelGGL(k_dirg_spline_adj, dim3(B), dim3(64), 0, st, p->Yadj, p->X, p->off, p->len, M);
+        hipLaunchKernelGGL(k_dirg_knots, dim3(B * M), dim3(64), 0, st, p->gpart, p->grecs, p->Yadj, p->X, p->off,
+                           p->len, params, B, M, orb, TDItag, rescaled, tdi2, initial_t_val);
+        hipLaunchKernelGGL(k_dirg_reduce, dim3(ceil_div((long)B * gradk::NGRAD, 256)), dim3(256), 0, st, grad,
+                           p->gpart, B, M);
+    }
     if (timed) (void)hipEventRecord(ev[2], st);
     return hipGetLastError() == hipSuccess ? OK : ERR_LAUNCH;
+}
+
+}  // namespace
+
+extern "C" int bbhx_hip_direct_fused(bbhx_direct_plan* p, double* d_h, double* h_h, int* flags, const double* params,
+                                     const double* t_start, const double* t_end, const int* ells_host,
+                                     const int* mms_host, int numModes, const double* dataFreqs,
+                                     const double* dataChannels, const double* noiseFactors, const double* qnm,
+                                     int table_len, double dspin, double initial_t_val, double mf_min, double mf_max,
+                                     const bbhx_response* orbits, int TDItag, int rescaled, int tdi2, int chunk,
+                                     int block, int grid, void* stream)
+{
+    return direct_launch(p, d_h, h_h, nullptr, flags, params, t_start, t_end, ells_host, mms_host, numModes,
+                         dataFreqs, dataChannels, noiseFactors, qnm, table_len, dspin, initial_t_val, mf_min, mf_max,
+                         orbits, TDItag, rescaled, tdi2, chunk, block, grid, stream);
+}
+
+extern "C" int bbhx_hip_direct_plan_enable_grad(bbhx_direct_plan* p)
+{
+    if (!p) return ERR_ARGS;
+    if (p->Yadj) return OK;
+    bool ok = true;
+    p->Yadj = dalloc<double>((size_t)p->S * p->M * NA, ok);
+    p->grecs = dalloc<gradk::GradRec>((size_t)p->B * p->M, ok);
+    p->gpart = dalloc<double>((size_t)p->B * p->M * gradk::NGRAD, ok);
+    if (!ok) {
+        for (void* b : {(void*)p->Yadj, (void*)p->grecs, (void*)p->gpart})
+            if (b) (void)hipFree(b);
+        p->Yadj = nullptr;
+        p->grecs = nullptr;
+        p->gpart = nullptr;
+        return ERR_ALLOC;
+    }
+    return OK;
+}
+
+extern "C" int bbhx_hip_direct_fused_grad(bbhx_direct_plan* p, double* d_h, double* h_h, double* grad, int* flags,
+                                          const double* params, const double* t_start, const double* t_end,
+                                          const int* ells_host, const int* mms_host, int numModes,
+                                          const double* dataFreqs, const double* dataChannels,
+                                          const double* noiseFactors, const double* qnm, int table_len, double dspin,
+                                          double initial_t_val, double mf_min, double mf_max,
+                                          const bbhx_response* orbits, int TDItag, int rescaled, int tdi2, int chunk,
+                                          int block, int grid, void* stream)
+{
+    if (!grad) return ERR_ARGS;
+    return direct_launch(p, d_h, h_h, grad, flags, params, t_start, t_end, ells_host, mms_host, numModes, dataFreqs,
+                         dataChannels, noiseFactors, qnm, table_len, dspin, initial_t_val, mf_min, mf_max, orbits,
+                         TDItag, rescaled, tdi2, chunk, block, grid, stream);
 }

@@ -1,0 +1,329 @@
+// Derivatives of the waveform and response with respect to the 11 source
+// parameters (ln M, q, chi1, chi2, d, phi_ref, cos inc, lam, sin beta, psi,
+// t_ref), shared by the adjoint kernels of the heterodyned and the direct
+// log-likelihood (no counterpart in the reference, which has no derivatives):
+// records with tangents per (template, mode) and the per-point contraction of
+// the waveform's and response's tangents with the likelihood's adjoint weights.
+#pragma once
+
+#include "common.h"
+#include "dual.h"
+#include "fused_device.h"
+
+namespace bbhx {
+namespace gradk {
+
+constexpr int NI = 4;  // intrinsic tangent directions: ln M, q, chi1, chi2
+using D4 = Dual<NI>;
+constexpr int NGRAD = 11;
+enum { G_LNM, G_Q, G_CHI1, G_CHI2, G_DIST, G_PHI, G_COSI, G_LAM, G_SINB, G_PSI, G_TREF };
+
+struct GradResp {
+    double u[3], v[3], k[3];
+    double dk[2][3], du[2][3], dv[2][3];  // d/d lam, d/d beta
+    cd Yp, Yc, dYp, dYc;                  // dY/d(cos inc)
+    double m;                             // azimuthal number
+    double inv_cosb;                      // d beta / d(sin beta)
+};
+
+struct GradRec {
+    phm::PhmRecT<D4> w;
+    GradResp s;
+};
+
+__device__ inline void grad_resp_setup(GradResp& r, int l, int m, double phi, double inc, double lam, double beta,
+                                       double psi)
+{
+    const double cl = cos(lam), sl = sin(lam), cb = cos(beta), sb = sin(beta), cp = cos(psi), sp = sin(psi);
+    r.k[0] = -cb * cl;
+    r.k[1] = -cb * sl;
+    r.k[2] = -sb;
+    r.u[0] = cp * sl - cl * sb * sp;
+    r.u[1] = -cl * cp - sb * sl * sp;
+    r.u[2] = cb * sp;
+    r.v[0] = -cl * cp * sb - sl * sp;
+    r.v[1] = -cp * sb * sl + cl * sp;
+    r.v[2] = cb * cp;
+    // d/d lam
+    r.dk[0][0] = cb * sl;
+    r.dk[0][1] = -cb * cl;
+    r.dk[0][2] = 0.0;
+    r.du[0][0] = cp * cl + sl * sb * sp;
+    r.du[0][1] = sl * cp - sb * cl * sp;
+    r.du[0][2] = 0.0;
+    r.dv[0][0] = sl * cp * sb - cl * sp;
+    r.dv[0][1] = -cp * sb * cl - sl * sp;
+    r.dv[0][2] = 0.0;
+    // d/d beta
+    r.dk[1][0] = sb * cl;
+    r.dk[1][1] = sb * sl;
+    r.dk[1][2] = -cb;
+    r.du[1][0] = -cl * cb * sp;
+    r.du[1][1] = -cb * sl * sp;
+    r.du[1][2] = -sb * sp;
+    r.dv[1][0] = -cl * cp * cb;
+    r.dv[1][1] = -cp * cb * sl;
+    r.dv[1][2] = -sb * cp;
+    r.inv_cosb = 1.0 / cb;
+    r.m = m;
+    // Y+ and Yx as resp_setup forms them, and their derivatives in inc
+    const Dual<1> th = Dual<1>::seed(inc, 0);
+    const Dual<1> fp = resp::ylm_fac(l, m, th), fm = resp::ylm_fac(l, -m, th);
+    const double sgn = (l & 1) ? -1.0 : 1.0;
+    const cd e = cexpi(m * phi);
+    const double a = 0.5 * (fp.v + sgn * fm.v), c = 0.5 * (fp.v - sgn * fm.v);
+    const double da = 0.5 * (fp.d[0] + sgn * fm.d[0]), dc = 0.5 * (fp.d[0] - sgn * fm.d[0]);
+    r.Yp = a * e;
+    r.Yc = mul_i(c * e);
+    // d/d(cos inc) = -(1 / sin inc) d/d inc
+    const double si = sin(inc);
+    const double dci = si != 0.0 ? -1.0 / si : 0.0;
+    r.dYp = (da * dci) * e;
+    r.dYc = mul_i((dc * dci) * e);
+}
+
+
+// The record with tangents of one (template, mode) (PhenomHMWaveform.cu set-up
+// code instantiated on Dual<4>, seeded in ln M, q, chi1, chi2) and the
+// response basis with its derivatives.
+__device__ inline void grad_setup_one(GradRec& g, const double* __restrict__ params, int B, int b, int l, int m,
+                                      const double* __restrict__ qnm, int table_len, double dspin)
+{
+    // masses from (ln M, q): m1 = M q / (1 + q), m2 = M / (1 + q)
+    const double m1 = params[fused::P_M1 * B + b], m2 = params[fused::P_M2 * B + b];
+    const double Mt = m1 + m2, q = m1 / m2;
+    const double dq = Mt / ((1.0 + q) * (1.0 + q));
+    D4 m1d(m1), m2d(m2);
+    m1d.d[0] = m1;
+    m1d.d[1] = dq;
+    m2d.d[0] = m2;
+    m2d.d[1] = -dq;
+    const D4 c1d = D4::seed(params[fused::P_CHI1 * B + b], 2), c2d = D4::seed(params[fused::P_CHI2 * B + b], 3);
+    D4 finmass, finspin, rdD, dmD, rd, dm;
+    phm::final_state(m1d, m2d, c1d, c2d, finmass, finspin);
+    fused::phenomd_ringdown(finmass, finspin, qnm, table_len, dspin, rdD, dmD);
+    phm::hm_ringdown(l, m, finmass, finspin, rd, dm);
+    phm::phm_setup(g.w, m1d * MSUN_SI, m2d * MSUN_SI, c1d, c2d, D4(params[fused::P_DIST * B + b]), l, m, rd, dm, rdD,
+                   dmD, false);
+    GradResp s;
+    grad_resp_setup(s, l, m, params[fused::P_PHI * B + b], params[fused::P_INC * B + b],
+                    params[fused::P_LAM * B + b], params[fused::P_BETA * B + b], params[fused::P_PSI * B + b]);
+    g.s = s;
+}
+
+__device__ inline double dsinc(double a, double cos_a, double sinc_a) { return a == 0.0 ? 0.0 : (cos_a - sinc_a) / a; }
+
+// Gradient contributions of the waveform and response of one (frequency, mode)
+// to acc[11], for a log-likelihood whose first-order change is
+//   d logL = a_A dA + a_phi d phi + Re sum_c W_c dT_c,
+// where A is the amplitude, phi the phase (PhenomHM phase + 2 pi f (t_ref - t0)
+// + the orbital Doppler phase) and T_c the TDI transfer functions at f and
+// t = t(f) + t_ref. weights(T, E, A, W, a_A, a_phi) supplies W, a_A, a_phi once
+// T, E = e^{i phi} and A are known. use_window: contribute only if t lies in
+// the observation window (the heterodyned form's rule for its nodes).
+template <class Weights>
+__device__ inline void grad_point_core(const GradRec& G, const resp::OrbitTab& o, double f, const fused::Window& win,
+                                       bool use_window, Weights weights, int TDItag, int rescaled, int tdi2,
+                                       double dist, double acc[NGRAD])
+{
+    D4 A, Ph, Tf;
+    phm::phm_eval(G.w, f, A, Ph, Tf);
+    const double t = Tf.v + win.t_ref;
+    if (use_window && !fused::in_window(win, t)) return;
+    const GradResp& R = G.s;
+
+    // orbit projections (values), their time derivatives, and the spacecraft
+    // projections' derivatives in lam, beta; the links' derivatives in lam,
+    // beta are formed later, link by link, to keep fewer values live
+    int w = int(floor(t * o.inv_dt));
+    w = w < 0 ? 0 : (w > o.N - 2 ? o.N - 2 : w);
+    const double a = (t - o.dt * w) * o.inv_dt;
+    const double* x0 = o.x + (long)w * 9;
+    const double* n0 = o.n + (long)w * 18;
+    const int li[3] = {o.li12, o.li23, o.li31};
+    double kp[3], dkp[3][3];
+    for (int i = 0; i < 3; i++) {
+        const double* lo = x0 + 3 * i;
+        const double* hi = lo + 9;
+        const double l0 = resp::dot3(R.k, lo), h0v = resp::dot3(R.k, hi);
+        kp[i] = fma(h0v - l0, a, l0);
+        dkp[0][i] = (h0v - l0) * o.inv_dt;
+        for (int q = 0; q < 2; q++) {
+            const double lq = resp::dot3(R.dk[q], lo), hq = resp::dot3(R.dk[q], hi);
+            dkp[1 + q][i] = fma(hq - lq, a, lq);
+        }
+    }
+    double kn[3], un[3], vn[3], dkn_t[3], dun_t[3], dvn_t[3];
+    for (int j = 0; j < 3; j++) {
+        const double* lo = n0 + 3 * li[j];
+        const double* hi = lo + 18;
+        double l0 = resp::dot3(R.k, lo), h0v = resp::dot3(R.k, hi);
+        kn[j] = fma(h0v - l0, a, l0);
+        dkn_t[j] = (h0v - l0) * o.inv_dt;
+        l0 = resp::dot3(R.u, lo);
+        h0v = resp::dot3(R.u, hi);
+        un[j] = fma(h0v - l0, a, l0);
+        dun_t[j] = (h0v - l0) * o.inv_dt;
+        l0 = resp::dot3(R.v, lo);
+        h0v = resp::dot3(R.v, hi);
+        vn[j] = fma(h0v - l0, a, l0);
+        dvn_t[j] = (h0v - l0) * o.inv_dt;
+    }
+
+    // orbital Doppler phase and its derivatives (Response.cu:278-296)
+    double sa, ca;
+    sincos_red(OMEGA0 * t, &sa, &ca);
+    const double c2f = 2. * PI / C_SI * f;
+    const double pRd = c2f * (R.k[0] * (AU_SI * ca) + R.k[1] * (AU_SI * sa));
+    const double pRd_t = c2f * AU_SI * OMEGA0 * (-R.k[0] * sa + R.k[1] * ca);
+    const double pRd_l = c2f * AU_SI * (R.dk[0][0] * ca + R.dk[0][1] * sa);
+    const double pRd_b = c2f * AU_SI * (R.dk[1][0] * ca + R.dk[1][1] * sa);
+    const double phase = Ph.v + ((f * win.dt_shift) * 2) * PI + pRd;
+
+    // single-link terms as resp_eval forms them
+    const double x = PI * f * o.L / C_SI;
+    double sx, cx;
+    sincos_red(x, &sx, &cx);
+    cd nHn[3];
+    double sm[3], sp[3], dsm[3], dsp[3];
+    for (int j = 0; j < 3; j++) {
+        nHn[j] = R.Yp * (un[j] * un[j] - vn[j] * vn[j]) + R.Yc * (2.0 * un[j] * vn[j]);
+        double s, c;
+        sincos_red(x * kn[j], &s, &c);
+        const double am = x * (1. - kn[j]), ap = x * (1. + kn[j]);
+        const double sin_m = sx * c - cx * s, sin_p = sx * c + cx * s;
+        const double cos_m = cx * c + sx * s, cos_p = cx * c - sx * s;
+        sm[j] = am == 0.0 ? 1.0 : sin_m / am;
+        sp[j] = ap == 0.0 ? 1.0 : sin_p / ap;
+        dsm[j] = dsinc(am, cos_m, sm[j]);
+        dsp[j] = dsinc(ap, cos_p, sp[j]);
+    }
+    // link L = 0: pair (23, 32) with e23; L = 1: (31, 13) with e31; L = 2: (12, 21) with e12
+    const int ka[3] = {1, 2, 0}, kb[3] = {2, 0, 1};
+    cd eL[3], gL[3];
+    for (int L = 0; L < 3; L++) {
+        eL[L] = cexpi_red(x * (1. + (kp[ka[L]] + kp[kb[L]]) * o.inv_L));
+        gL[L] = mul_i(x * nHn[L]);
+    }
+    // Gm = g sm e (G23, G31, G12), Gp = g sp e (G32, G13, G21)
+    cd Gm[3], Gp[3];
+    for (int L = 0; L < 3; L++) {
+        Gm[L] = (gL[L] * sm[L]) * eL[L];
+        Gp[L] = (gL[L] * sp[L]) * eL[L];
+    }
+    const cd &G23 = Gm[0], &G32 = Gp[0], &G31 = Gm[1], &G13 = Gp[1], &G12 = Gm[2], &G21 = Gp[2];
+
+    // TDI combination (Response.cu:197-261) and its adjoint weights beta
+    const double s2x = 2.0 * sx * cx, c2x = cx * cx - sx * sx;
+    const cd z = mk(c2x, s2x);
+    const cd one = mk(1.0, 0.0);
+    cd T[3];
+    cd bm[3], bp[3];  // beta of Gm[L], Gp[L]
+    const cd E1 = cexpi_red(phase);
+    const double Av = A.v;
+    cd W[3];
+    double aA = 0.0, aPh = 0.0;
+    if (TDItag == 1) {
+        const cd fac = rescaled ? one : mul_i((2. * s2x) * z);
+        T[0] = fac * (G21 + z * G12 - G31 - z * G13);
+        T[1] = fac * (G32 + z * G23 - G12 - z * G21);
+        T[2] = fac * (G13 + z * G31 - G23 - z * G32);
+        weights(T, E1, Av, W, aA, aPh);
+        const cd wX = W[0] * fac, wY = W[1] * fac, wZ = W[2] * fac;
+        bp[2] = wX - z * wY;    // G21
+        bm[2] = z * wX - wY;    // G12
+        bm[1] = z * wZ - wX;    // G31
+        bp[1] = wZ - z * wX;    // G13
+        bp[0] = wY - z * wZ;    // G32
+        bm[0] = z * wY - wZ;    // G23
+    } else {
+        cd fAE = one, fT = one;
+        if (!rescaled) {
+            fAE = mul_i((SQRT2 * s2x) * z);
+            const double c3x = c2x * cx - s2x * sx, s3x = s2x * cx + c2x * sx;
+            fT = (2. * SQRT2 * s2x * sx) * mk(c3x, s3x);
+        }
+        cd f2 = one;
+        if (tdi2) {
+            const double s4x = 2.0 * s2x * c2x, c4x = c2x * c2x - s2x * s2x;
+            f2 = mul_i((-2. * s4x) * mk(c4x, s4x));
+        }
+        const cd fa = 2.0 * (f2 * fAE), ft = 2.0 * (f2 * fT);
+        const cd Ac = 0.5 * ((one + z) * (G31 + G13) - G23 - z * G32 - G21 - z * G12);
+        const cd Ec = (0.5 * INVSQRT3) * ((one - z) * (G13 - G31) + (mk(2.0, 0.0) + z) * (G12 - G32) +
+                                         (one + 2. * z) * (G21 - G23));
+        const cd Tc = INVSQRT6 * (G21 - G12 + G32 - G23 + G13 - G31);
+        T[0] = fa * Ac;
+        T[1] = fa * Ec;
+        T[2] = ft * Tc;
+        weights(T, E1, Av, W, aA, aPh);
+        const cd wA = (W[0] * fa) * 0.5, wE = (W[1] * fa) * (0.5 * INVSQRT3), wT = (W[2] * ft) * INVSQRT6;
+        bm[1] = wA * (one + z) - wE * (one - z) - wT;        // G31
+        bp[1] = wA * (one + z) + wE * (one - z) + wT;        // G13
+        bm[0] = mk(0.0, 0.0) - wA - wE * (one + 2. * z) - wT;  // G23
+        bp[0] = mk(0.0, 0.0) - wA * z - wE * (mk(2.0, 0.0) + z) + wT;  // G32
+        bp[2] = mk(0.0, 0.0) - wA + wE * (one + 2. * z) + wT;  // G21
+        bm[2] = mk(0.0, 0.0) - wA * z + wE * (mk(2.0, 0.0) + z) - wT;  // G12
+    }
+
+    // Q_d = sum_ij beta_ij dG_ij for d = t, lam, beta (0..2), psi (3), cos inc (4)
+    cd Q[5] = {mk(0.0, 0.0), mk(0.0, 0.0), mk(0.0, 0.0), mk(0.0, 0.0), mk(0.0, 0.0)};
+    const cd ix = mk(0.0, x);
+    for (int L = 0; L < 3; L++) {
+        // this link's projections differentiated in t (0), lam (1), beta (2)
+        double dknL[3], dunL[3], dvnL[3];
+        dknL[0] = dkn_t[L];
+        dunL[0] = dun_t[L];
+        dvnL[0] = dvn_t[L];
+        {
+            const double* lo = n0 + 3 * li[L];
+            const double* hi = lo + 18;
+            for (int q = 0; q < 2; q++) {
+                double lq = resp::dot3(R.dk[q], lo), hq = resp::dot3(R.dk[q], hi);
+                dknL[1 + q] = fma(hq - lq, a, lq);
+                lq = resp::dot3(R.du[q], lo);
+                hq = resp::dot3(R.du[q], hi);
+                dunL[1 + q] = fma(hq - lq, a, lq);
+                lq = resp::dot3(R.dv[q], lo);
+                hq = resp::dot3(R.dv[q], hi);
+                dvnL[1 + q] = fma(hq - lq, a, lq);
+            }
+        }
+        const cd BL = bm[L] * sm[L] + bp[L] * sp[L];
+        const cd CL = bp[L] * dsp[L] - bm[L] * dsm[L];
+        const cd eB = eL[L] * BL;
+        const cd eg = eL[L] * gL[L];
+        const cd egC = eg * CL * x;
+        const cd egB = eg * BL * ix * o.inv_L;
+        const cd ixeB = ix * eB;
+        for (int d = 0; d < 3; d++) {
+            const double dU = 2.0 * (un[L] * dunL[d] - vn[L] * dvnL[d]);
+            const double dV = 2.0 * (dunL[d] * vn[L] + un[L] * dvnL[d]);
+            const cd dnHn = R.Yp * dU + R.Yc * dV;
+            Q[d] += ixeB * dnHn + egC * dknL[d] + egB * (dkp[d][ka[L]] + dkp[d][kb[L]]);
+        }
+        // psi: du = v, dv = -u
+        {
+            const double dU = 4.0 * un[L] * vn[L];
+            const double dV = 2.0 * (vn[L] * vn[L] - un[L] * un[L]);
+            Q[3] += ixeB * (R.Yp * dU + R.Yc * dV);
+        }
+        // cos inc: only the Y factors
+        Q[4] += ixeB * (R.dYp * (un[L] * un[L] - vn[L] * vn[L]) + R.dYc * (2.0 * un[L] * vn[L]));
+    }
+
+    cd WT = mk(0.0, 0.0);
+    for (int c = 0; c < 3; c++) WT += W[c] * T[c];
+    for (int k = 0; k < NI; k++) acc[k] += Q[0].re * Tf.d[k] + aA * A.d[k] + aPh * (Ph.d[k] + pRd_t * Tf.d[k]);
+    acc[G_DIST] += -aA * Av / dist;
+    acc[G_PHI] += -R.m * WT.im;  // T ~ e^{i m phi_ref}
+    acc[G_COSI] += Q[4].re;
+    acc[G_LAM] += Q[1].re + aPh * pRd_l;
+    acc[G_SINB] += (Q[2].re + aPh * pRd_b) * R.inv_cosb;
+    acc[G_PSI] += Q[3].re;
+    acc[G_TREF] += Q[0].re + aPh * (2. * PI * f + pRd_t);
+}
+
+
+}  // namespace gradk
+}  // namespace bbhx

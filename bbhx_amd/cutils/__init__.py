@@ -83,6 +83,10 @@ def load_library():
         "bbhx_hip_direct_fused": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _ip, _ip, C.c_int, _vp, _vp, _vp, _vp, C.c_int,
                                   C.c_double, C.c_double, C.c_double, C.c_double, _vp, C.c_int, C.c_int, C.c_int,
                                   C.c_int, C.c_int, C.c_int, _vp],
+        "bbhx_hip_direct_fused_grad": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _ip, _ip, C.c_int, _vp, _vp, _vp, _vp,
+                                       C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, _vp, C.c_int,
+                                       C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _vp],
+        "bbhx_hip_direct_plan_enable_grad": [_vp],
         "bbhx_hip_release_workspace": [],
         "bbhx_hip_version": [],
         "bbhx_hip_timing_enable": [C.c_int],
@@ -105,6 +109,7 @@ def exported_symbols():
         "bbhx_hip_direct_sum", "bbhx_hip_hdyn", "bbhx_hip_direct_like", "bbhx_hip_hdyn_fused",
         "bbhx_hip_hdyn_fused_grad", "bbhx_hip_direct_plan_create", "bbhx_hip_direct_plan_destroy",
         "bbhx_hip_direct_plan_bytes", "bbhx_hip_direct_plan_band", "bbhx_hip_direct_fused", "bbhx_hip_hdyn_prep",
+        "bbhx_hip_direct_fused_grad", "bbhx_hip_direct_plan_enable_grad",
         "bbhx_hip_release_workspace", "bbhx_hip_version", "bbhx_hip_timing_enable", "bbhx_hip_timing_read",
     ]
 
@@ -382,6 +387,10 @@ class DirectPlan:
     def nbytes(self):
         return int(load_library().bbhx_hip_direct_plan_bytes(self._h))
 
+    def enable_grad(self):
+        """Allocate the gradient buffers (288 B per (knot, mode) + records with tangents)."""
+        _check(load_library().bbhx_hip_direct_plan_enable_grad(self._h), "direct_plan_enable_grad")
+
     def band(self):
         """(B, 2) int32 dense band [st, et) of each binary from the last call."""
         out = np.zeros((self.num_bin_all, 2), dtype=np.int32)
@@ -415,6 +424,26 @@ def direct_fused(plan, d_h, h_h, flags, params, t_start, t_end, ells, mms, data_
         float(initial_t_val), float(mf_min), float(mf_max), response.handle, int(TDItag), int(bool(rescaled)),
         int(bool(tdi2)), plan.chunk, plan.block, plan.grid, _stream())
     _check(st, "direct_fused")
+
+
+def direct_fused_grad(plan, d_h, h_h, grad, flags, params, t_start, t_end, ells, mms, data_freqs, data_channels,
+                      noise_factors, qnm, dspin, initial_t_val, mf_min, mf_max, response, TDItag, rescaled, tdi2):
+    """Fused direct likelihood of a ragged batch and its (11, B) gradient (no reference counterpart)."""
+    torch = _torch()
+    f64, c128 = torch.float64, torch.complex128
+    el, elp = _host_i32(ells)
+    mm, mmp = _host_i32(mms)
+    B = plan.num_bin_all
+    if tuple(params.shape) != (11, B):
+        raise ValueError("params must be (11, B) for this plan")
+    st = load_library().bbhx_hip_direct_fused_grad(
+        plan.handle, _dev(d_h, c128), _dev(h_h, c128), _dev(grad, f64, "grad"), _dev(flags, torch.int32, "flags"),
+        _dev(params, f64, "params"),
+        _dev(t_start, f64), _dev(t_end, f64), elp, mmp, int(el.size), _dev(data_freqs, f64),
+        _dev(data_channels, c128), _dev(noise_factors, f64), _dev(qnm, f64), int(qnm.numel()) // 8, float(dspin),
+        float(initial_t_val), float(mf_min), float(mf_max), response.handle, int(TDItag), int(bool(rescaled)),
+        int(bool(tdi2)), plan.chunk, plan.block, plan.grid, _stream())
+    _check(st, "direct_fused_grad")
 
 
 def hdyn_prep(data_constants, totals, data_freqs, data_channels, h0, psd, nodes, df):
@@ -494,6 +523,7 @@ class BBHxHipBackend:
         self.direct_fused = direct_fused
         self.DirectPlan = DirectPlan
         self.hdyn_prep = hdyn_prep
+        self.direct_fused_grad = direct_fused_grad
         self.device = torch.device("cuda", torch.cuda.current_device())
 
     @property

@@ -129,6 +129,60 @@ class Likelihood(BBHxParallelModule):
         return out
 
 
+    # ---------------------------------------------------------- gradient (fused ragged path)
+
+    GRAD_COORDS = ("lnM", "q", "chi1z", "chi2z", "distance", "phi_ref", "cos_inc", "lam", "sin_beta", "psi", "t_ref")
+
+    def get_ll_grad_fused_device(self, plan, rows, t_start, t_end, flags, modes=None):
+        """Fused direct log-likelihood (B,) and its gradient (11, B) in the
+        sampling coordinates (``GRAD_COORDS``), from device inputs; no host
+        synchronisation. The plan's gradient buffers are allocated on first use.
+        The observation window and the sparse grids are held fixed."""
+        torch = self.xp
+        gen = self.waveform_gen
+        if modes is None:
+            modes = gen.amp_phase_gen.allowable_modes
+        ells = np.asarray([l for l, m in modes], dtype=np.int32)
+        mms = np.asarray([m for l, m in modes], dtype=np.int32)
+        B = plan.num_bin_all
+        if not getattr(plan, "_grad_ready", False):
+            plan.enable_grad()
+            plan._grad_ready = True
+        d_h = torch.empty(B, dtype=torch.complex128, device=self.device)
+        h_h = torch.empty(B, dtype=torch.complex128, device=self.device)
+        grad = torch.empty((11, B), dtype=torch.float64, device=self.device)
+        ap, resp = gen.amp_phase_gen, gen.response_gen
+        self.backend.direct_fused_grad(plan, d_h, h_h, grad, flags, rows, t_start, t_end, ells, mms, self.data_freqs,
+                                       self.data_channels, self.noise_factors, ap.qnm_packed, ap.dspin,
+                                       ap.initial_t_val, ap.mf_min, ap.mf_max, resp.cpp_response, resp.TDItag_int,
+                                       resp.rescaled, resp.tdi2)
+        self.d_h, self.h_h = d_h, h_h
+        return -1 / 2 * (self.d_d + h_h - 2 * d_h).real, grad
+
+    def get_ll_and_grad_fused(self, x, lengths, plan=None, modes=None, t_obs_start=0.0, t_obs_end=1.0):
+        """Direct log-likelihood (B,) and gradient (B, 11) at sampling
+        coordinates x (B, 11) (see ``HeterodynedLikelihood.sampling_to_params``)
+        for templates with sparse grid lengths ``lengths``; numpy arrays."""
+        torch = self.xp
+        params = HeterodynedLikelihood.sampling_to_params(np.asarray(x, dtype=np.float64))
+        rows, t_start, t_end = fused_param_rows(self.waveform_gen, params, t_obs_start, t_obs_end, self.device)
+        lengths = np.asarray(lengths, dtype=np.int32)
+        if lengths.shape != (rows.shape[1],):
+            raise ValueError("lengths must have one entry per template")
+        if plan is None:
+            plan = self.direct_plan(lengths, modes)
+        flags = torch.zeros(rows.shape[1], dtype=torch.int32, device=self.device)
+        ll, grad = self.get_ll_grad_fused_device(plan, rows, t_start, t_end, flags, modes)
+        if np.any(flags.cpu().numpy() != 0):
+            raise ValueError("The search for an observation window did not run correctly for some template.")
+        return ll.cpu().numpy(), grad.T.cpu().numpy()
+
+    def loglike(self, x, lengths, plan=None, modes=None):
+        """Differentiable direct log-likelihood: x (B, 11) float64 torch tensor of
+        sampling coordinates; backward through the adjoint kernels (``DirectLogLike``)."""
+        return DirectLogLike.apply(x, self, lengths, plan, modes)
+
+
 def fused_param_rows(template_gen, params, t_obs_start, t_obs_end, device):
     """Device inputs of the fused kernels: (11, B) parameter rows (m1 >= m2
     enforced as phenomhm.py:175-199 does; f_ref dropped), t_start, t_end
@@ -404,3 +458,23 @@ class HeterodynedLogLike(torch.autograd.Function):
     def backward(ctx, grad_out):
         (g,) = ctx.saved_tensors
         return grad_out[:, None] * g, None
+
+
+class DirectLogLike(torch.autograd.Function):
+    """torch.autograd.Function over the fused direct log-likelihood of a ragged
+    batch: forward runs the adjoint launch sequence (value and gradient in one
+    pass) and keeps the (B, 11) gradient for backward. Arguments: x (B, 11)
+    sampling coordinates (float64 tensor), the Likelihood, lengths, plan, modes."""
+
+    @staticmethod
+    def forward(ctx, x, like, lengths, plan, modes):
+        dev = x.device
+        ll, grad = like.get_ll_and_grad_fused(x.detach().cpu().numpy(), lengths, plan=plan, modes=modes)
+        g = torch.as_tensor(grad, dtype=x.dtype, device=dev)
+        ctx.save_for_backward(g)
+        return torch.as_tensor(ll, dtype=x.dtype, device=dev)
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        (g,) = ctx.saved_tensors
+        return grad_out[:, None] * g, None, None, None, None

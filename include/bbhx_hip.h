@@ -223,6 +223,25 @@ int bbhx_hip_direct_fused(bbhx_direct_plan* p, double* d_h, double* h_h, int* fl
                           double initial_t_val, double mf_min, double mf_max, const bbhx_response* orbits,
                           int TDItag, int rescaled, int tdi2, int chunk, int block, int grid, void* stream);
 
+/* Gradient of the fused direct log-likelihood with respect to the 11 sampling
+ * coordinates (ln M, q, chi1, chi2, d, phi_ref, cos inc, lam, sin beta, psi,
+ * t_ref); no counterpart in the reference, which has no derivatives.
+ * enable_grad allocates the plan's gradient buffers once (288 B per (knot,
+ * mode) and 4.9 KB per (binary, mode)). direct_fused_grad takes the arguments
+ * of bbhx_hip_direct_fused plus grad (11, B) (overwritten) and writes d_h, h_h
+ * as that call does. The sparse grids and the observation window are held
+ * fixed (their dependence on the parameters is not differentiated). The
+ * adjoint records are summed with FP64 atomics, so the gradient is not bit for
+ * bit reproducible (its value is, to rounding); d_h, h_h are. */
+int bbhx_hip_direct_plan_enable_grad(bbhx_direct_plan* p);
+int bbhx_hip_direct_fused_grad(bbhx_direct_plan* p, double* d_h, double* h_h, double* grad, int* flags,
+                               const double* params, const double* t_start, const double* t_end,
+                               const int* ells_host, const int* mms_host, int numModes, const double* dataFreqs,
+                               const double* dataChannels, const double* noiseFactors, const double* qnm,
+                               int table_len, double dspin, double initial_t_val, double mf_min, double mf_max,
+                               const bbhx_response* orbits, int TDItag, int rescaled, int tdi2, int chunk, int block,
+                               int grid, void* stream);
+
 /* Heterodyne constants of a reference template (HeterodynedLikelihood.
  * init_heterodyne_info, likelihood.py:499-533; the reference's unused kernel
  * for it is prep_hdyn, Likelihood.cu:238-352). dataFreqs (N,) sorted, already
