@@ -536,7 +536,7 @@ __device__ inline int adj_q(int j) { return j < 2 ? j : j + 1; }
 // Lam_c = 4 w_c (d~_c - h~_c) (d logL = Re sum_c conj(Lam_c) dh_c with
 // h_c = sum_m conj(T_c A e^{i phase}); Likelihood.cu:357-375 differentiated).
 template <class Q>
-__device__ inline void mode_adjoint(double cb[32], const Q& q, double x, double ts, double te, const cd Lam[3])
+__device__ inline bool mode_adjoint(double cb[32], const Q& q, double x, double ts, double te, const cd Lam[3])
 {
     const double x2 = x * x, x3 = x * x2;
     auto ev = [&](int k) {
@@ -544,7 +544,7 @@ __device__ inline void mode_adjoint(double cb[32], const Q& q, double x, double 
         return c.y + c.c1 * x + c.c2 * x2 + c.c3 * x3;
     };
     const double tf = ev(2);
-    if (!((tf >= ts) && ((tf <= te) || (te <= 0.0)))) return;
+    if (!((tf >= ts) && ((tf <= te) || (te <= 0.0)))) return false;
     const double A = ev(0);
     const cd E = cexpi_red(ev(1));
     const cd AE = A * E;
@@ -568,6 +568,21 @@ __device__ inline void mode_adjoint(double cb[32], const Q& q, double x, double 
         cb[4 * j + 2] = a[j] * x2;
         cb[4 * j + 3] = a[j] * x3;
     }
+    return true;
+}
+
+// One step of reduce32: lanes whose bit BIT is set keep the upper H values,
+// the others the lower H, and each adds what its partner (lane ^ BIT) sends.
+template <int H, int BIT>
+__device__ inline void halve(double v[32], int lane)
+{
+    const bool up = (lane & BIT) != 0;
+#pragma unroll
+    for (int j = 0; j < H; j++) {
+        const double send = up ? v[j] : v[j + H];
+        const double keep = up ? v[j + H] : v[j];
+        v[j] = keep + __shfl_xor(send, BIT, 64);
+    }
 }
 
 // Sum of v[0..32) over the 64 lanes of a wavefront by recursive halving (each
@@ -575,16 +590,11 @@ __device__ inline void mode_adjoint(double cb[32], const Q& q, double x, double 
 // returns the total of value l >> 1 (31 exchanges instead of 32 x 6).
 __device__ inline double reduce32(double v[32], int lane)
 {
-#pragma unroll
-    for (int h = 16, bit = 32; h >= 1; h >>= 1, bit >>= 1) {
-        const bool up = (lane & bit) != 0;
-#pragma unroll
-        for (int j = 0; j < h; j++) {
-            const double send = up ? v[j] : v[j + h];
-            const double keep = up ? v[j + h] : v[j];
-            v[j] = keep + __shfl_xor(send, bit, 64);
-        }
-    }
+    halve<16, 32>(v, lane);
+    halve<8, 16>(v, lane);
+    halve<4, 8>(v, lane);
+    halve<2, 4>(v, lane);
+    halve<1, 2>(v, lane);
     return v[0] + __shfl_xor(v[0], 1, 64);
 }
 
@@ -598,13 +608,10 @@ __device__ inline void mode_adjoint_per_lane(double* __restrict__ Ym, const SegR
     while (f >= bnd.y && i < L - 2) bnd = *reinterpret_cast<const double2*>(rm + (long)++i * M);
     if (!(bnd.x <= f && f < bnd.y)) return;
     double cb[32];
-#pragma unroll
-    for (int j = 0; j < 32; j++) cb[j] = 0.0;
-    mode_adjoint(cb, *(rm + (long)i * M), f - bnd.x, ts, te, Lam);
+    if (!mode_adjoint(cb, *(rm + (long)i * M), f - bnd.x, ts, te, Lam)) return;
     double* y = Ym + (long)i * M * NA;
 #pragma unroll
-    for (int j = 0; j < 32; j++)
-        if (cb[j] != 0.0) unsafeAtomicAdd(y + adj_q(j >> 2) * 4 + (j & 3), cb[j]);
+    for (int j = 0; j < 32; j++) unsafeAtomicAdd(y + adj_q(j >> 2) * 4 + (j & 3), cb[j]);
 }
 
 // Persistent dense-band kernel: partial[it] = (Re, Im of sum conj(d~) h~, sum |h~|^2, 0)
@@ -772,11 +779,14 @@ __global__ __launch_bounds__(256) void k_dir_like_t(double* __restrict__ partial
                             double cb[32];
 #pragma unroll
                             for (int j = 0; j < 32; j++) cb[j] = 0.0;
-                            if (in) mode_adjoint(cb, *rc, f[r] - xlo, ts, te, h[r]);
-                            const double tot = reduce32(cb, lane);
-                            if ((lane & 1) == 0) {
-                                const int j = lane >> 1;
-                                if (tot != 0.0) unsafeAtomicAdd(Ym + (long)cur * M * NA + adj_q(j >> 2) * 4 + (j & 3), tot);
+                            bool used = false;
+                            if (in) used = mode_adjoint(cb, *rc, f[r] - xlo, ts, te, h[r]);
+                            if (__ballot(used) != 0) {
+                                const double tot = reduce32(cb, lane);
+                                if ((lane & 1) == 0) {
+                                    const int j = lane >> 1;
+                                    unsafeAtomicAdd(Ym + (long)cur * M * NA + adj_q(j >> 2) * 4 + (j & 3), tot);
+                                }
                             }
                             pending = pending && !in;
                             if (__ballot(pending) != 0) {

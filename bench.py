@@ -191,6 +191,29 @@ def bench_direct_ragged(args):
         f"plan {plan.nbytes() / 2**30:.2f} GiB; {dense} dense (bin, template) pairs "
         f"({dense / (ms * 1e-3):.4g}/s); split {split}")
 
+    # gradient (11 parameters) of the same batch: the adjoint launch sequence
+    grad_info = None
+    if not args.no_grad:
+        for _ in range(max(args.warmup, 1)):
+            flags.zero_()
+            like.get_ll_grad_fused_device(plan, rows, ts, te, flags, MODES)
+        torch.cuda.synchronize()
+        timing_enable(True)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            flags.zero_()
+            llg, _ = like.get_ll_grad_fused_device(plan, rows, ts, te, flags, MODES)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        gs, ge, gn = timing_read()
+        timing_enable(False)
+        gms = (t1 - t0) * 1e3 / args.steps
+        same = bool(np.array_equal(llg.cpu().numpy(), eager))
+        grad_info = {"grad_ms": gms, "grad_evals_per_s": B / (gms * 1e-3), "grad_cost_vs_forward": gms / ms,
+                     "grad_setup_ms": gs / max(gn, 1), "grad_eval_ms": ge / max(gn, 1), "logL_equal_forward": same}
+        log(f"gradient: {gms:.3f} ms/step ({gms / ms:.2f}x the forward), setup {gs / max(gn, 1):.2f} ms, adjoint "
+            f"{ge / max(gn, 1):.2f} ms; logL equal to the forward's: {same}")
+
     # roofline of the dense-band kernel: the oracle's counted FP64 work per dense
     # (bin, template) x the pairs of this batch, over the kernel's event-timed duration
     model = json.loads((ROOT / "bbhx_amd" / "flop_model.json").read_text())["direct"]
@@ -252,7 +275,7 @@ def bench_direct_ragged(args):
                    "lengths_min": int(lengths.min()), "lengths_max": int(lengths.max()),
                    "lengths_sum": int(lengths.sum()), "dense_bins": int(f.size), "modes": 6, "tdi": "AET tdi2",
                    "launch": {k: tuned[k] for k in ("chunk", "block", "grid")}, "graph_vs_eager": diff,
-                   "dense_bin_templates": dense, **split},
+                   "dense_bin_templates": dense, **split, "gradient": grad_info},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
@@ -270,6 +293,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=4096)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-grad", action="store_true", help="direct workloads: skip the gradient timing")
     ap.add_argument("--no-grad-1024", action="store_true",
                     help="skip the gradient workload of 1024 templates at K = 256")
     ap.add_argument("--workload", default="heterodyned", choices=("heterodyned", "direct-ragged"))
