@@ -30,6 +30,7 @@
 //                 (Likelihood.cu:357-471); one partial sum per work item
 //   k_dir_reduce  per binary, the partials in chunk order (deterministic)
 
+#include <type_traits>
 #include <vector>
 
 #include "bbhx_hip.h"
@@ -68,6 +69,9 @@ struct bbhx_direct_plan {
     double* Yadj = nullptr;              // (S, M, 9, 4) adjoint records
     bbhx::gradk::GradRec* grecs = nullptr;  // (B, M) records with tangents
     double* gpart = nullptr;             // (B, M, 11) per-mode gradients
+    // FP32-bulk form (bbhx_hip_direct_plan_set_precision)
+    int precision = 0;                   // 0: FP64, 1: FP32 bulk, FP64 phase and t(f)
+    void* CfF = nullptr;                 // (S, M) 192-B segment records of that form
     int max_chunks = 0;           // absolute chunks of min_chunk bins
 };
 
@@ -89,6 +93,28 @@ struct alignas(32) SegRec {
 constexpr int RS = sizeof(SegRec) / sizeof(double);  // 40 doubles
 constexpr int RQ = 4;                                 // offset of q[0] in doubles
 static_assert(sizeof(SegRec) == 320, "segment record layout");
+
+// Segment record of the FP32-bulk form: bounds and the phase and t(f) cubics
+// in FP64 (phases reach 1e8 rad, and t(f) decides the observation window
+// exactly as in the FP64 form), amplitude and the six transfer-function parts
+// (T1..T3 re, im) in FP32.
+struct alignas(64) SegRecF {
+    double x_lo, x_hi;
+    double ph[4];    // y, c1, c2, c3
+    double tf[4];
+    float q[7][4];   // A, T1 re, T1 im, T2 re, T2 im, T3 re, T3 im: y, c1, c2, c3
+};
+static_assert(sizeof(SegRecF) == 192, "FP32-bulk segment record layout");
+
+// single-precision complex (the mode sum of the FP32-bulk form)
+struct cf {
+    float re, im;
+};
+
+__device__ inline cd to_cd(cd a) { return a; }
+__device__ inline cd to_cd(cf a) { return mk((double)a.re, (double)a.im); }
+__device__ inline void zero(cd& a) { a = mk(0.0, 0.0); }
+__device__ inline void zero(cf& a) { a = cf{0.0f, 0.0f}; }
 
 __device__ inline const Cubic& cubic_of(const SegRec& r, int k) { return r.q[k]; }
 __device__ inline const Cubic& cubic_of(const Cubic (&c)[NQ], int k) { return c[k]; }
@@ -251,7 +277,8 @@ __global__ __launch_bounds__(64) void k_dir_grid(double* __restrict__ X, double*
 // Thomas algorithm over the binary's L knots (the system of spline.hip);
 // consecutive lanes read consecutive quantities of the same knot.
 __global__ __launch_bounds__(64) void k_dir_spline(const double* __restrict__ X, double* __restrict__ Cf,
-                                                   const long* __restrict__ off, const int* __restrict__ len, int M)
+                                                   SegRecF* __restrict__ CfF, const long* __restrict__ off,
+                                                   const int* __restrict__ len, int M)
 {
     const int b = blockIdx.x;
     const int s = threadIdx.x;
@@ -267,6 +294,31 @@ __global__ __launch_bounds__(64) void k_dir_spline(const double* __restrict__ X,
     double* dp = Cf + 1;  // d' then the slopes (= c1)
     double* C2 = Cf + 2;
     double* cp = Cf + 3;  // c' then c3
+    // the FP32-bulk record of point i (CfF != nullptr): this lane's quantity
+    auto put_f = [&](int i, double x_lo, double x_hi, double c0, double c1, double c2, double c3) {
+        SegRecF& rf = CfF[(base + i) * M + m];
+        if (q == 1) {
+            rf.ph[0] = c0;
+            rf.ph[1] = c1;
+            rf.ph[2] = c2;
+            rf.ph[3] = c3;
+        } else if (q == 2) {
+            rf.tf[0] = c0;
+            rf.tf[1] = c1;
+            rf.tf[2] = c2;
+            rf.tf[3] = c3;
+        } else {
+            const int k = q == 0 ? 0 : q - 2;
+            rf.q[k][0] = (float)c0;
+            rf.q[k][1] = (float)c1;
+            rf.q[k][2] = (float)c2;
+            rf.q[k][3] = (float)c3;
+        }
+        if (q == 0) {
+            rf.x_lo = x_lo;
+            rf.x_hi = x_hi;
+        }
+    };
 
     double x0 = xi(0), x1 = xi(1), x2 = xi(2);
     double y0 = yi(0), y1 = yi(1), y2 = yi(2);
@@ -315,15 +367,18 @@ __global__ __launch_bounds__(64) void k_dir_spline(const double* __restrict__ X,
         Cf[((base + n - 1) * M + m) * RS] = xn;
         Cf[((base + n - 1) * M + m) * RS + 1] = xn;
     }
+    if (CfF) put_f(n - 1, xn, xn, yn, 0.0, 0.0, 0.0);
     for (int i = n - 2; i >= 0; i--) {
         const double si = dp[at(i)] - cp[at(i)] * snext;
         const double x_i = xi(i), y_i = yi(i);
         const double dx = xn - x_i;
         const double slope = (yn - y_i) / dx;
         const double t = (si + snext - 2 * slope) / dx;
+        const double c2 = (slope - si) / dx - t, c3 = t / dx;
         dp[at(i)] = si;
-        C2[at(i)] = (slope - si) / dx - t;
-        cp[at(i)] = t / dx;
+        C2[at(i)] = c2;
+        cp[at(i)] = c3;
+        if (CfF) put_f(i, x_i, xn, y_i, si, c2, c3);
         if (q == 0) {
             Cf[((base + i) * M + m) * RS] = x_i;
             Cf[((base + i) * M + m) * RS + 1] = xn;
@@ -518,6 +573,70 @@ __device__ inline void mode_per_lane(cd h[3], const SegRec* __restrict__ rm, int
     add_mode(h, cq, f - bnd.x, ts, te);
 }
 
+// e^{i phase} in FP32 from an FP64 phase: the phase in revolutions reduced to
+// [-1/2, 1/2] in FP64 (for |phase| <= 1e9 rad the product and the rounding cost
+// < 2e-8 rad), then the hardware single-precision cosine and sine, which take
+// their argument in revolutions (error <= 1.3e-7). Their magnitude falls short
+// of 1 by ~3e-8 on average, a bias that does not average out over the dense
+// bins (it scales every template); *corr receives delta = (1 - c^2 - s^2) / 2,
+// formed in FP64 (products of two floats are exact there; in FP32 the sum
+// near 1 cannot resolve it), and the caller scales by 1 + delta.
+__device__ inline cf cexpi_f32(double phase, float* corr)
+{
+    double rev = phase * 0.15915494309189535;  // 1 / (2 pi)
+    rev -= rint(rev);
+    const float r = (float)rev;
+    const float c = __builtin_amdgcn_cosf(r), s = __builtin_amdgcn_sinf(r);
+    const double cd_ = c, sd_ = s;
+    *corr = (float)(-0.5 * fma(cd_, cd_, fma(sd_, sd_, -1.0)));
+    return cf{c, s};
+}
+
+// add_mode of the FP32-bulk form: t(f) and the phase evaluated in FP64 (the
+// same expressions as the FP64 form, so the window decision is identical),
+// the other seven cubics and the mode sum in FP32.
+__device__ inline void add_mode(cf h[3], const SegRecF& q, double x, double ts, double te)
+{
+    const double x2 = x * x, x3 = x * x2;
+    const double tf = q.tf[0] + q.tf[1] * x + q.tf[2] * x2 + q.tf[3] * x3;
+    if ((tf >= ts) && ((tf <= te) || (te <= 0.0))) {
+        float corr;
+        const cf E = cexpi_f32(q.ph[0] + q.ph[1] * x + q.ph[2] * x2 + q.ph[3] * x3, &corr);
+        const float xf = (float)x, xf2 = xf * xf, xf3 = xf2 * xf;
+        auto ev = [&](int k) { return q.q[k][0] + q.q[k][1] * xf + q.q[k][2] * xf2 + q.q[k][3] * xf3; };
+        const float A0 = ev(0);
+        const float A = fmaf(A0, corr, A0);  // |e^{i phase}| = 1 restored
+        const float ar = A * E.re, ai = A * E.im;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const float tr = ev(1 + 2 * c), ti = ev(2 + 2 * c);
+            // conj(T A e^{i phase})
+            h[c].re += tr * ar - ti * ai;
+            h[c].im -= tr * ai + ti * ar;
+        }
+    }
+}
+
+// mode_per_lane of the FP32-bulk form: the whole 192-B record in one round of loads
+__device__ inline void mode_per_lane(cf h[3], const SegRecF* __restrict__ rm, int M, int L, double f, int i,
+                                     double ts, double te)
+{
+    SegRecF rec = rm[(long)i * M];
+    // keeps the compiler from sinking these loads below the bounds check
+#pragma unroll
+    for (int k = 0; k < 4; k++) asm volatile("" ::"v"(rec.ph[k]), "v"(rec.tf[k]));
+#pragma unroll
+    for (int k = 0; k < 7; k++) asm volatile("" ::"v"(rec.q[k][0]), "v"(rec.q[k][1]), "v"(rec.q[k][2]), "v"(rec.q[k][3]));
+    if (!(rec.x_lo <= f && f < rec.x_hi)) {
+        double2 bnd = make_double2(rec.x_lo, rec.x_hi);
+        while (f < bnd.x && i > 0) bnd = *reinterpret_cast<const double2*>(rm + (long)--i * M);
+        while (f >= bnd.y && i < L - 2) bnd = *reinterpret_cast<const double2*>(rm + (long)++i * M);
+        if (!(bnd.x <= f && f < bnd.y)) return;
+        rec = rm[(long)i * M];
+    }
+    add_mode(h, rec, f - rec.x_lo, ts, te);
+}
+
 constexpr int RB = 2;  // iterations of 64 dense bins per block of a wavefront
 
 // ---------------------------------------------------------------- adjoint helpers
@@ -634,11 +753,11 @@ __device__ inline void mode_adjoint_per_lane(double* __restrict__ Ym, const SegR
 // mode sum of a block, Lam = 4 w (d~ - h~) per bin and channel, then the modes
 // again, each bin's coefficient adjoints (mode_adjoint) summed over the lanes
 // that share a segment (reduce32) and added to the segment's adjoint record.
-template <bool ADJ>
+template <bool ADJ, class Rec>
 __global__ __launch_bounds__(256) void k_dir_like_t(double* __restrict__ partial, double* __restrict__ Yadj,
                                                     const int* __restrict__ band,
                                                   const int* __restrict__ total, const int* __restrict__ perm,
-                                                  int* __restrict__ next, const SegRec* __restrict__ R,
+                                                  int* __restrict__ next, const Rec* __restrict__ R,
                                                   const double* __restrict__ grid_lo,
                                                   const double* __restrict__ grid_istep,
                                                   const long* __restrict__ off, const int* __restrict__ len,
@@ -647,6 +766,8 @@ __global__ __launch_bounds__(256) void k_dir_like_t(double* __restrict__ partial
                                                   const cd* __restrict__ dch, const double* __restrict__ wts, int N,
                                                   int B, int M, int chunk)
 {
+    static_assert(!ADJ || std::is_same<Rec, SegRec>::value, "the adjoint pass reads FP64 records");
+    using HT = typename std::conditional<std::is_same<Rec, SegRecF>::value, cf, cd>::type;
     __shared__ double red[3][256 / 64];
     __shared__ int s_next;
     const int nitems = total[0];
@@ -674,20 +795,20 @@ __global__ __launch_bounds__(256) void k_dir_like_t(double* __restrict__ partial
         double a0 = 0.0, a1 = 0.0, a2 = 0.0;
         for (int kb = kw0; kb < kw1; kb += RB * 64) {
             double f[RB], lf[RB];
-            cd h[RB][3];
+            HT h[RB][3];
 #pragma unroll
             for (int r = 0; r < RB; r++) {
                 const int k = kb + 64 * r + lane;
                 f[r] = k < kw1 ? freqs[k] : 0.0;
                 lf[r] = (double)__log2f((float)f[r]);
 #pragma unroll
-                for (int ch = 0; ch < 3; ch++) h[r][ch] = mk(0.0, 0.0);
+                for (int ch = 0; ch < 3; ch++) zero(h[r][ch]);
             }
             for (int m = 0; m < M; m++) {
                 const double gis = grid_istep[b * M + m];
                 if (!(gis > 0.0)) continue;  // degenerate grid: no segment holds any f
                 const double glo = grid_lo[b * M + m];
-                const SegRec* rm = R + base * M + m;
+                const Rec* rm = R + base * M + m;
                 int cur = -1;  // the walker's segment (wavefront-uniform); -1: not placed
 #pragma unroll
                 for (int r = 0; r < RB; r++) {
@@ -708,7 +829,7 @@ __global__ __launch_bounds__(256) void k_dir_like_t(double* __restrict__ partial
                     // makes progress; the bound only guards against an unsorted one
                     for (int guard = 2 * L + 8; guard > 0; guard--) {
                         if (__ballot(pending) == 0) break;
-                        const SegRec* rc = rm + (long)cur * M;
+                        const Rec* rc = rm + (long)cur * M;
                         const double xlo = rc->x_lo, xhi = rc->x_hi;
                         const bool below = pending && f[r] < xlo;
                         if (__ballot(below) != 0) {
@@ -733,17 +854,17 @@ __global__ __launch_bounds__(256) void k_dir_like_t(double* __restrict__ partial
 #pragma unroll
                     for (int ch = 0; ch < 3; ch++) {
                         const double w = wts[(long)ch * N + k];
-                        const cd ht = h[r][ch] * w;
+                        const cd ht = to_cd(h[r][ch]) * w;
                         const cd d = dch[(long)ch * N + k];
                         const cd dh = conj(d) * ht;
                         a0 += dh.re;
                         a1 += dh.im;
                         a2 += ht.re * ht.re + ht.im * ht.im;
-                        if (ADJ) h[r][ch] = (4.0 * w) * (d - ht);  // Lam
+                        if constexpr (ADJ) h[r][ch] = (4.0 * w) * (d - ht);  // Lam
                     }
                 }
             }
-            if (ADJ) {
+            if constexpr (ADJ) {
                 for (int m = 0; m < M; m++) {
                     const double gis = grid_istep[b * M + m];
                     if (!(gis > 0.0)) continue;
@@ -1027,7 +1148,7 @@ T* dalloc(size_t n, bool& ok)
 void plan_free(bbhx_direct_plan* p)
 {
     void* bufs[] = {p->off, p->len, p->recs, p->X, p->Cf, p->grid_lo, p->grid_istep, p->band,
-                    p->total, p->partial, p->perm, p->cnt, p->next, p->Yadj, p->grecs, p->gpart};
+                    p->total, p->partial, p->perm, p->cnt, p->next, p->Yadj, p->grecs, p->gpart, p->CfF};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
 }
@@ -1111,6 +1232,7 @@ extern "C" long bbhx_hip_direct_plan_bytes(const bbhx_direct_plan* p)
     long bytes = SM * 8 * (1 + RS) + (long)p->B * p->M * (sizeof(FusedRec) + 16) + p->max_items * 36 + p->B * 24 +
                  4L * (p->max_chunks + 1);
     if (p->Yadj) bytes += SM * 8 * NA + (long)p->B * p->M * (sizeof(bbhx::gradk::GradRec) + 8 * bbhx::gradk::NGRAD);
+    if (p->CfF) bytes += SM * (long)sizeof(SegRecF);
     return bytes;
 }
 
@@ -1163,7 +1285,10 @@ int direct_launch(bbhx_direct_plan* p, double* d_h, double* h_h, double* grad, i
     hipLaunchKernelGGL(k_dir_grid, dim3(B * M), dim3(64), sizeof(double) * (size_t)p->Lmax, st, p->X, p->Cf,
                        p->grid_lo, p->grid_istep, flags, p->recs, p->off, p->len, params, t_start, t_end, B, M, ml,
                        orb, TDItag, rescaled, tdi2, initial_t_val, mf_min, mf_max);
-    hipLaunchKernelGGL(k_dir_spline, dim3(B), dim3(64), 0, st, p->X, p->Cf, p->off, p->len, M);
+    // the FP32-bulk form (not for the gradient, whose adjoint pass reads the FP64 records)
+    const bool f32 = !grad && p->precision == 1;
+    hipLaunchKernelGGL(k_dir_spline, dim3(B), dim3(64), 0, st, p->X, p->Cf,
+                       f32 ? static_cast<SegRecF*>(p->CfF) : nullptr, p->off, p->len, M);
     hipLaunchKernelGGL(k_dir_band, dim3(1), dim3(1024), 0, st, p->band, p->total, p->X, p->off, p->len, dataFreqs,
                        p->N, B, M, chunk);
     const int nchunks = (p->N + chunk - 1) / chunk;  // <= max_chunks since chunk >= min_chunk
@@ -1181,13 +1306,18 @@ int direct_launch(bbhx_direct_plan* p, double* d_h, double* h_h, double* grad, i
     }
     if (timed) (void)hipEventRecord(ev[1], st);
     if (grad)
-        hipLaunchKernelGGL(k_dir_like_t<true>, dim3(grid), dim3(block), 0, st, p->partial, p->Yadj, p->band, p->total,
+        hipLaunchKernelGGL((k_dir_like_t<true, SegRec>), dim3(grid), dim3(block), 0, st, p->partial, p->Yadj, p->band, p->total,
                            p->perm, p->next, reinterpret_cast<const SegRec*>(p->Cf), p->grid_lo, p->grid_istep,
                            p->off, p->len, t_start, t_end, dataFreqs, reinterpret_cast<const cd*>(dataChannels),
                            noiseFactors, p->N, B, M, chunk);
+    else if (f32)
+        hipLaunchKernelGGL((k_dir_like_t<false, SegRecF>), dim3(grid), dim3(block), 0, st, p->partial, nullptr,
+                           p->band, p->total, p->perm, p->next, static_cast<const SegRecF*>(p->CfF), p->grid_lo,
+                           p->grid_istep, p->off, p->len, t_start, t_end, dataFreqs,
+                           reinterpret_cast<const cd*>(dataChannels), noiseFactors, p->N, B, M, chunk);
     else
-        hipLaunchKernelGGL(k_dir_like_t<false>, dim3(grid), dim3(block), 0, st, p->partial, nullptr, p->band,
-                           p->total, p->perm, p->next, reinterpret_cast<const SegRec*>(p->Cf), p->grid_lo,
+        hipLaunchKernelGGL((k_dir_like_t<false, SegRec>), dim3(grid), dim3(block), 0, st, p->partial, nullptr,
+                           p->band, p->total, p->perm, p->next, reinterpret_cast<const SegRec*>(p->Cf), p->grid_lo,
                            p->grid_istep, p->off, p->len, t_start, t_end, dataFreqs,
                            reinterpret_cast<const cd*>(dataChannels), noiseFactors, p->N, B, M, chunk);
     hipLaunchKernelGGL(k_dir_reduce, dim3(ceil_div(B, 256)), dim3(256), 0, st, d_h, h_h, p->partial, p->band,
@@ -1216,6 +1346,18 @@ extern "C" int bbhx_hip_direct_fused(bbhx_direct_plan* p, double* d_h, double* h
     return direct_launch(p, d_h, h_h, nullptr, flags, params, t_start, t_end, ells_host, mms_host, numModes,
                          dataFreqs, dataChannels, noiseFactors, qnm, table_len, dspin, initial_t_val, mf_min, mf_max,
                          orbits, TDItag, rescaled, tdi2, chunk, block, grid, stream);
+}
+
+extern "C" int bbhx_hip_direct_plan_set_precision(bbhx_direct_plan* p, int precision)
+{
+    if (!p || (precision != 0 && precision != 1)) return ERR_ARGS;
+    if (precision == 1 && !p->CfF) {
+        bool ok = true;
+        p->CfF = dalloc<SegRecF>((size_t)p->S * p->M, ok);
+        if (!ok) return ERR_ALLOC;
+    }
+    p->precision = precision;
+    return OK;
 }
 
 extern "C" int bbhx_hip_direct_plan_enable_grad(bbhx_direct_plan* p)

@@ -87,6 +87,7 @@ def load_library():
                                        C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, _vp, C.c_int,
                                        C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _vp],
         "bbhx_hip_direct_plan_enable_grad": [_vp],
+        "bbhx_hip_direct_plan_set_precision": [_vp, C.c_int],
         "bbhx_hip_release_workspace": [],
         "bbhx_hip_version": [],
         "bbhx_hip_timing_enable": [C.c_int],
@@ -109,7 +110,7 @@ def exported_symbols():
         "bbhx_hip_direct_sum", "bbhx_hip_hdyn", "bbhx_hip_direct_like", "bbhx_hip_hdyn_fused",
         "bbhx_hip_hdyn_fused_grad", "bbhx_hip_direct_plan_create", "bbhx_hip_direct_plan_destroy",
         "bbhx_hip_direct_plan_bytes", "bbhx_hip_direct_plan_band", "bbhx_hip_direct_fused", "bbhx_hip_hdyn_prep",
-        "bbhx_hip_direct_fused_grad", "bbhx_hip_direct_plan_enable_grad",
+        "bbhx_hip_direct_fused_grad", "bbhx_hip_direct_plan_enable_grad", "bbhx_hip_direct_plan_set_precision",
         "bbhx_hip_release_workspace", "bbhx_hip_version", "bbhx_hip_timing_enable", "bbhx_hip_timing_read",
     ]
 
@@ -366,6 +367,7 @@ class DirectPlan:
         self.num_modes, self.data_length = int(num_modes), int(data_length)
         self.min_chunk = int(min(min_chunk, chunk))
         self.set_launch(chunk, block, grid)
+        self.precision = 0
         h = _vp()
         _check(lib.bbhx_hip_direct_plan_create(C.byref(h), int(lens.size), lp, self.num_modes, self.data_length,
                                                self.min_chunk), "direct_plan_create")
@@ -390,6 +392,15 @@ class DirectPlan:
     def enable_grad(self):
         """Allocate the gradient buffers (288 B per (knot, mode) + records with tangents)."""
         _check(load_library().bbhx_hip_direct_plan_enable_grad(self._h), "direct_plan_enable_grad")
+
+    def set_precision(self, precision):
+        """0: FP64; 1: FP32 bulk (amplitude and transfer-function cubics, phase
+        factor and mode sum in FP32; phase, t(f) and the inner products in FP64).
+        The first switch to 1 allocates 192 B per (knot, mode)."""
+        precision = int(precision)
+        if precision != self.precision:
+            _check(load_library().bbhx_hip_direct_plan_set_precision(self._h, precision), "direct_plan_set_precision")
+            self.precision = precision
 
     def band(self):
         """(B, 2) int32 dense band [st, et) of each binary from the last call."""

@@ -78,11 +78,15 @@ class Likelihood(BBHxParallelModule):
         return self.backend.DirectPlan(lengths, M, self.data_stream_length, chunk=chunk, block=block, grid=grid,
                                        min_chunk=min_chunk)
 
-    def get_ll_fused_device(self, plan, rows, t_start, t_end, flags, modes=None):
+    def get_ll_fused_device(self, plan, rows, t_start, t_end, flags, modes=None, precision=None):
         """Fused direct log-likelihood of device inputs (``fused_inputs``): a
         device tensor, with no host synchronisation (graph-capturable; flags
-        must be zeroed before the call)."""
+        must be zeroed before the call). precision (None: the plan's current
+        one): 0 FP64, 1 FP32 bulk with FP64 phase (``DirectPlan.set_precision``;
+        the first switch to 1 allocates, so make it before a graph capture)."""
         torch = self.xp
+        if precision is not None:
+            plan.set_precision(precision)
         gen = self.waveform_gen
         if modes is None:
             modes = gen.amp_phase_gen.allowable_modes
@@ -101,10 +105,11 @@ class Likelihood(BBHxParallelModule):
         return -1 / 2 * (self.d_d + h_h - 2 * d_h).real
 
     def get_ll_fused(self, params, lengths, return_extracted_snr=False, phase_marginalize=False, plan=None,
-                     modes=None, t_obs_start=0.0, t_obs_end=1.0):
+                     modes=None, t_obs_start=0.0, t_obs_end=1.0, precision=0):
         """Direct log-likelihood of templates with per-template sparse grid
         lengths (``lengths[b]``), fused: no template is materialised. Same
-        params layout and result as ``get_ll(params, length=...)``."""
+        params layout and result as ``get_ll(params, length=...)``.
+        precision 1: the FP32-bulk form (``get_ll_fused_device``)."""
         torch = self.xp
         rows, t_start, t_end = fused_param_rows(self.waveform_gen, params, t_obs_start, t_obs_end, self.device)
         lengths = np.asarray(lengths, dtype=np.int32)
@@ -113,7 +118,7 @@ class Likelihood(BBHxParallelModule):
         if plan is None:
             plan = self.direct_plan(lengths, modes)
         flags = torch.zeros(rows.shape[1], dtype=torch.int32, device=self.device)
-        ll = self.get_ll_fused_device(plan, rows, t_start, t_end, flags, modes)
+        ll = self.get_ll_fused_device(plan, rows, t_start, t_end, flags, modes, precision=precision)
         fl = flags.cpu().numpy()
         if np.any(fl & 1):
             raise ValueError("The search for a window at x_min did not run correctly. If t_ref is close to zero, "

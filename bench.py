@@ -191,6 +191,44 @@ def bench_direct_ragged(args):
         f"plan {plan.nbytes() / 2**30:.2f} GiB; {dense} dense (bin, template) pairs "
         f"({dense / (ms * 1e-3):.4g}/s); split {split}")
 
+    # FP32-bulk form of the same batch (phase, t(f) and inner products FP64), its own graph
+    f32_info = None
+    if not args.no_fp32:
+        hh64 = like.h_h.real.cpu().numpy()
+        flags.zero_()
+        like.get_ll_fused_device(plan, rows, ts, te, flags, MODES, precision=1)  # allocates the records
+        torch.cuda.synchronize()
+        g32 = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g32):
+            flags.zero_()
+            out32 = like.get_ll_fused_device(plan, rows, ts, te, flags, MODES)
+        torch.cuda.synchronize()
+        g32.replay()
+        torch.cuda.synchronize()
+        ll32 = out32.cpu().numpy()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            g32.replay()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        ms32 = (t1 - t0) * 1e3 / args.steps
+        timing_enable(True)
+        for _ in range(2):
+            flags.zero_()
+            like.get_ll_fused_device(plan, rows, ts, te, flags, MODES)
+        s32, e32, n32 = timing_read()
+        timing_enable(False)
+        plan.set_precision(0)
+        dll = np.abs(ll32 - eager)
+        f32_info = {"ms": ms32, "evals_per_s": B / (ms32 * 1e-3), "speedup_vs_fp64": ms / ms32,
+                    "setup_ms": s32 / max(n32, 1), "dense_ms": e32 / max(n32, 1),
+                    "max_abs_dlogL_vs_fp64": float(dll.max()), "median_abs_dlogL_vs_fp64": float(np.median(dll)),
+                    "max_rel_dlogL_vs_hh": float(np.max(dll / np.maximum(np.abs(hh64), 1.0))),
+                    "templates_within_1e-3": int(np.sum(dll < 1e-3))}
+        log(f"FP32 bulk: {ms32:.3f} ms/step ({ms / ms32:.2f}x the FP64 form), dense {e32 / max(n32, 1):.2f} ms; "
+            f"|dlogL| vs FP64 max {dll.max():.2e}, median {np.median(dll):.2e}; "
+            f"{int(np.sum(dll < 1e-3))}/{B} within 1e-3")
+
     # gradient (11 parameters) of the same batch: the adjoint launch sequence
     grad_info = None
     if not args.no_grad:
@@ -275,7 +313,7 @@ def bench_direct_ragged(args):
                    "lengths_min": int(lengths.min()), "lengths_max": int(lengths.max()),
                    "lengths_sum": int(lengths.sum()), "dense_bins": int(f.size), "modes": 6, "tdi": "AET tdi2",
                    "launch": {k: tuned[k] for k in ("chunk", "block", "grid")}, "graph_vs_eager": diff,
-                   "dense_bin_templates": dense, **split, "gradient": grad_info},
+                   "dense_bin_templates": dense, **split, "fp32_bulk": f32_info, "gradient": grad_info},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
@@ -294,6 +332,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-grad", action="store_true", help="direct workloads: skip the gradient timing")
+    ap.add_argument("--no-fp32", action="store_true", help="direct workloads: skip the FP32-bulk form")
     ap.add_argument("--no-grad-1024", action="store_true",
                     help="skip the gradient workload of 1024 templates at K = 256")
     ap.add_argument("--workload", default="heterodyned", choices=("heterodyned", "direct-ragged"))

@@ -203,6 +203,15 @@ long bbhx_hip_direct_plan_bytes(const bbhx_direct_plan* p);
  * plan: band_host (B, 2) receives [st, et), the dense bins the binary's
  * template covers (waveformbuild.py:139-156). Waits for the device. */
 int bbhx_hip_direct_plan_band(const bbhx_direct_plan* p, int* band_host);
+/* Arithmetic of later bbhx_hip_direct_fused calls on this plan (no
+ * counterpart in the reference, which computes in FP64 only): 0 = FP64
+ * (default); 1 = FP32 bulk: the amplitude and transfer-function cubics, the
+ * phase factor e^{i phase} and the mode sum in FP32, while the phase and t(f)
+ * cubics (and the reduction of the phase modulo 2 pi) and the noise-weighted
+ * inner products stay FP64. The first switch to 1 allocates 192 B per (knot,
+ * mode) of FP32-bulk segment records (call it outside graph capture). The
+ * gradient entry point always computes in FP64. */
+int bbhx_hip_direct_plan_set_precision(bbhx_direct_plan* p, int precision);
 
 /* params (11, B) and t_start, t_end (B,) as for bbhx_hip_hdyn_fused; dataFreqs
  * (N,) sorted; dataChannels (3, N) complex = noise weight x data, as

@@ -2,7 +2,10 @@
 against the FP64 CPU oracle run template by template at each template's own
 length, against the kernel-by-kernel GPU path, and under hipGraph capture.
 
-Tolerance: |delta logL| < 1e-6 absolute (FP64 on both sides).
+Tolerance: |delta logL| < 1e-6 absolute (FP64 on both sides). The FP32-bulk
+form (amplitude and transfer-function cubics, phase factor and mode sum in
+FP32; phase, t(f) and the inner products in FP64): |delta logL| < 1e-3 against
+the oracle, and d_h, h_h within 1e-5 of |h_h| (relative).
 """
 
 import numpy as np
@@ -51,6 +54,26 @@ def test_ragged_fused_matches_oracle_per_template(direct_problem):
     print("fused", ll, "oracle", ll_o, flush=True)
     assert np.all(np.isfinite(ll))
     assert np.max(np.abs(ll - ll_o)) < 1e-6
+
+
+def test_ragged_fp32_bulk_matches_oracle(direct_problem):
+    like, olike, p = direct_problem["like"], direct_problem["olike"], direct_problem["p"]
+    plan = like.direct_plan(LENGTHS, MODES)
+    ll64 = like.get_ll_fused(p, LENGTHS, modes=MODES, plan=plan, precision=0)
+    dh64, hh64 = like.d_h.cpu().numpy(), like.h_h.cpu().numpy()
+    ll32 = like.get_ll_fused(p, LENGTHS, modes=MODES, plan=plan, precision=1)
+    dh32, hh32 = like.d_h.cpu().numpy(), like.h_h.cpu().numpy()
+    ll_o = np.array([olike.get_ll(p[:, b:b + 1], length=int(L), modes=MODES)[0][0] for b, L in enumerate(LENGTHS)])
+    err = np.abs(ll32 - ll_o)
+    scale = np.maximum(np.abs(hh64.real), 1.0)
+    print("fp32 bulk", ll32, "oracle", ll_o, "|dlogL|", err, "h_h", hh64.real, flush=True)
+    assert np.all(np.isfinite(ll32))
+    assert np.max(err) < 1e-3
+    assert np.max(np.abs(dh32 - dh64) / scale) < 1e-5
+    assert np.max(np.abs(hh32 - hh64) / scale) < 1e-5
+    # back to FP64 on the same plan: identical to the first FP64 call
+    ll64b = like.get_ll_fused(p, LENGTHS, modes=MODES, plan=plan, precision=0)
+    assert np.array_equal(ll64, ll64b)
 
 
 def test_ragged_fused_matches_plain_gpu_path(direct_problem):
