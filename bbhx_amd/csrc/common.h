@@ -147,6 +147,42 @@ __device__ inline cd cexpi(double phase)
 // e^{i theta} for large |theta| (phases of 1e5-1e8 rad)
 __device__ inline cd cexpi_red(double theta) { return cexpi(theta); }
 
+// e^{i theta} for large |theta| by table: theta reduced modulo 2 pi, then
+// r = j pi/64 + b with integer |j| <= 64 and |b| <= pi/128; e^{i j pi/64} from a
+// 129-entry table (built by fill_etab, normally in LDS) and e^{i b} from its
+// series to b^7 / b^6 (truncation < 1e-18 relative): ~24 operations against
+// ~35 for sincos_red, within a few ulp of it. Any theta (also NaN or inf)
+// gives an index in [0, 128].
+constexpr double PI64_HI = 0.04908738521234052;     // pi / 64 rounded
+constexpr double PI64_LO = 1.9135106236677393e-18;  // pi / 64 - PI64_HI
+__device__ inline cd cexpi_tab(double theta, const double2* __restrict__ tab)
+{
+    const double r = reduce_2pi(theta);
+    const double j = rint(r * 20.371832715762604);  // 64 / pi
+    double b = fma(-j, PI64_HI, r);
+    b = fma(-j, PI64_LO, b);
+    const double2 e = tab[(int)j + 64];
+    const double b2 = b * b;
+    double ps = fma(b2, -1.0 / 5040.0, 1.0 / 120.0);
+    ps = fma(b2, ps, -1.0 / 6.0);
+    const double sb = fma(b * b2, ps, b);
+    double pc = fma(b2, -1.0 / 720.0, 1.0 / 24.0);
+    pc = fma(b2, pc, -0.5);
+    const double cb = fma(b2, pc, 1.0);
+    return cd{e.x * cb - e.y * sb, e.y * cb + e.x * sb};
+}
+
+// The table of cexpi_tab: tab[j] = (cos, sin)((j - 64) pi / 64), j = 0..128,
+// filled by threads tid = 0..nt-1 of a workgroup (a barrier must follow).
+__device__ inline void fill_etab(double2* tab, int tid, int nt)
+{
+    for (int j = tid; j < 129; j += nt) {
+        double sn, cs;
+        sincos_small(fma((double)(j - 64), PI64_HI, (double)(j - 64) * PI64_LO), &sn, &cs);
+        tab[j] = make_double2(cs, sn);
+    }
+}
+
 // i * a
 __host__ __device__ inline cd mul_i(cd a) { return cd{-a.im, a.re}; }
 

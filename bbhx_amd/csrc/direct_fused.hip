@@ -79,6 +79,7 @@ namespace {
 
 constexpr int NQ = 9;
 
+
 // one spline segment of one quantity: value at the knot and c1, c2, c3
 struct alignas(32) Cubic {
     double y, c1, c2, c3;
@@ -528,7 +529,7 @@ __global__ void k_dir_perm(int* __restrict__ perm, int* __restrict__ cnt, const 
 // h[ch] += conj(T_ch A e^{i phase}) of one mode at x = f - x_i into its segment, if
 // t_start <= tf <= t_end (t_end <= 0: no upper limit) (WaveformBuild.cu:48-60, :80-143)
 template <class Q>
-__device__ inline void add_mode(cd h[3], const Q& q, double x, double ts, double te)
+__device__ inline void add_mode(cd h[3], const Q& q, double x, double ts, double te, const double2* tab)
 {
     const double x2 = x * x, x3 = x * x2;
     auto ev = [&](int k) {
@@ -537,7 +538,7 @@ __device__ inline void add_mode(cd h[3], const Q& q, double x, double ts, double
     };
     const double tf = ev(2);
     if ((tf >= ts) && ((tf <= te) || (te <= 0.0))) {
-        const cd ap = ev(0) * cexpi_red(ev(1));
+        const cd ap = ev(0) * cexpi_tab(ev(1), tab);
         h[0] += conj(mk(ev(3), ev(4)) * ap);
         h[1] += conj(mk(ev(5), ev(6)) * ap);
         h[2] += conj(mk(ev(7), ev(8)) * ap);
@@ -550,7 +551,7 @@ __device__ inline void add_mode(cd h[3], const Q& q, double x, double ts, double
 // all nine cubics in one round of loads; the index then moves until
 // x_lo <= f < x_hi or leaves [0, L - 2] (no contribution, WaveformBuild.cu:92-93).
 __device__ inline void mode_per_lane(cd h[3], const SegRec* __restrict__ rm, int M, int L, double f, int i, double ts,
-                                     double te)
+                                     double te, const double2* tab)
 {
     const SegRec* r = rm + (long)i * M;
     double2 bnd = *reinterpret_cast<const double2*>(r);
@@ -570,7 +571,7 @@ __device__ inline void mode_per_lane(cd h[3], const SegRec* __restrict__ rm, int
 #pragma unroll
         for (int q = 0; q < NQ; q++) cq[q] = r->q[q];
     }
-    add_mode(h, cq, f - bnd.x, ts, te);
+    add_mode(h, cq, f - bnd.x, ts, te, tab);
 }
 
 // e^{i phase} in FP32 from an FP64 phase: the phase in revolutions reduced to
@@ -595,7 +596,7 @@ __device__ inline cf cexpi_f32(double phase, float* corr)
 // add_mode of the FP32-bulk form: t(f) and the phase evaluated in FP64 (the
 // same expressions as the FP64 form, so the window decision is identical),
 // the other seven cubics and the mode sum in FP32.
-__device__ inline void add_mode(cf h[3], const SegRecF& q, double x, double ts, double te)
+__device__ inline void add_mode(cf h[3], const SegRecF& q, double x, double ts, double te, const double2*)
 {
     const double x2 = x * x, x3 = x * x2;
     const double tf = q.tf[0] + q.tf[1] * x + q.tf[2] * x2 + q.tf[3] * x3;
@@ -619,7 +620,7 @@ __device__ inline void add_mode(cf h[3], const SegRecF& q, double x, double ts, 
 
 // mode_per_lane of the FP32-bulk form: the whole 192-B record in one round of loads
 __device__ inline void mode_per_lane(cf h[3], const SegRecF* __restrict__ rm, int M, int L, double f, int i,
-                                     double ts, double te)
+                                     double ts, double te, const double2*)
 {
     SegRecF rec = rm[(long)i * M];
     // keeps the compiler from sinking these loads below the bounds check
@@ -634,7 +635,7 @@ __device__ inline void mode_per_lane(cf h[3], const SegRecF* __restrict__ rm, in
         if (!(bnd.x <= f && f < bnd.y)) return;
         rec = rm[(long)i * M];
     }
-    add_mode(h, rec, f - rec.x_lo, ts, te);
+    add_mode(h, rec, f - rec.x_lo, ts, te, nullptr);
 }
 
 constexpr int RB = 2;  // iterations of 64 dense bins per block of a wavefront
@@ -655,7 +656,8 @@ __device__ inline int adj_q(int j) { return j < 2 ? j : j + 1; }
 // Lam_c = 4 w_c (d~_c - h~_c) (d logL = Re sum_c conj(Lam_c) dh_c with
 // h_c = sum_m conj(T_c A e^{i phase}); Likelihood.cu:357-375 differentiated).
 template <class Q>
-__device__ inline bool mode_adjoint(double cb[32], const Q& q, double x, double ts, double te, const cd Lam[3])
+__device__ inline bool mode_adjoint(double cb[32], const Q& q, double x, double ts, double te, const cd Lam[3],
+                                    const double2* tab)
 {
     const double x2 = x * x, x3 = x * x2;
     auto ev = [&](int k) {
@@ -665,7 +667,7 @@ __device__ inline bool mode_adjoint(double cb[32], const Q& q, double x, double 
     const double tf = ev(2);
     if (!((tf >= ts) && ((tf <= te) || (te <= 0.0)))) return false;
     const double A = ev(0);
-    const cd E = cexpi_red(ev(1));
+    const cd E = cexpi_tab(ev(1), tab);
     const cd AE = A * E;
     double a[8];
     cd P = mk(0.0, 0.0);
@@ -720,14 +722,15 @@ __device__ inline double reduce32(double v[32], int lane)
 // adjoint of one lane's bin for one mode, segment searched by the lane (as
 // mode_per_lane), added to the segment's adjoint record
 __device__ inline void mode_adjoint_per_lane(double* __restrict__ Ym, const SegRec* __restrict__ rm, int M, int L,
-                                             double f, int i, double ts, double te, const cd Lam[3])
+                                             double f, int i, double ts, double te, const cd Lam[3],
+                                             const double2* tab)
 {
     double2 bnd = *reinterpret_cast<const double2*>(rm + (long)i * M);
     while (f < bnd.x && i > 0) bnd = *reinterpret_cast<const double2*>(rm + (long)--i * M);
     while (f >= bnd.y && i < L - 2) bnd = *reinterpret_cast<const double2*>(rm + (long)++i * M);
     if (!(bnd.x <= f && f < bnd.y)) return;
     double cb[32];
-    if (!mode_adjoint(cb, *(rm + (long)i * M), f - bnd.x, ts, te, Lam)) return;
+    if (!mode_adjoint(cb, *(rm + (long)i * M), f - bnd.x, ts, te, Lam, tab)) return;
     double* y = Ym + (long)i * M * NA;
 #pragma unroll
     for (int j = 0; j < 32; j++) unsafeAtomicAdd(y + adj_q(j >> 2) * 4 + (j & 3), cb[j]);
@@ -753,8 +756,11 @@ __device__ inline void mode_adjoint_per_lane(double* __restrict__ Ym, const SegR
 // mode sum of a block, Lam = 4 w (d~ - h~) per bin and channel, then the modes
 // again, each bin's coefficient adjoints (mode_adjoint) summed over the lanes
 // that share a segment (reduce32) and added to the segment's adjoint record.
+// at least 4 wavefronts per SIMD (at most 128 VGPRs): measured faster than the
+// 3 that the unconstrained register allocation gives, despite a few spilled
+// registers in the rarely taken per-lane path
 template <bool ADJ, class Rec>
-__global__ __launch_bounds__(256) void k_dir_like_t(double* __restrict__ partial, double* __restrict__ Yadj,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_dir_like_t(double* __restrict__ partial, double* __restrict__ Yadj,
                                                     const int* __restrict__ band,
                                                   const int* __restrict__ total, const int* __restrict__ perm,
                                                   int* __restrict__ next, const Rec* __restrict__ R,
@@ -770,8 +776,10 @@ __global__ __launch_bounds__(256) void k_dir_like_t(double* __restrict__ partial
     using HT = typename std::conditional<std::is_same<Rec, SegRecF>::value, cf, cd>::type;
     __shared__ double red[3][256 / 64];
     __shared__ int s_next;
+    __shared__ double2 s_tab[129];
     const int nitems = total[0];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nw = blockDim.x >> 6;
+    if constexpr (!std::is_same<Rec, SegRecF>::value) fill_etab(s_tab, tid, blockDim.x);  // before the first barrier
     for (;;) {
         // the next item in the chunk-major order (dynamic: the items running at
         // any time stay a window of consecutive entries, i.e. the same chunks)
@@ -819,7 +827,7 @@ __global__ __launch_bounds__(256) void k_dir_like_t(double* __restrict__ partial
                     const int e_first = __builtin_amdgcn_readlane(est, __builtin_ctzll(pm));
                     const int e_last = __builtin_amdgcn_readlane(est, 63 - __builtin_clzll(pm));
                     if (e_last - e_first > 2) {
-                        if (pending) mode_per_lane(h[r], rm, M, L, f[r], est, ts, te);
+                        if (pending) mode_per_lane(h[r], rm, M, L, f[r], est, ts, te, s_tab);
                         cur = -1;
                         continue;
                     }
@@ -838,7 +846,7 @@ __global__ __launch_bounds__(256) void k_dir_like_t(double* __restrict__ partial
                             continue;
                         }
                         const bool in = pending && f[r] < xhi;
-                        if (in) add_mode(h[r], *rc, f[r] - xlo, ts, te);
+                        if (in) add_mode(h[r], *rc, f[r] - xlo, ts, te, s_tab);
                         pending = pending && !in;
                         if (__ballot(pending) != 0) {
                             if (cur == L - 2) pending = false;  // right of the grid
@@ -881,7 +889,7 @@ __global__ __launch_bounds__(256) void k_dir_like_t(double* __restrict__ partial
                         const int e_first = __builtin_amdgcn_readlane(est, __builtin_ctzll(pm));
                         const int e_last = __builtin_amdgcn_readlane(est, 63 - __builtin_clzll(pm));
                         if (e_last - e_first > 2) {
-                            if (pending) mode_adjoint_per_lane(Ym, rm, M, L, f[r], est, ts, te, h[r]);
+                            if (pending) mode_adjoint_per_lane(Ym, rm, M, L, f[r], est, ts, te, h[r], s_tab);
                             cur = -1;
                             continue;
                         }
@@ -901,7 +909,7 @@ __global__ __launch_bounds__(256) void k_dir_like_t(double* __restrict__ partial
 #pragma unroll
                             for (int j = 0; j < 32; j++) cb[j] = 0.0;
                             bool used = false;
-                            if (in) used = mode_adjoint(cb, *rc, f[r] - xlo, ts, te, h[r]);
+                            if (in) used = mode_adjoint(cb, *rc, f[r] - xlo, ts, te, h[r], s_tab);
                             if (__ballot(used) != 0) {
                                 const double tot = reduce32(cb, lane);
                                 if ((lane & 1) == 0) {
