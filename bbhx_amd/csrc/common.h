@@ -134,34 +134,22 @@ __device__ inline void sincos_small(double x, double* s, double* c)
     *c = ((k + 1) & 2) ? -cc : cc;
 }
 
-// sin and cos of any moderate or large argument (reduction modulo 2 pi first)
-__device__ inline void sincos_red(double theta, double* s, double* c) { sincos_small(reduce_2pi(theta), s, c); }
-
-__device__ inline cd cexpi(double phase)
-{
-    double s, c;
-    sincos_red(phase, &s, &c);
-    return cd{c, s};
-}
-
-// e^{i theta} for large |theta| (phases of 1e5-1e8 rad)
-__device__ inline cd cexpi_red(double theta) { return cexpi(theta); }
-
 // e^{i theta} for large |theta| by table: theta reduced modulo 2 pi, then
 // r = j pi/64 + b with integer |j| <= 64 and |b| <= pi/128; e^{i j pi/64} from a
 // 129-entry table (built by fill_etab, normally in LDS) and e^{i b} from its
 // series to b^7 / b^6 (truncation < 1e-18 relative): ~24 operations against
-// ~35 for sincos_red, within a few ulp of it. Any theta (also NaN or inf)
+// ~35 for sincos_small after reduce_2pi, within a few ulp of it. Any theta (also NaN or inf)
 // gives an index in [0, 128].
 constexpr double PI64_HI = 0.04908738521234052;     // pi / 64 rounded
 constexpr double PI64_LO = 1.9135106236677393e-18;  // pi / 64 - PI64_HI
-__device__ inline cd cexpi_tab(double theta, const double2* __restrict__ tab)
+template <class E>
+__device__ inline cd cexpi_tab(double theta, const E* __restrict__ tab)
 {
     const double r = reduce_2pi(theta);
     const double j = rint(r * 20.371832715762604);  // 64 / pi
     double b = fma(-j, PI64_HI, r);
     b = fma(-j, PI64_LO, b);
-    const double2 e = tab[(int)j + 64];
+    const E e = tab[(int)j + 64];
     const double b2 = b * b;
     double ps = fma(b2, -1.0 / 5040.0, 1.0 / 120.0);
     ps = fma(b2, ps, -1.0 / 6.0);
@@ -182,6 +170,154 @@ __device__ inline void fill_etab(double2* tab, int tid, int nt)
         tab[j] = make_double2(cs, sn);
     }
 }
+
+// e^{i j pi/64}, j = -64..64 (index j + 64), correctly rounded: the table of
+// cexpi_tab in constant memory for code without an LDS copy.
+struct ETabEntry {
+    double x, y;
+};
+__constant__ const ETabEntry ETAB[129] = {
+    {-1.0, 0.0},
+    {-0.9987954562051724, -0.049067674327418015},
+    {-0.9951847266721969, -0.0980171403295606},
+    {-0.989176509964781, -0.14673047445536175},
+    {-0.9807852804032304, -0.19509032201612828},
+    {-0.970031253194544, -0.2429801799032639},
+    {-0.9569403357322088, -0.2902846772544624},
+    {-0.9415440651830208, -0.33688985339222005},
+    {-0.9238795325112867, -0.3826834323650898},
+    {-0.9039892931234433, -0.4275550934302821},
+    {-0.881921264348355, -0.47139673682599764},
+    {-0.8577286100002721, -0.5141027441932218},
+    {-0.8314696123025452, -0.5555702330196022},
+    {-0.8032075314806449, -0.5956993044924334},
+    {-0.773010453362737, -0.6343932841636455},
+    {-0.7409511253549591, -0.6715589548470184},
+    {-0.7071067811865476, -0.7071067811865476},
+    {-0.6715589548470184, -0.7409511253549591},
+    {-0.6343932841636455, -0.773010453362737},
+    {-0.5956993044924334, -0.8032075314806449},
+    {-0.5555702330196022, -0.8314696123025452},
+    {-0.5141027441932218, -0.8577286100002721},
+    {-0.47139673682599764, -0.881921264348355},
+    {-0.4275550934302821, -0.9039892931234433},
+    {-0.3826834323650898, -0.9238795325112867},
+    {-0.33688985339222005, -0.9415440651830208},
+    {-0.2902846772544624, -0.9569403357322088},
+    {-0.2429801799032639, -0.970031253194544},
+    {-0.19509032201612828, -0.9807852804032304},
+    {-0.14673047445536175, -0.989176509964781},
+    {-0.0980171403295606, -0.9951847266721969},
+    {-0.049067674327418015, -0.9987954562051724},
+    {-1.891501989729776e-58, -1.0},
+    {0.049067674327418015, -0.9987954562051724},
+    {0.0980171403295606, -0.9951847266721969},
+    {0.14673047445536175, -0.989176509964781},
+    {0.19509032201612828, -0.9807852804032304},
+    {0.2429801799032639, -0.970031253194544},
+    {0.2902846772544624, -0.9569403357322088},
+    {0.33688985339222005, -0.9415440651830208},
+    {0.3826834323650898, -0.9238795325112867},
+    {0.4275550934302821, -0.9039892931234433},
+    {0.47139673682599764, -0.881921264348355},
+    {0.5141027441932218, -0.8577286100002721},
+    {0.5555702330196022, -0.8314696123025452},
+    {0.5956993044924334, -0.8032075314806449},
+    {0.6343932841636455, -0.773010453362737},
+    {0.6715589548470184, -0.7409511253549591},
+    {0.7071067811865476, -0.7071067811865476},
+    {0.7409511253549591, -0.6715589548470184},
+    {0.773010453362737, -0.6343932841636455},
+    {0.8032075314806449, -0.5956993044924334},
+    {0.8314696123025452, -0.5555702330196022},
+    {0.8577286100002721, -0.5141027441932218},
+    {0.881921264348355, -0.47139673682599764},
+    {0.9039892931234433, -0.4275550934302821},
+    {0.9238795325112867, -0.3826834323650898},
+    {0.9415440651830208, -0.33688985339222005},
+    {0.9569403357322088, -0.2902846772544624},
+    {0.970031253194544, -0.2429801799032639},
+    {0.9807852804032304, -0.19509032201612828},
+    {0.989176509964781, -0.14673047445536175},
+    {0.9951847266721969, -0.0980171403295606},
+    {0.9987954562051724, -0.049067674327418015},
+    {1.0, 0.0},
+    {0.9987954562051724, 0.049067674327418015},
+    {0.9951847266721969, 0.0980171403295606},
+    {0.989176509964781, 0.14673047445536175},
+    {0.9807852804032304, 0.19509032201612828},
+    {0.970031253194544, 0.2429801799032639},
+    {0.9569403357322088, 0.2902846772544624},
+    {0.9415440651830208, 0.33688985339222005},
+    {0.9238795325112867, 0.3826834323650898},
+    {0.9039892931234433, 0.4275550934302821},
+    {0.881921264348355, 0.47139673682599764},
+    {0.8577286100002721, 0.5141027441932218},
+    {0.8314696123025452, 0.5555702330196022},
+    {0.8032075314806449, 0.5956993044924334},
+    {0.773010453362737, 0.6343932841636455},
+    {0.7409511253549591, 0.6715589548470184},
+    {0.7071067811865476, 0.7071067811865476},
+    {0.6715589548470184, 0.7409511253549591},
+    {0.6343932841636455, 0.773010453362737},
+    {0.5956993044924334, 0.8032075314806449},
+    {0.5555702330196022, 0.8314696123025452},
+    {0.5141027441932218, 0.8577286100002721},
+    {0.47139673682599764, 0.881921264348355},
+    {0.4275550934302821, 0.9039892931234433},
+    {0.3826834323650898, 0.9238795325112867},
+    {0.33688985339222005, 0.9415440651830208},
+    {0.2902846772544624, 0.9569403357322088},
+    {0.2429801799032639, 0.970031253194544},
+    {0.19509032201612828, 0.9807852804032304},
+    {0.14673047445536175, 0.989176509964781},
+    {0.0980171403295606, 0.9951847266721969},
+    {0.049067674327418015, 0.9987954562051724},
+    {-1.891501989729776e-58, 1.0},
+    {-0.049067674327418015, 0.9987954562051724},
+    {-0.0980171403295606, 0.9951847266721969},
+    {-0.14673047445536175, 0.989176509964781},
+    {-0.19509032201612828, 0.9807852804032304},
+    {-0.2429801799032639, 0.970031253194544},
+    {-0.2902846772544624, 0.9569403357322088},
+    {-0.33688985339222005, 0.9415440651830208},
+    {-0.3826834323650898, 0.9238795325112867},
+    {-0.4275550934302821, 0.9039892931234433},
+    {-0.47139673682599764, 0.881921264348355},
+    {-0.5141027441932218, 0.8577286100002721},
+    {-0.5555702330196022, 0.8314696123025452},
+    {-0.5956993044924334, 0.8032075314806449},
+    {-0.6343932841636455, 0.773010453362737},
+    {-0.6715589548470184, 0.7409511253549591},
+    {-0.7071067811865476, 0.7071067811865476},
+    {-0.7409511253549591, 0.6715589548470184},
+    {-0.773010453362737, 0.6343932841636455},
+    {-0.8032075314806449, 0.5956993044924334},
+    {-0.8314696123025452, 0.5555702330196022},
+    {-0.8577286100002721, 0.5141027441932218},
+    {-0.881921264348355, 0.47139673682599764},
+    {-0.9039892931234433, 0.4275550934302821},
+    {-0.9238795325112867, 0.3826834323650898},
+    {-0.9415440651830208, 0.33688985339222005},
+    {-0.9569403357322088, 0.2902846772544624},
+    {-0.970031253194544, 0.2429801799032639},
+    {-0.9807852804032304, 0.19509032201612828},
+    {-0.989176509964781, 0.14673047445536175},
+    {-0.9951847266721969, 0.0980171403295606},
+    {-0.9987954562051724, 0.049067674327418015},
+    {-1.0, 0.0},
+};
+
+// e^{i theta} and sin, cos of any moderate or large argument (phases of
+// 1e5-1e8 rad) by cexpi_tab on the constant table
+__device__ inline cd cexpi_red(double theta) { return cexpi_tab(theta, ETAB); }
+__device__ inline void sincos_red(double theta, double* s, double* c)
+{
+    const cd e = cexpi_tab(theta, ETAB);
+    *s = e.im;
+    *c = e.re;
+}
+__device__ inline cd cexpi(double phase) { return cexpi_tab(phase, ETAB); }
 
 // i * a
 __host__ __device__ inline cd mul_i(cd a) { return cd{-a.im, a.re}; }
