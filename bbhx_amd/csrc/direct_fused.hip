@@ -123,10 +123,7 @@ __device__ inline const Cubic& cubic_of(const Cubic (&c)[NQ], int k) { return c[
 __global__ __launch_bounds__(64) void k_dir_setup(FusedRec* __restrict__ recs, const double* __restrict__ params, int B, ModeList modes,
                             int M, const double* __restrict__ qnm, int table_len, double dspin)
 {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= B * M) return;
-    const int b = idx / M, mi = idx % M;
-    setup_one(recs[idx], params, B, b, modes.ell[mi], modes.mm[mi], qnm, table_len, dspin);
+    setup_split(recs, params, B, modes, M, qnm, table_len, dspin);
 }
 
 // numpy.logspace point i of num between 10^lo and 10^hi (start + i step, the
@@ -1288,7 +1285,8 @@ int direct_launch(bbhx_direct_plan* p, double* d_h, double* h_h, double* grad, i
     hipEvent_t ev[3];
     const bool timed = timer_take3(ev);
     if (timed) (void)hipEventRecord(ev[0], st);
-    hipLaunchKernelGGL(k_dir_setup, dim3(ceil_div((long)B * M, 64)), dim3(64), 0, st, p->recs, params, B, ml, M, qnm,
+    hipLaunchKernelGGL(k_dir_setup, dim3(SETUP_PARTS * ceil_div((long)B * M, 64)), dim3(64), 0, st, p->recs, params, B,
+                       ml, M, qnm,
                        table_len, dspin);
     hipLaunchKernelGGL(k_dir_grid, dim3(B * M), dim3(64), sizeof(double) * (size_t)p->Lmax, st, p->X, p->Cf,
                        p->grid_lo, p->grid_istep, flags, p->recs, p->off, p->len, params, t_start, t_end, B, M, ml,

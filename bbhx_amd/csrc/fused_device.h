@@ -44,20 +44,50 @@ __device__ inline void phenomd_ringdown(S finmass, S finspin, const double* __re
          finmass;
 }
 
-// Record of template b, mode mi (params: (11, B) rows).
-__device__ inline void setup_one(FusedRec& r, const double* __restrict__ params, int B, int b, int l, int m,
-                                 const double* __restrict__ qnm, int table_len, double dspin)
+// Record of template b (params: (11, B) rows), mode (l, m), in parts.
+// One of the three independent dependency chains of a record's set-up (they
+// write disjoint fields of the record): PART 0, the mode-dependent PhenomHM
+// coefficients; PART 1, the (2,2) reference (PhenomD amplitude, Mf_ref, phi0,
+// t0); PART 2, the response record. A set-up kernel gives each part its own
+// wavefronts (a branch inside one wavefront would run them one after another),
+// so the set-up takes the longest chain instead of their sum.
+constexpr int SETUP_PARTS = 3;
+
+template <int PART>
+__device__ inline void setup_part(FusedRec& r, const double* __restrict__ params, int B, int b, int l, int m,
+                                  const double* __restrict__ qnm, int table_len, double dspin)
 {
-    const double m1 = params[P_M1 * B + b], m2 = params[P_M2 * B + b];
-    const double c1 = params[P_CHI1 * B + b], c2 = params[P_CHI2 * B + b];
-    double finmass, finspin;
-    phm::final_state(m1, m2, c1, c2, finmass, finspin);
-    double rdD, dmD, rd, dm;
-    phenomd_ringdown(finmass, finspin, qnm, table_len, dspin, rdD, dmD);
-    phm::hm_ringdown(l, m, finmass, finspin, rd, dm);
-    phm::phm_setup(r.w, m1 * MSUN_SI, m2 * MSUN_SI, c1, c2, params[P_DIST * B + b], l, m, rd, dm, rdD, dmD, false);
-    resp::resp_setup(r.s, l, m, params[P_PHI * B + b], params[P_INC * B + b], params[P_LAM * B + b],
-                     params[P_BETA * B + b], params[P_PSI * B + b]);
+    if constexpr (PART == 2) {
+        resp::resp_setup(r.s, l, m, params[P_PHI * B + b], params[P_INC * B + b], params[P_LAM * B + b],
+                         params[P_BETA * B + b], params[P_PSI * B + b]);
+    } else {
+        const double m1 = params[P_M1 * B + b], m2 = params[P_M2 * B + b];
+        const double c1 = params[P_CHI1 * B + b], c2 = params[P_CHI2 * B + b];
+        double finmass, finspin;
+        phm::final_state(m1, m2, c1, c2, finmass, finspin);
+        double rdD, dmD, rd = 0.0, dm = 0.0;
+        phenomd_ringdown(finmass, finspin, qnm, table_len, dspin, rdD, dmD);
+        if constexpr (PART == 0) phm::hm_ringdown(l, m, finmass, finspin, rd, dm);
+        phm::phm_setup<double, PART>(r.w, m1 * MSUN_SI, m2 * MSUN_SI, c1, c2, params[P_DIST * B + b], l, m, rd,
+                                     dm, rdD, dmD, false);
+    }
+}
+
+// Body of a set-up kernel of 64-thread workgroups over SETUP_PARTS * nb blocks,
+// nb = ceil(B * M / 64): block k does part k / nb of records 64 (k % nb) + tid.
+__device__ inline void setup_split(FusedRec* __restrict__ recs, const double* __restrict__ params, int B,
+                                   const ModeList& modes, int M, const double* __restrict__ qnm, int table_len,
+                                   double dspin)
+{
+    const int nb = (B * M + 63) / 64;
+    const int part = blockIdx.x / nb;
+    const int idx = (blockIdx.x - part * nb) * 64 + threadIdx.x;
+    if (idx >= B * M) return;
+    const int b = idx / M, mi = idx % M;
+    const int l = modes.ell[mi], m = modes.mm[mi];
+    if (part == 0) setup_part<0>(recs[idx], params, B, b, l, m, qnm, table_len, dspin);
+    else if (part == 1) setup_part<1>(recs[idx], params, B, b, l, m, qnm, table_len, dspin);
+    else setup_part<2>(recs[idx], params, B, b, l, m, qnm, table_len, dspin);
 }
 
 __device__ inline double wsum(double v)

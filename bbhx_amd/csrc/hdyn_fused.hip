@@ -36,10 +36,7 @@ namespace {
 __global__ __launch_bounds__(64) void k_fused_setup(FusedRec* __restrict__ recs, const double* __restrict__ params, int B, ModeList modes,
                               int M, const double* __restrict__ qnm, int table_len, double dspin)
 {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= B * M) return;
-    const int b = idx / M, mi = idx % M;
-    setup_one(recs[idx], params, B, b, modes.ell[mi], modes.mm[mi], qnm, table_len, dspin);
+    setup_split(recs, params, B, modes, M, qnm, table_len, dspin);
 }
 
 // One 64-lane wavefront (one workgroup) per template. Dynamic LDS holds the
@@ -176,7 +173,7 @@ extern "C" int bbhx_hip_hdyn_fused(double* d_h, double* h_h, const double* param
     const bool timed = timer_take3(ev);
     if (g_timer.on && !timed) return ERR_LAUNCH;
     if (timed) (void)hipEventRecord(ev[0], st);
-    hipLaunchKernelGGL(k_fused_setup, dim3(ceil_div(nrec, 64)), dim3(64), 0, st, recs, params, numBinAll, ml,
+    hipLaunchKernelGGL(k_fused_setup, dim3(SETUP_PARTS * ceil_div(nrec, 64)), dim3(64), 0, st, recs, params, numBinAll, ml,
                        numModes, qnm, table_len, dspin);
     if (hipGetLastError() != hipSuccess) return ERR_LAUNCH;
     if (timed) (void)hipEventRecord(ev[1], st);

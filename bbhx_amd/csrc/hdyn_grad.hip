@@ -53,10 +53,7 @@ __global__ __launch_bounds__(64) void k_grad_setup_fwd(FusedRec* __restrict__ re
                                                        int B, ModeList modes, int M, const double* __restrict__ qnm,
                                                        int table_len, double dspin)
 {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= B * M) return;
-    const int b = idx / M, mi = idx % M;
-    setup_one(recs[idx], params, B, b, modes.ell[mi], modes.mm[mi], qnm, table_len, dspin);
+    setup_split(recs, params, B, modes, M, qnm, table_len, dspin);
 }
 
 // the records with tangents, one lane per (template, mode)
@@ -265,7 +262,8 @@ extern "C" int bbhx_hip_hdyn_fused_grad(double* d_h, double* h_h, double* grad, 
     hipLaunchKernelGGL(k_grad_setup, dim3(ceil_div((long)nrec, 64)), dim3(64), 0, side->stream, grecs, params,
                        numBinAll, ml, numModes, qnm, table_len, dspin);
     if (hipEventRecord(side->join, side->stream) != hipSuccess) return ERR_LAUNCH;
-    hipLaunchKernelGGL(k_grad_setup_fwd, dim3(ceil_div((long)nrec, 64)), dim3(64), 0, st, recs, params, numBinAll, ml,
+    hipLaunchKernelGGL(k_grad_setup_fwd, dim3(SETUP_PARTS * ceil_div((long)nrec, 64)), dim3(64), 0, st, recs, params,
+                       numBinAll, ml,
                        numModes, qnm, table_len, dspin);
     if (hipGetLastError() != hipSuccess) return ERR_LAUNCH;
     if (timed) (void)hipEventRecord(ev[1], st);

@@ -714,17 +714,38 @@ __device__ inline void hm_map(S Rholm, S Mf_RD_lm, S Mf_RD_22, int mm, nd<S> flm
     }
 }
 
-// Full per-(binary, mode) setup. Masses in solar masses (m1_SI/MSUN), distance in m.
+// Per-(binary, mode) setup. Masses in solar masses (m1_SI/MSUN), distance in m.
 // Mf_RD_lm/Mf_DM_lm: this mode's QNM frequencies; Mf_RD_D/Mf_DM_D: the PhenomD
 // (2,2) ringdown frequencies from the QNM table (IMRPhenomHMCore, PhenomHMWaveform.cu:2671-2844).
 // Returns false for an unsupported mode.
-template <class S>
+// PART -1: the whole record. The record has two independent dependency chains,
+// which a caller can run in different wavefronts (they write disjoint fields):
+// PART 1, the (2,2) reference: the PhenomD amplitude coefficients and Mf_ref,
+// phi0, t0 from a PhenomD phase with unit ringdown ratios (set up in a local
+// record); PART 0, everything else (the phase with this mode's ratios, the
+// maps, the PN amplitude inputs, the scalings).
+template <class S, int PART = -1>
 __device__ inline bool phm_setup(PhmRecT<S>& r, nd<S> m1_SI, nd<S> m2_SI, nd<S> chi1_in, nd<S> chi2_in,
                                  nd<S> distance, int ell, int mm, nd<S> Mf_RD_lm, nd<S> Mf_DM_lm,
                                  nd<S> Mf_RD_D, nd<S> Mf_DM_D, bool phenomd)
 {
     S m1 = m1_SI / MSUN_SI, m2 = m2_SI / MSUN_SI, chi1 = chi1_in, chi2 = chi2_in;
     enforce_primary(m1, m2, chi1, chi2);
+    if constexpr (PART == 1) {
+        PhmRecT<S> t;
+        setup_phenomd(t, m1, m2, chi1, chi2, 1.0, 1.0, Mf_RD_D, Mf_DM_D, true);
+        S ph22, dph22;
+        phase_imr(t, t.a_fmax, ph22, &dph22);
+        r.Mf_ref = t.a_fmax;
+        r.phi0 = 0.5 * (ph22 + 0.0);
+        r.t0 = dph22;
+        r.a_fRD = t.a_fRD; r.a_fDM = t.a_fDM; r.a_g1 = t.a_g1; r.a_g2 = t.a_g2; r.a_g3 = t.a_g3;
+        r.a_fmax = t.a_fmax; r.a_rho1 = t.a_rho1; r.a_rho2 = t.a_rho2; r.a_rho3 = t.a_rho3;
+        r.a_d0 = t.a_d0; r.a_d1 = t.a_d1; r.a_d2 = t.a_d2; r.a_d3 = t.a_d3; r.a_d4 = t.a_d4;
+        r.a_pf23 = t.a_pf23; r.a_pf1 = t.a_pf1; r.a_pf43 = t.a_pf43; r.a_pf53 = t.a_pf53; r.a_pf2 = t.a_pf2;
+        r.a_amp0 = t.a_amp0;
+        return true;
+    }
     S finmass, finspin;
     final_state(m1, m2, chi1, chi2, finmass, finspin);
     S Mf_RD_22, Mf_DM_22;
@@ -743,12 +764,14 @@ __device__ inline bool phm_setup(PhmRecT<S>& r, nd<S> m1_SI, nd<S> m2_SI, nd<S> 
         ok = false;
 
     // (2,2) PhenomD: f_ref -> amplitude peak, phi0 and t0 there (PhenomHMWaveform.cu:2752-2784)
-    setup_phenomd(r, m1, m2, chi1, chi2, 1.0, 1.0, Mf_RD_D, Mf_DM_D, true);
-    r.Mf_ref = r.a_fmax;
-    S ph22, dph22;
-    phase_imr(r, r.Mf_ref, ph22, &dph22);
-    r.phi0 = 0.5 * (ph22 + 0.0);
-    r.t0 = dph22;
+    if constexpr (PART == -1) {
+        setup_phenomd(r, m1, m2, chi1, chi2, 1.0, 1.0, Mf_RD_D, Mf_DM_D, true);
+        r.Mf_ref = r.a_fmax;
+        S ph22, dph22;
+        phase_imr(r, r.Mf_ref, ph22, &dph22);
+        r.phi0 = 0.5 * (ph22 + 0.0);
+        r.t0 = dph22;
+    }
     const S Mtot = (m1_SI + m2_SI) / MSUN_SI;
     r.amp_scale = Mtot * MRSUN_SI * Mtot * MTSUN_SI / distance;
     r.M_sec = (m1 + m2) * MTSUN_SI;
