@@ -96,6 +96,8 @@ __device__ inline double wsum(double v)
     return v;
 }
 
+constexpr int MAX_RESIDUAL_WAVES = 8;
+
 struct Window {
     double t_ref, ts, te, dt_shift;
 };
@@ -105,15 +107,14 @@ __device__ inline bool in_window(const Window& w, double tf)
     return (tf >= w.ts) && ((tf <= w.te) || (w.te <= 0.0));
 }
 
-// Residual pass of one 64-lane wavefront: lane l owns nodes l, l+64, ...; for
-// each node the three channels are summed over the modes and r = h / h0 is
-// written to rs[c * K + j] (LDS).
+// Residual pass of one template by one workgroup: thread t owns nodes t,
+// t + blockDim.x, ...; for each node the three channels are summed over the
+// modes (in mode order) and r = h / h0 is written to rs[c * K + j] (LDS).
 __device__ inline void residuals(cd* rs, const FusedRec* __restrict__ rb, int M, const Window& win,
                                  const double* __restrict__ freqs, const cd* __restrict__ h0, int K,
                                  const resp::OrbitTab& orb, int TDItag, int rescaled, int tdi2)
 {
-    const int lane = threadIdx.x;
-    for (int j = lane; j < K; j += 64) {
+    for (int j = threadIdx.x; j < K; j += blockDim.x) {
         const double f = freqs[j];
         cd h[3] = {mk(0.0, 0.0), mk(0.0, 0.0), mk(0.0, 0.0)};
         for (int mi = 0; mi < M; mi++) {
@@ -143,15 +144,16 @@ __device__ inline void residuals(cd* rs, const FusedRec* __restrict__ rb, int M,
     }
 }
 
-// Heterodyned sums (Likelihood.cu:52-111) over the segments j = lane+1,
-// lane+65, ... < jend; segment j joins nodes j-1 and j. Returns the wavefront
-// totals (d_h re, im, h_h re, im) in every lane.
+// Heterodyned sums (Likelihood.cu:52-111) over the segments j = t+1,
+// t+1 + blockDim.x, ... < jend of thread t; segment j joins nodes j-1 and j.
+// Returns the workgroup totals (d_h re, im, h_h re, im) in every thread: each
+// wavefront reduces its lanes, then the (at most 8) wavefront totals are added
+// in wavefront order.
 __device__ inline void segment_sums(const cd* rs, const double* __restrict__ freqs, const cd* __restrict__ dc, int K,
                                     int jend, double out[4])
 {
-    const int lane = threadIdx.x;
     double l1r = 0.0, l1i = 0.0, l2r = 0.0, l2i = 0.0;
-    for (int j = lane + 1; j < jend; j += 64) {
+    for (int j = threadIdx.x + 1; j < jend; j += blockDim.x) {
         const double fj = freqs[j], fprev = freqs[j - 1];
         if (fprev == 0.0) continue;
         const double inv = rcp_pos(fj - fprev);  // nodes increase
@@ -176,6 +178,31 @@ __device__ inline void segment_sums(const cd* rs, const double* __restrict__ fre
     out[1] = wsum(l1i);
     out[2] = wsum(l2r);
     out[3] = wsum(l2i);
+    const int nw = blockDim.x >> 6;
+    if (nw > 1) {
+        __shared__ double part[4][MAX_RESIDUAL_WAVES];
+        if ((threadIdx.x & 63) == 0)
+            for (int k = 0; k < 4; k++) part[k][threadIdx.x >> 6] = out[k];
+        __syncthreads();
+        for (int k = 0; k < 4; k++) {
+            double s = part[k][0];
+            for (int i = 1; i < nw; i++) s += part[k][i];
+            out[k] = s;
+        }
+    }
+}
+
+// Wavefronts per template of the residual pass (workgroup of 64 x that many
+// threads): enough for about 2048 wavefronts in the launch (two per SIMD of
+// MI355X's 1024), at most one per 64 nodes and at most MAX_RESIDUAL_WAVES.
+// The forward and the gradient's weights pass use the same count, so their
+// sums agree bit for bit.
+inline int residual_waves(int B, int K)
+{
+    int nw = (2048 + B - 1) / B;
+    nw = nw < (K + 63) / 64 ? nw : (K + 63) / 64;
+    nw = nw < MAX_RESIDUAL_WAVES ? nw : MAX_RESIDUAL_WAVES;
+    return nw > 1 ? nw : 1;
 }
 
 // Host helpers shared by the fused entry points (hdyn_fused.hip): three timing

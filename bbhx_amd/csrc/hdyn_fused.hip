@@ -39,12 +39,14 @@ __global__ __launch_bounds__(64) void k_fused_setup(FusedRec* __restrict__ recs,
     setup_split(recs, params, B, modes, M, qnm, table_len, dspin);
 }
 
-// One 64-lane wavefront (one workgroup) per template. Dynamic LDS holds the
-// residuals r[3][K] of this template. The register budget is capped so that two
-// wavefronts share each SIMD (256 VGPRs each; a few values spill to scratch):
+// One workgroup per template, of residual_waves(B, K) wavefronts (one at large
+// batches; several at small ones, so that the chip still holds two per SIMD).
+// Dynamic LDS holds the residuals r[3][K] of this template. The register budget
+// is capped so that two wavefronts share each SIMD (256 VGPRs each; a few
+// values spill to scratch):
 // measured on MI355X at 4096 templates x 128 nodes, 0.23 ms against 0.30 ms for
 // one wavefront per SIMD without spills, and 0.38 ms for three per SIMD.
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_fused_eval(
+__global__ __launch_bounds__(64 * MAX_RESIDUAL_WAVES) __attribute__((amdgpu_waves_per_eu(2))) void k_fused_eval(
     double* __restrict__ d_h, double* __restrict__ h_h, const FusedRec* __restrict__ recs,
     const double* __restrict__ params, const double* __restrict__ t_start, const double* __restrict__ t_end, int B,
     int M, const double* __restrict__ freqs, const cd* __restrict__ h0, const cd* __restrict__ dc, int K,
@@ -179,7 +181,8 @@ extern "C" int bbhx_hip_hdyn_fused(double* d_h, double* h_h, const double* param
     if (timed) (void)hipEventRecord(ev[1], st);
     const int jend = include_last_segment ? K : K - 1;
     const size_t lds = sizeof(cd) * 3 * (size_t)K;
-    hipLaunchKernelGGL(k_fused_eval, dim3(numBinAll), dim3(64), lds, st, d_h, h_h, recs, params, t_start, t_end,
+    hipLaunchKernelGGL(k_fused_eval, dim3(numBinAll), dim3(64 * residual_waves(numBinAll, K)), lds, st, d_h, h_h,
+                       recs, params, t_start, t_end,
                        numBinAll, numModes, freqs, reinterpret_cast<const cd*>(h0),
                        reinterpret_cast<const cd*>(dataConstants), K, orb, TDItag, rescaled, tdi2, jend,
                        initial_t_val);

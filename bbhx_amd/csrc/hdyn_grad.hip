@@ -13,7 +13,7 @@
 //                (ln M, q, chi1, chi2) (Dual<4> instantiation of the set-up
 //                code), plus the polarisation basis, wave vector and Y factors
 //                with their derivatives in lam, beta (analytic) and cos inc.
-// k_grad_weights one 64-lane wavefront per template:
+// k_grad_weights one workgroup per template (as the forward kernel's):
 //   1. forward: residuals r = h / h0 on the K nodes (LDS) and d_h, h_h,
 //      exactly as the forward kernel;
 //   2. adjoint of the heterodyned sums: for each node and channel the complex
@@ -73,7 +73,7 @@ __global__ __launch_bounds__(64) void k_grad_setup(GradRec* __restrict__ grecs, 
 __device__ inline void adjoint_weights(cd* gs, const cd* rs, const double* __restrict__ freqs,
                                        const cd* __restrict__ h0, const cd* __restrict__ dc, int K, int jend)
 {
-    for (int j = threadIdx.x; j < K; j += 64) {
+    for (int j = threadIdx.x; j < K; j += blockDim.x) {
         cd gam[3] = {mk(0.0, 0.0), mk(0.0, 0.0), mk(0.0, 0.0)};
         // segment s joins prev = s-1 and cur = s; node j is cur of s = j and prev of s = j+1
         for (int side = 0; side < 2; side++) {
@@ -128,7 +128,7 @@ __device__ inline void grad_point(const GradRec& G, const resp::OrbitTab& o, dou
 // Forward pass and adjoint weights of one template (one wavefront): d_h, h_h
 // as the forward kernel computes them, and gw[(b * 3 + c) * K + j] = the
 // complex weight g of node j, channel c.
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_grad_weights(
+__global__ __launch_bounds__(64 * MAX_RESIDUAL_WAVES) __attribute__((amdgpu_waves_per_eu(2))) void k_grad_weights(
     double* __restrict__ d_h, double* __restrict__ h_h, cd* __restrict__ gw, const FusedRec* __restrict__ recs,
     const double* __restrict__ params, const double* __restrict__ t_start, const double* __restrict__ t_end, int B,
     int M, const double* __restrict__ freqs, const cd* __restrict__ h0, const cd* __restrict__ dc, int K,
@@ -268,7 +268,8 @@ extern "C" int bbhx_hip_hdyn_fused_grad(double* d_h, double* h_h, double* grad, 
     if (hipGetLastError() != hipSuccess) return ERR_LAUNCH;
     if (timed) (void)hipEventRecord(ev[1], st);
     const int jend = include_last_segment ? K : K - 1;
-    hipLaunchKernelGGL(k_grad_weights, dim3(numBinAll), dim3(64), sizeof(cd) * 3 * (size_t)K, st, d_h, h_h, gw, recs,
+    hipLaunchKernelGGL(k_grad_weights, dim3(numBinAll), dim3(64 * residual_waves(numBinAll, K)),
+                       sizeof(cd) * 3 * (size_t)K, st, d_h, h_h, gw, recs,
                        params, t_start, t_end, numBinAll, numModes, freqs, reinterpret_cast<const cd*>(h0),
                        reinterpret_cast<const cd*>(dataConstants), K, orb, TDItag, rescaled, tdi2, jend,
                        initial_t_val);
