@@ -1,0 +1,172 @@
+"""Size-independent properties at the benchmark shapes, where the FP64 CPU
+oracle would take minutes, and the extreme sparse-grid lengths against the
+oracle.
+
+Heterodyned, 4096 templates, K = 128 nodes, 2^20 dense bins (the heterodyned
+benchmark shape):
+  * a template equal to the reference has r = h / h0 = 1 at every node, so
+    d_h = sum_j sum_c A0[c, j] and h_h = sum_j sum_c B0[c, j] over the
+    heterodyne segments (Likelihood.cu:91-103 with r0 = 1, r1 = 0); h and h0
+    come from different kernels (fused vs kernel-by-kernel), so r = 1 holds to
+    rounding: 1e-9 relative;
+  * 4096 copies of that template give bit-identical results (determinism);
+  * the amplitude is proportional to 1 / distance: doubling every template's
+    distance halves d_h and quarters h_h (1e-12 relative).
+
+Direct, 4 yr of data at 5 s cadence (12.6 M dense bins), sparse length 4096
+(the large direct shape, fewer templates):
+  * the same distance scaling of d_h, h_h (1e-12 relative);
+  * the FP32-bulk form stays within 3e-7 of h_h of the FP64 form in d_h and
+    h_h (measured up to 1.3e-7 on uneven batches).
+
+Sparse lengths 4, 5 (the smallest the plan accepts) and 6144 (the largest)
+against the oracle run template by template: |delta logL| <= 1e-6 + 1e-12
+|logL| (a 4-knot grid interpolates badly, |logL| reaches 5e7).
+"""
+
+import numpy as np
+import pytest
+
+from tests.params import MODES, YRSID_SI, draw, perturb
+
+pytestmark = pytest.mark.gpu
+
+
+def _heterodyned_problem():
+    from bbhx_amd.likelihood import HeterodynedLikelihood
+    from bbhx_amd.utils.orbits import EqualArmlengthOrbits
+    from bbhx_amd.utils.sensitivity import AET1SensitivityMatrix
+    from bbhx_amd.waveformbuild import BBHWaveformFD
+
+    ref = draw(16, 1000)[:, 0]
+    f = np.arange(1, 2**20 + 1) / YRSID_SI
+    orbits = EqualArmlengthOrbits(dt=1e4, t_max=3.0 * YRSID_SI)
+    wave = BBHWaveformFD(response_kwargs=dict(orbits=orbits, tdi2=True))
+    sens = AET1SensitivityMatrix(f)
+    psd = np.array([sens[0], sens[1], sens[2]])
+    inj = wave(*ref, freqs=f, length=1024, modes=MODES, direct=False, fill=True)[0].cpu().numpy()
+    rng = np.random.default_rng(1001)
+    df = f[1] - f[0]
+    data = inj + np.sqrt(psd / (4 * df)) * (rng.standard_normal(psd.shape) + 1j * rng.standard_normal(psd.shape))
+    het = HeterodynedLikelihood(wave, f, data, ref, 128, sens_mat=sens, template_gen_kwargs=dict(modes=MODES),
+                                reference_gen_kwargs=dict(length=8096, modes=MODES))
+    return het, ref
+
+
+@pytest.fixture(scope="module")
+def het_full():
+    return _heterodyned_problem()
+
+
+def test_heterodyned_reference_identity_full_size(het_full):
+    het, ref = het_full
+    B = 4096
+    p = np.repeat(ref[:, None], B, axis=1)
+    ll = het.get_ll(p, fused=True)
+    d_h, h_h = het.hdyn_d_h.cpu().numpy(), het.hdyn_h_h.cpu().numpy()
+    K = het.freqs.numel()
+    dc = het.data_constants.cpu().numpy().reshape(4, 3, K)
+    jend = K if het.include_last_segment else K - 1
+    dh_exp = dc[0, :, 1:jend].sum()
+    hh_exp = dc[2, :, 1:jend].real.sum()
+    print("d_h", d_h[0], "expected", dh_exp, "h_h", h_h[0], "expected", hh_exp, flush=True)
+    assert abs(d_h[0] - dh_exp) <= 1e-9 * abs(hh_exp)
+    assert abs(h_h[0] - hh_exp) <= 1e-9 * abs(hh_exp)
+    assert np.all(ll == ll[0]) and np.all(d_h == d_h[0]) and np.all(h_h == h_h[0])
+
+
+def test_heterodyned_distance_scaling_full_size(het_full):
+    het, ref = het_full
+    p = perturb(ref, 4096, 1001, scale=1e-3)
+    het.get_ll(p, fused=True)
+    d1, h1 = het.hdyn_d_h.cpu().numpy(), het.hdyn_h_h.cpu().numpy()
+    q = p.copy()
+    q[4] *= 2.0
+    het.get_ll(q, fused=True)
+    d2, h2 = het.hdyn_d_h.cpu().numpy(), het.hdyn_h_h.cpu().numpy()
+    scale = np.abs(h1)
+    assert np.max(np.abs(2 * d2 - d1) / scale) < 1e-12
+    assert np.max(np.abs(4 * h2 - h1) / scale) < 1e-12
+
+
+@pytest.fixture(scope="module")
+def direct_large():
+    from bbhx_amd.likelihood import Likelihood
+    from bbhx_amd.utils.orbits import EqualArmlengthOrbits
+    from bbhx_amd.utils.sensitivity import AET1SensitivityMatrix
+    from bbhx_amd.waveformbuild import BBHWaveformFD
+
+    cadence = 5.0
+    n = int(4.0 * YRSID_SI // cadence)
+    f = np.fft.rfftfreq(n, cadence)[1:]
+    p = draw(64, 1005)
+    orbits = EqualArmlengthOrbits(dt=1e4, t_max=5.0 * YRSID_SI)
+    wave = BBHWaveformFD(response_kwargs=dict(orbits=orbits, tdi2=True))
+    sens = AET1SensitivityMatrix(f)
+    psd = np.array([sens[0], sens[1], sens[2]])
+    inj = wave(*p[:, 0], freqs=f, length=1024, modes=MODES, direct=False, fill=True)[0].cpu().numpy()
+    rng = np.random.default_rng(1006)
+    df = f[1] - f[0]
+    data = inj + np.sqrt(psd / (4 * df)) * (rng.standard_normal(psd.shape) + 1j * rng.standard_normal(psd.shape))
+    like = Likelihood(wave, f, data, psd)
+    lengths = np.full(p.shape[1], 4096, dtype=np.int32)
+    plan = like.direct_plan(lengths, MODES, chunk=16384, min_chunk=4096)
+    return like, plan, p, lengths
+
+
+def test_direct_distance_scaling_large(direct_large):
+    like, plan, p, lengths = direct_large
+    like.get_ll_fused(p, lengths, modes=MODES, plan=plan)
+    d1, h1 = like.d_h.cpu().numpy(), like.h_h.cpu().numpy()
+    q = p.copy()
+    q[4] *= 2.0
+    like.get_ll_fused(q, lengths, modes=MODES, plan=plan)
+    d2, h2 = like.d_h.cpu().numpy(), like.h_h.cpu().numpy()
+    scale = np.maximum(np.abs(h1), 1e-300)
+    print("h_h range", np.abs(h1).min(), np.abs(h1).max(), flush=True)
+    assert np.all(np.abs(h1) > 0)
+    assert np.max(np.abs(2 * d2 - d1) / scale) < 1e-12
+    assert np.max(np.abs(4 * h2 - h1) / scale) < 1e-12
+
+
+def test_direct_fp32_bulk_error_bound_large(direct_large):
+    like, plan, p, lengths = direct_large
+    like.get_ll_fused(p, lengths, modes=MODES, plan=plan, precision=0)
+    d64, h64 = like.d_h.cpu().numpy(), like.h_h.cpu().numpy()
+    like.get_ll_fused(p, lengths, modes=MODES, plan=plan, precision=1)
+    d32, h32 = like.d_h.cpu().numpy(), like.h_h.cpu().numpy()
+    plan.set_precision(0)
+    scale = np.abs(h64)
+    rd, rh = np.abs(d32 - d64) / scale, np.abs(h32 - h64) / scale
+    print("FP32 bulk: max |d d_h| / h_h", rd.max(), "max |d h_h| / h_h", rh.max(), flush=True)
+    assert rd.max() < 3e-7 and rh.max() < 3e-7
+
+
+def test_direct_extreme_lengths_against_oracle():
+    from bbhx_amd.likelihood import Likelihood
+    from bbhx_amd.utils.orbits import EqualArmlengthOrbits
+    from bbhx_amd.utils.sensitivity import AET1SensitivityMatrix
+    from bbhx_amd.waveformbuild import BBHWaveformFD
+    from oracle.pipeline import OracleLikelihood, OracleWaveform
+
+    orbits = EqualArmlengthOrbits(dt=1e4, t_max=3.0 * YRSID_SI)
+    ref = draw(1, 1001)[:, 0]
+    ref[0], ref[1] = 1.3e6, 4.1e5
+    ref[11] = 0.62 * YRSID_SI
+    f = np.arange(1, 2**15 + 1) * (4.0 / YRSID_SI)
+    owave = OracleWaveform(orbits.as_host_dict(), tdi2=True)
+    inj = owave(*ref, freqs=f, length=1024, modes=MODES, direct=False, fill=True, squeeze=True)[0]
+    sens = AET1SensitivityMatrix(f)
+    psd = np.array([sens[0], sens[1], sens[2]])
+    rng = np.random.default_rng(1007)
+    df = f[1] - f[0]
+    data = inj + np.sqrt(psd / (4 * df)) * (rng.standard_normal(psd.shape) + 1j * rng.standard_normal(psd.shape))
+    wave = BBHWaveformFD(response_kwargs=dict(orbits=orbits, tdi2=True))
+    like = Likelihood(wave, f, data, psd)
+    olike = OracleLikelihood(owave, f, data, psd)
+    lengths = np.array([4, 5, 6144, 6144], dtype=np.int32)
+    p = perturb(ref, lengths.size, 23, scale=1e-5)
+    ll = like.get_ll_fused(p, lengths, modes=MODES)
+    ll_o = np.array([olike.get_ll(p[:, b:b + 1], length=int(L), modes=MODES)[0][0] for b, L in enumerate(lengths)])
+    print("fused", ll, "oracle", ll_o, flush=True)
+    assert np.all(np.abs(ll - ll_o) <= 1e-6 + 1e-12 * np.abs(ll_o))
