@@ -144,10 +144,16 @@ def bench_direct_ragged(args):
     df = f[1] - f[0]
     data = inj + np.sqrt(psd / (4 * df)) * (rng.standard_normal(psd.shape) + 1j * rng.standard_normal(psd.shape))
     like = Likelihood(wave, f, data, psd)
-    log("autotuning the launch shape")
-    tuned = tune_direct(like, p, lengths, MODES, reps=2,
-                        progress=lambda c, b, g, ms: log(f"  chunk {c} block {b} grid {g}: {ms:.2f} ms"))
-    log(f"autotuner: chunk {tuned['chunk']}, block {tuned['block']}, grid {tuned['grid']} ({tuned['ms']:.2f} ms)")
+    if args.launch:
+        # a fixed launch shape (profiling: the trace then holds one shape only)
+        c, bl, gr = (int(v) for v in args.launch.split(","))
+        tuned = {"chunk": c, "block": bl, "grid": gr, "ms": None}
+        log(f"fixed launch shape: chunk {c}, block {bl}, grid {gr}")
+    else:
+        log("autotuning the launch shape")
+        tuned = tune_direct(like, p, lengths, MODES, reps=2,
+                            progress=lambda c, b, g, ms: log(f"  chunk {c} block {b} grid {g}: {ms:.2f} ms"))
+        log(f"autotuner: chunk {tuned['chunk']}, block {tuned['block']}, grid {tuned['grid']} ({tuned['ms']:.2f} ms)")
     plan = like.direct_plan(lengths, MODES, chunk=tuned["chunk"], block=tuned["block"], grid=tuned["grid"])
     rows, ts, te = fused_param_rows(like.waveform_gen, p, 0.0, 1.0, like.device)
     flags = torch.zeros(rows.shape[1], dtype=torch.int32, device=like.device)
@@ -333,6 +339,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-grad", action="store_true", help="direct workloads: skip the gradient timing")
     ap.add_argument("--no-fp32", action="store_true", help="direct workloads: skip the FP32-bulk form")
+    ap.add_argument("--launch", default="", help="direct workloads: fixed 'chunk,block,grid' instead of autotuning")
     ap.add_argument("--no-grad-1024", action="store_true",
                     help="skip the gradient workload of 1024 templates at K = 256")
     ap.add_argument("--workload", default="heterodyned", choices=("heterodyned", "direct-ragged"))
