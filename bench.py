@@ -36,6 +36,32 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+def graph_time(fn, steps, warmup):
+    """Capture fn() (device work only, no host synchronisation) in a hipGraph
+    after warm-up calls on a side stream; returns (ms per replay over `steps`
+    replays, the captured outputs)."""
+    import torch
+
+    st = torch.cuda.Stream()
+    st.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(st):
+        for _ in range(max(warmup, 1)):
+            fn()
+    torch.cuda.current_stream().wait_stream(st)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        out = fn()
+    torch.cuda.synchronize()
+    g.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.replay()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps, out, g
+
+
 def build_problem(batch, nodes, dense_bits, seed=1000):
     import torch
 
@@ -387,39 +413,43 @@ def main():
         return het.get_ll_device(rows, t_start, t_end)
 
     log("warmup")
-    for _ in range(args.warmup):
+    # the whole step (record set-up, residual/sum kernel, logL arithmetic) as one
+    # hipGraph: the timed region replays it (no allocation, no host work per step)
+    ms_per_step, ll_graph, graph = graph_time(step, args.steps, args.warmup)
+    value = B / (ms_per_step * 1e-3)
+    ll_eager = step().cpu().numpy()
+    graph.replay()
+    torch.cuda.synchronize()
+    graph_diff = float(np.max(np.abs(ll_graph.cpu().numpy() - ll_eager)))
+    ll_fused = ll_graph.cpu().numpy()
+    ll_dev = ll_graph
+    # set-up / evaluation split from eager calls (a captured graph carries no timing events)
+    timing_enable(True)
+    t2 = time.perf_counter()
+    for _ in range(args.steps):
         step()
     torch.cuda.synchronize()
-
-    timing_enable(True)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ll_dev = step()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
+    t3 = time.perf_counter()
     ms_setup, ms_eval, ncalls = timing_read()
     timing_enable(False)
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
-    value = B / ((t1 - t0) / args.steps)
-    ll_fused = ll_dev.cpu().numpy()
-    log(f"fused FP64: {ms_per_step:.3f} ms/step, {value:.4g} evals/s; setup {ms_setup / max(ncalls, 1):.3f} ms, "
-        f"eval {ms_eval / max(ncalls, 1):.3f} ms per call")
+    ms_eager = (t3 - t2) * 1e3 / args.steps
+    log(f"fused FP64 (hipGraph replay): {ms_per_step:.3f} ms/step, {value:.4g} evals/s; eager {ms_eager:.3f} ms/step; "
+        f"setup {ms_setup / max(ncalls, 1):.3f} ms, eval {ms_eval / max(ncalls, 1):.3f} ms per call; "
+        f"|graph - eager| = {graph_diff:.1e}")
 
-    # gradient (11 parameters) of the same log-likelihoods: adjoint kernel
-    for _ in range(max(args.warmup, 1)):
+    # gradient (11 parameters) of the same log-likelihoods: adjoint kernels, also
+    # timed as hipGraph replays (the entry point forks a side stream and joins it
+    # with events, which a graph captures)
+    gms, (ll_g, grad), _ = graph_time(lambda: het.get_ll_grad_device(rows, t_start, t_end), args.steps, args.warmup)
+    timing_enable(True)
+    for _ in range(args.steps):
         het.get_ll_grad_device(rows, t_start, t_end)
     torch.cuda.synchronize()
-    timing_enable(True)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ll_g, grad = het.get_ll_grad_device(rows, t_start, t_end)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
     gms_setup, gms_eval, gn = timing_read()
     timing_enable(False)
-    grad_rate = B / ((t1 - t0) / args.steps)
+    grad_rate = B / (gms * 1e-3)
     grad_ll_diff = float(torch.max(torch.abs(ll_g - ll_dev)).item())
-    log(f"gradient: {(t1 - t0) * 1e3 / args.steps:.3f} ms/step, {grad_rate:.4g} grad evals/s "
+    log(f"gradient (hipGraph replay): {gms:.3f} ms/step, {grad_rate:.4g} grad evals/s "
         f"({value / grad_rate:.2f}x the forward time); setup {gms_setup / max(gn, 1):.3f} ms, "
         f"eval {gms_eval / max(gn, 1):.3f} ms; max |logL(grad kernel) - logL(forward)| = {grad_ll_diff:.2e}")
 
@@ -430,20 +460,8 @@ def main():
         prob_g = build_problem(1024, 256, args.dense_bits, seed=1003)
         het_g = prob_g["het"]
         rg, tsg, teg = het_g.fused_inputs(prob_g["templates"], {})
-        for _ in range(max(args.warmup, 1)):
-            het_g.get_ll_device(rg, tsg, teg)
-            het_g.get_ll_grad_device(rg, tsg, teg)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            het_g.get_ll_device(rg, tsg, teg)
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        for _ in range(args.steps):
-            het_g.get_ll_grad_device(rg, tsg, teg)
-        torch.cuda.synchronize()
-        t2 = time.perf_counter()
-        fwd_ms, grd_ms = (t1 - t0) * 1e3 / args.steps, (t2 - t1) * 1e3 / args.steps
+        fwd_ms, _, _ = graph_time(lambda: het_g.get_ll_device(rg, tsg, teg), args.steps, args.warmup)
+        grd_ms, _, _ = graph_time(lambda: het_g.get_ll_grad_device(rg, tsg, teg), args.steps, args.warmup)
         grad_1024 = {"templates": 1024, "sparse_nodes": 256, "forward_ms": fwd_ms, "grad_ms": grd_ms,
                      "grad_evals_per_s": 1024 / (grd_ms * 1e-3), "grad_cost_vs_forward": grd_ms / fwd_ms}
         log(f"gradient (1024 x K=256): {grd_ms:.3f} ms/step, {grad_1024['grad_evals_per_s']:.4g} grad evals/s, "
@@ -500,7 +518,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded injection + Gaussian noise, analytic A/E/T PSD)",
-        "config": {"workload": "heterodyned logL, fused FP64", "templates": B, "sparse_nodes": args.nodes,
+        "config": {"workload": "heterodyned logL, fused FP64, hipGraph replay", "templates": B,
+                   "sparse_nodes": args.nodes, "eager_ms_per_step": ms_eager, "graph_vs_eager": graph_diff,
                    "dense_bins": 2**args.dense_bits, "modes": 6, "tdi": "AET tdi2",
                    "plain_fp64_path_evals_per_s": plain_rate, "max_abs_dlogL_vs_cpu_oracle": dll,
                    "grad_evals_per_s": grad_rate, "grad_cost_vs_forward": value / grad_rate,
