@@ -753,11 +753,13 @@ __device__ inline void mode_adjoint_per_lane(double* __restrict__ Ym, const SegR
 // mode sum of a block, Lam = 4 w (d~ - h~) per bin and channel, then the modes
 // again, each bin's coefficient adjoints (mode_adjoint) summed over the lanes
 // that share a segment (reduce32) and added to the segment's adjoint record.
-// at least 4 wavefronts per SIMD (at most 128 VGPRs): measured faster than the
-// 3 that the unconstrained register allocation gives, despite a few spilled
-// registers in the rarely taken per-lane path
+// FP64 forms: at least 4 wavefronts per SIMD (at most 128 VGPRs), measured
+// faster than the 3 that the unconstrained register allocation gives, despite a
+// few spilled registers in the rarely taken per-lane path; FP32-bulk form: at
+// least 6 (at most 80 VGPRs), measured 6 % faster than its unconstrained 5
 template <bool ADJ, class Rec>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_dir_like_t(double* __restrict__ partial, double* __restrict__ Yadj,
+__global__ __launch_bounds__(256) __attribute__((
+    amdgpu_waves_per_eu(std::is_same<Rec, SegRecF>::value ? 6 : 4, 8))) void k_dir_like_t(double* __restrict__ partial, double* __restrict__ Yadj,
                                                     const int* __restrict__ band,
                                                   const int* __restrict__ total, const int* __restrict__ perm,
                                                   int* __restrict__ next, const Rec* __restrict__ R,
