@@ -9,8 +9,9 @@
 //  k_hdyn_reduce  sums the node chunks in a fixed order (deterministic)
 //  k_direct_like  ragged tiles of one binary band; noise-weights the template
 //                 in place (as the reference does) and reduces conj(d) h and
-//                 |h|^2 over the tile; k_direct_like_reduce adds the tiles of
-//                 each binary in a fixed order. No atomics anywhere.
+//                 |h|^2 over the tile; k_direct_like_reduce (one workgroup per
+//                 binary) adds the tiles of each binary in a fixed order. No
+//                 atomics anywhere.
 
 #include <vector>
 
@@ -170,21 +171,28 @@ __global__ __launch_bounds__(DL_THREADS) void k_direct_like(double* __restrict__
     }
 }
 
-__global__ void k_direct_like_reduce(cd* __restrict__ d_h, cd* __restrict__ h_h, const double* __restrict__ partial,
-                                     const InterpBin* __restrict__ bins, int numBinAll, int total_tiles)
+// One workgroup per binary: its threads take the binary's tiles in a fixed
+// stride and the workgroup adds the per-thread sums in a fixed tree (a band of
+// 12.6 M bins is ~6000 tiles, too many for one thread's dependent loads).
+__global__ __launch_bounds__(DL_THREADS) void k_direct_like_reduce(cd* __restrict__ d_h, cd* __restrict__ h_h,
+                                                                   const double* __restrict__ partial,
+                                                                   const InterpBin* __restrict__ bins, int numBinAll,
+                                                                   int total_tiles)
 {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= numBinAll) return;
+    const int b = blockIdx.x;
     const int t0 = bins[b].tile_begin;
     const int t1 = (b + 1 < numBinAll) ? bins[b + 1].tile_begin : total_tiles;
     double a = 0.0, c = 0.0, e = 0.0;
-    for (int t = t0; t < t1; t++) {
+    for (int t = t0 + (int)threadIdx.x; t < t1; t += DL_THREADS) {
         a += partial[3 * (long)t + 0];
         c += partial[3 * (long)t + 1];
         e += partial[3 * (long)t + 2];
     }
-    d_h[b] = d_h[b] + mk(4.0 * a, 4.0 * c);
-    h_h[b] = h_h[b] + mk(4.0 * e, 0.0);
+    block_sum3(a, c, e);
+    if (threadIdx.x == 0) {
+        d_h[b] = d_h[b] + mk(4.0 * a, 4.0 * c);
+        h_h[b] = h_h[b] + mk(4.0 * e, 0.0);
+    }
 }
 
 inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
@@ -258,7 +266,7 @@ int bbhx_hip_direct_like(double* d_h, double* h_h, const double* dataChannels, c
     hipLaunchKernelGGL(k_direct_like, dim3((unsigned)tiles), dim3(DL_THREADS), 0, st, partial, bins, numBinAll,
                        reinterpret_cast<const cd*>(dataChannels), noise_weight_times_df, data_stream_length);
     if (hipGetLastError() != hipSuccess) return ERR_LAUNCH;
-    hipLaunchKernelGGL(k_direct_like_reduce, dim3(ceil_div(numBinAll, 256)), dim3(256), 0, st,
+    hipLaunchKernelGGL(k_direct_like_reduce, dim3(numBinAll), dim3(DL_THREADS), 0, st,
                        reinterpret_cast<cd*>(d_h), reinterpret_cast<cd*>(h_h), partial, bins, numBinAll, (int)tiles);
     return hipGetLastError() == hipSuccess ? OK : ERR_LAUNCH;
 }

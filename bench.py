@@ -261,6 +261,39 @@ def bench_direct_ragged(args):
             f"|dlogL| vs FP64 max {dll.max():.2e}, median {np.median(dll):.2e}; "
             f"{int(np.sum(dll < 1e-3))}/{B} within 1e-3")
 
+    # the unfused inner product (the reference's direct_like kernel, a10) on a few
+    # templates materialised by the kernel-by-kernel path: HBM-bound, 168 B per
+    # dense bin and template (h read 48 and written back noise-weighted 48, as the
+    # reference does, d 48, w 24); event-timed on the current stream
+    unfused = None
+    if not args.no_unfused:
+        nb = min(8, B)
+        Lu = int(lengths[0])
+        tc, ist, iln = like.waveform_gen(*p[:, :nb], freqs=like.data_freqs, length=Lu, modes=MODES, direct=False,
+                                         fill=False)
+        tcs = [t.flatten() for t in tc]
+        ptrs = np.asarray([t.data_ptr() for t in tcs], dtype=np.int64)
+        dh_u = torch.zeros(nb, dtype=torch.complex128, device=like.device)
+        hh_u = torch.zeros(nb, dtype=torch.complex128, device=like.device)
+        args_u = (dh_u, hh_u, like.data_channels, like.noise_factors, ptrs, ist, iln, like.data_stream_length, nb)
+        like.like_gen(*args_u)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        reps = 5
+        e0.record()
+        for _ in range(reps):
+            like.like_gen(*args_u)
+        e1.record()
+        torch.cuda.synchronize()
+        ums = e0.elapsed_time(e1) / reps
+        ubytes = 168.0 * float(np.sum(iln))
+        unfused = {"kernel": "k_direct_like", "templates": nb, "sparse_length": Lu, "dense_bins": int(np.sum(iln)),
+                   "ms": ums, "bytes": ubytes, "GB_per_s": ubytes / (ums * 1e-3) / 1e9,
+                   "hbm_frac": ubytes / (ums * 1e-3) / (HBM_PEAK_GBS * 1e9)}
+        log(f"unfused inner product (k_direct_like): {nb} templates, {int(np.sum(iln))} dense bins, {ums:.3f} ms, "
+            f"{unfused['GB_per_s']:.0f} GB/s ({unfused['hbm_frac']:.2f} of HBM peak)")
+        del tc, tcs, dh_u, hh_u, args_u
+
     # gradient (11 parameters) of the same batch: the adjoint launch sequence
     grad_info = None
     if not args.no_grad:
@@ -345,7 +378,8 @@ def bench_direct_ragged(args):
                    "lengths_min": int(lengths.min()), "lengths_max": int(lengths.max()),
                    "lengths_sum": int(lengths.sum()), "dense_bins": int(f.size), "modes": 6, "tdi": "AET tdi2",
                    "launch": {k: tuned[k] for k in ("chunk", "block", "grid")}, "graph_vs_eager": diff,
-                   "dense_bin_templates": dense, **split, "fp32_bulk": f32_info, "gradient": grad_info},
+                   "dense_bin_templates": dense, **split, "fp32_bulk": f32_info, "gradient": grad_info,
+                   "unfused_inner_product": unfused},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
@@ -365,6 +399,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-grad", action="store_true", help="direct workloads: skip the gradient timing")
     ap.add_argument("--no-fp32", action="store_true", help="direct workloads: skip the FP32-bulk form")
+    ap.add_argument("--no-unfused", action="store_true",
+                    help="direct workloads: skip the unfused inner-product (k_direct_like) bandwidth measurement")
     ap.add_argument("--launch", default="", help="direct workloads: fixed 'chunk,block,grid' instead of autotuning")
     ap.add_argument("--no-grad-1024", action="store_true",
                     help="skip the gradient workload of 1024 templates at K = 256")
