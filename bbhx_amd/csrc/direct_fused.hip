@@ -28,7 +28,7 @@
 //                 mode, WaveformBuild.cu:80-143, window and mode sum :48-60)
 //                 and accumulates the noise-weighted inner products
 //                 (Likelihood.cu:357-471); one partial sum per work item
-//   k_dir_reduce  per binary, the partials in chunk order (deterministic)
+//   k_dir_reduce  one wavefront per binary: its partials in a fixed order (deterministic)
 
 #include <type_traits>
 #include <vector>
@@ -952,23 +952,31 @@ __global__ __launch_bounds__(256) __attribute__((
     }
 }
 
-__global__ void k_dir_reduce(double* __restrict__ d_h, double* __restrict__ h_h, const double* __restrict__ partial,
-                             const int* __restrict__ band, const int* __restrict__ total, int B)
+// One wavefront per binary: lane l adds the binary's items l, l + 64, ... (in
+// item order), then the wavefront adds the 64 lane sums in a fixed tree; the
+// result does not depend on the schedule of the dense-band kernel.
+__global__ __launch_bounds__(64) void k_dir_reduce(double* __restrict__ d_h, double* __restrict__ h_h,
+                                                   const double* __restrict__ partial, const int* __restrict__ band,
+                                                   const int* __restrict__ total, int B)
 {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= B) return;
+    const int b = blockIdx.x;
     const int c0 = band[3 * b + 2];
     const int c1 = b + 1 < B ? band[3 * (b + 1) + 2] : total[0];
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-    for (int c = c0; c < c1; c++) {
+    for (int c = c0 + (int)threadIdx.x; c < c1; c += 64) {
         s0 += partial[4 * (long)c];
         s1 += partial[4 * (long)c + 1];
         s2 += partial[4 * (long)c + 2];
     }
-    d_h[2 * b] = 4.0 * s0;
-    d_h[2 * b + 1] = 4.0 * s1;
-    h_h[2 * b] = 4.0 * s2;
-    h_h[2 * b + 1] = 0.0;
+    s0 = wsum(s0);
+    s1 = wsum(s1);
+    s2 = wsum(s2);
+    if (threadIdx.x == 0) {
+        d_h[2 * b] = 4.0 * s0;
+        d_h[2 * b + 1] = 4.0 * s1;
+        h_h[2 * b] = 4.0 * s2;
+        h_h[2 * b + 1] = 0.0;
+    }
 }
 
 // ---------------------------------------------------------------- gradient kernels
@@ -1328,7 +1336,7 @@ int direct_launch(bbhx_direct_plan* p, double* d_h, double* h_h, double* grad, i
                            p->band, p->total, p->perm, p->next, reinterpret_cast<const SegRec*>(p->Cf), p->grid_lo,
                            p->grid_istep, p->off, p->len, t_start, t_end, dataFreqs,
                            reinterpret_cast<const cd*>(dataChannels), noiseFactors, p->N, B, M, chunk);
-    hipLaunchKernelGGL(k_dir_reduce, dim3(ceil_div(B, 256)), dim3(256), 0, st, d_h, h_h, p->partial, p->band,
+    hipLaunchKernelGGL(k_dir_reduce, dim3(B), dim3(64), 0, st, d_h, h_h, p->partial, p->band,
                        p->total, B);
     if (grad) {
         hipLaunchKernelGGL(k_dirg_spline_adj, dim3(B), dim3(64), 0, st, p->Yadj, p->X, p->off, p->len, M);
