@@ -271,6 +271,8 @@ __global__ __launch_bounds__(64) void k_dir_grid(double* __restrict__ X, double*
     }
 }
 
+constexpr int DSPL_BLOCK = 8;  // rows loaded ahead of the recurrence in each sweep of k_dir_spline
+
 // Not-a-knot spline of one binary: lane s = mode * 9 + quantity runs the
 // Thomas algorithm over the binary's L knots (the system of spline.hip);
 // consecutive lanes read consecutive quantities of the same knot.
@@ -330,22 +332,36 @@ __global__ __launch_bounds__(64) void k_dir_spline(const double* __restrict__ X,
         cp[at(0)] = cprev;
         dp[at(0)] = dprev;
     }
+    // each sweep loads DSPL_BLOCK rows ahead of its recurrence (one memory
+    // latency per block) and divides once per row
     double xa = x0, xb = x1, ya = y0, yb = y1;
-    for (int i = 1; i < n - 1; i++) {
-        const double xc = xi(i + 1), yc = yi(i + 1);
-        const double ex1 = xb - xa, ex2 = xc - xb;
-        const double t1 = (yb - ya) / ex1, t2 = (yc - yb) / ex2;
-        const double rhs = 3.0 * (ex2 * t1 + ex1 * t2);
-        const double diag = 2 * (ex1 + ex2), lo = ex2, up = ex1;
-        const double mm = diag - lo * cprev;
-        cprev = up / mm;
-        dprev = (rhs - lo * dprev) / mm;
-        cp[at(i)] = cprev;
-        dp[at(i)] = dprev;
-        xa = xb;
-        xb = xc;
-        ya = yb;
-        yb = yc;
+    for (int i0 = 1; i0 < n - 1; i0 += DSPL_BLOCK) {
+        double xv[DSPL_BLOCK], yv[DSPL_BLOCK];
+#pragma unroll
+        for (int u = 0; u < DSPL_BLOCK; u++) {
+            const int i = i0 + u;
+            xv[u] = i < n - 1 ? xi(i + 1) : 0.0;
+            yv[u] = i < n - 1 ? yi(i + 1) : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < DSPL_BLOCK; u++) {
+            const int i = i0 + u;
+            if (i >= n - 1) break;
+            const double xc = xv[u], yc = yv[u];
+            const double ex1 = xb - xa, ex2 = xc - xb;
+            const double t1 = (yb - ya) / ex1, t2 = (yc - yb) / ex2;
+            const double rhs = 3.0 * (ex2 * t1 + ex1 * t2);
+            const double diag = 2 * (ex1 + ex2), lo = ex2, up = ex1;
+            const double im = 1.0 / (diag - lo * cprev);
+            cprev = up * im;
+            dprev = (rhs - lo * dprev) * im;
+            cp[at(i)] = cprev;
+            dp[at(i)] = dprev;
+            xa = xb;
+            xb = xc;
+            ya = yb;
+            yb = yc;
+        }
     }
     {
         const double xm3 = xi(n - 3), xm2 = xi(n - 2), xm1 = xi(n - 1);
@@ -366,13 +382,26 @@ __global__ __launch_bounds__(64) void k_dir_spline(const double* __restrict__ X,
         Cf[((base + n - 1) * M + m) * RS + 1] = xn;
     }
     if (CfF) put_f(n - 1, xn, xn, yn, 0.0, 0.0, 0.0);
-    for (int i = n - 2; i >= 0; i--) {
-        const double si = dp[at(i)] - cp[at(i)] * snext;
-        const double x_i = xi(i), y_i = yi(i);
-        const double dx = xn - x_i;
-        const double slope = (yn - y_i) / dx;
-        const double t = (si + snext - 2 * slope) / dx;
-        const double c2 = (slope - si) / dx - t, c3 = t / dx;
+    for (int i0 = n - 2; i0 >= 0; i0 -= DSPL_BLOCK) {
+      double dv[DSPL_BLOCK], cv[DSPL_BLOCK], xv[DSPL_BLOCK], yv[DSPL_BLOCK];
+#pragma unroll
+      for (int u = 0; u < DSPL_BLOCK; u++) {
+          const int i = i0 - u;
+          dv[u] = i >= 0 ? dp[at(i)] : 0.0;
+          cv[u] = i >= 0 ? cp[at(i)] : 0.0;
+          xv[u] = i >= 0 ? xi(i) : 0.0;
+          yv[u] = i >= 0 ? yi(i) : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < DSPL_BLOCK; u++) {
+        const int i = i0 - u;
+        if (i < 0) break;
+        const double si = dv[u] - cv[u] * snext;
+        const double x_i = xv[u], y_i = yv[u];
+        const double idx = 1.0 / (xn - x_i);
+        const double slope = (yn - y_i) * idx;
+        const double t = (si + snext - 2 * slope) * idx;
+        const double c2 = (slope - si) * idx - t, c3 = t * idx;
         dp[at(i)] = si;
         C2[at(i)] = c2;
         cp[at(i)] = c3;
@@ -384,6 +413,7 @@ __global__ __launch_bounds__(64) void k_dir_spline(const double* __restrict__ X,
         snext = si;
         xn = x_i;
         yn = y_i;
+      }
     }
 }
 

@@ -8,7 +8,10 @@
 // rows. It is solved by the Thomas algorithm (LU without pivoting; the system
 // is nonsingular and the pivots stay positive for increasing knots, see the
 // end-row analysis in the test suite), building each row on the fly from x and
-// y, so no band matrix is ever materialised:
+// y, so no band matrix is ever materialised. A lane's sweeps are one long
+// dependent chain: each sweep loads SPL_BLOCK rows ahead of the recurrence
+// (one memory latency per block) and divides once per row (a reciprocal
+// instead of two or four divisions):
 //   forward sweep: c'_i -> lower_diag slab, d'_i -> B slab
 //   backward sweep: s_i -> B, then the segment coefficients
 //   c1 = s_i, c2 = (m_i - s_i)/dx - t, c3 = t/dx, t = (s_i + s_{i+1} - 2 m_i)/dx
@@ -21,6 +24,8 @@
 using namespace bbhx;
 
 namespace {
+
+constexpr int SPL_BLOCK = 16;  // rows whose loads are issued together in each sweep
 
 __global__ __launch_bounds__(64) void k_spline(const double* __restrict__ x, const double* __restrict__ y,
                                                double* __restrict__ Bs, double* __restrict__ c1,
@@ -46,22 +51,34 @@ __global__ __launch_bounds__(64) void k_spline(const double* __restrict__ x, con
         double dprev = b / dx2;
         cp[0] = cprev;
         dp[0] = dprev;
-        // interior rows
+        // interior rows, in blocks of SPL_BLOCK whose loads are all issued before
+        // the recurrence runs through the block (one memory latency per block,
+        // not per row)
         double xa = x0, xb = x1, ya = y0, yb = y1;
-        double xc = x2, yc = y2;
-        for (int i = 1; i < n - 1; i++) {
-            xc = xs[i + 1];
-            yc = ys[i + 1];
-            double ex1 = xb - xa, ex2 = xc - xb;
-            double t1 = (yb - ya) / ex1, t2 = (yc - yb) / ex2;
-            double rhs = 3.0 * (ex2 * t1 + ex1 * t2);
-            double diag = 2 * (ex1 + ex2), lo = ex2, up = ex1;
-            double m = diag - lo * cprev;
-            cprev = up / m;
-            dprev = (rhs - lo * dprev) / m;
-            cp[i] = cprev;
-            dp[i] = dprev;
-            xa = xb; xb = xc; ya = yb; yb = yc;
+        for (int i0 = 1; i0 < n - 1; i0 += SPL_BLOCK) {
+            double xv[SPL_BLOCK], yv[SPL_BLOCK];
+#pragma unroll
+            for (int u = 0; u < SPL_BLOCK; u++) {
+                const int i = i0 + u;
+                xv[u] = i < n - 1 ? xs[i + 1] : 0.0;
+                yv[u] = i < n - 1 ? ys[i + 1] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < SPL_BLOCK; u++) {
+                const int i = i0 + u;
+                if (i >= n - 1) break;
+                const double xc = xv[u], yc = yv[u];
+                double ex1 = xb - xa, ex2 = xc - xb;
+                double t1 = (yb - ya) / ex1, t2 = (yc - yb) / ex2;
+                double rhs = 3.0 * (ex2 * t1 + ex1 * t2);
+                double diag = 2 * (ex1 + ex2), lo = ex2, up = ex1;
+                const double im = 1.0 / (diag - lo * cprev);
+                cprev = up * im;
+                dprev = (rhs - lo * dprev) * im;
+                cp[i] = cprev;
+                dp[i] = dprev;
+                xa = xb; xb = xc; ya = yb; yb = yc;
+            }
         }
         // row n-1 (not-a-knot)
         double xm3 = xs[n - 3], xm2 = xs[n - 2], xm1 = xs[n - 1];
@@ -79,19 +96,34 @@ __global__ __launch_bounds__(64) void k_spline(const double* __restrict__ x, con
     c1[(long)s * n + n - 1] = 0.0;
     c2[(long)s * n + n - 1] = 0.0;
     cp[n - 1] = 0.0;
-    for (int i = n - 2; i >= 0; i--) {
-        double si = dp[i] - cp[i] * snext;
-        dp[i] = si;
-        double xi = xs[i], yi = ys[i];
-        double dx = xn - xi;
-        double slope = (yn - yi) / dx;
-        double t = (si + snext - 2 * slope) / dx;
-        c1[(long)s * n + i] = si;
-        c2[(long)s * n + i] = (slope - si) / dx - t;
-        cp[i] = t / dx;
-        snext = si;
-        xn = xi;
-        yn = yi;
+    for (int i0 = n - 2; i0 >= 0; i0 -= SPL_BLOCK) {
+        // rows i0, i0 - 1, ...: every load of the block before its first store
+        double dv[SPL_BLOCK], cv[SPL_BLOCK], xv[SPL_BLOCK], yv[SPL_BLOCK];
+#pragma unroll
+        for (int u = 0; u < SPL_BLOCK; u++) {
+            const int i = i0 - u;
+            dv[u] = i >= 0 ? dp[i] : 0.0;
+            cv[u] = i >= 0 ? cp[i] : 0.0;
+            xv[u] = i >= 0 ? xs[i] : 0.0;
+            yv[u] = i >= 0 ? ys[i] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < SPL_BLOCK; u++) {
+            const int i = i0 - u;
+            if (i < 0) break;
+            double si = dv[u] - cv[u] * snext;
+            dp[i] = si;
+            double xi = xv[u], yi = yv[u];
+            const double idx = 1.0 / (xn - xi);
+            const double slope = (yn - yi) * idx;
+            const double t = (si + snext - 2 * slope) * idx;
+            c1[(long)s * n + i] = si;
+            c2[(long)s * n + i] = (slope - si) * idx - t;
+            cp[i] = t * idx;
+            snext = si;
+            xn = xi;
+            yn = yi;
+        }
     }
 }
 
