@@ -26,7 +26,8 @@ def test_header_declares_the_backend_methods():
               "bbhx_hip_interpolate", "bbhx_hip_InterpTDI", "bbhx_hip_direct_sum", "bbhx_hip_hdyn",
               "bbhx_hip_direct_like", "bbhx_hip_hdyn_fused", "bbhx_hip_hdyn_fused_grad", "bbhx_hip_direct_plan_create",
               "bbhx_hip_direct_plan_destroy", "bbhx_hip_direct_plan_band", "bbhx_hip_direct_fused",
-              "bbhx_hip_hdyn_prep", "bbhx_hip_direct_plan_enable_grad", "bbhx_hip_direct_fused_grad"):
+              "bbhx_hip_hdyn_prep", "bbhx_hip_direct_plan_enable_grad", "bbhx_hip_direct_fused_grad",
+              "bbhx_hip_direct_plan_set_precision"):
         assert n in names, n
 
 
@@ -73,6 +74,8 @@ def test_version_and_argument_errors_need_no_gpu():
     assert lib.bbhx_hip_direct_plan_bytes(None) == -1
     assert lib.bbhx_hip_direct_plan_band(None, None) == 1
     assert lib.bbhx_hip_direct_plan_enable_grad(None) == 1
+    assert lib.bbhx_hip_direct_plan_set_precision(None, 0) == 1
+    assert lib.bbhx_hip_direct_plan_set_precision(None, 1) == 1
     # heterodyne constants: too few points or nodes, df <= 0, missing arrays
     lib.bbhx_hip_hdyn_prep.argtypes = [C.c_void_p] * 6 + [C.c_int, C.c_void_p, C.c_int, C.c_double, C.c_void_p]
     assert lib.bbhx_hip_hdyn_prep(None, None, None, None, None, None, 1, None, 8, 1e-8, None) == 1
