@@ -128,7 +128,8 @@ def test_ragged_fused_graph_replays(direct_problem):
 
 def test_ragged_fp32_bulk_graph_replays(direct_problem):
     """The FP32-bulk form captured in a hipGraph (records allocated before the
-    capture): replays with new inputs equal eager calls bit for bit."""
+    capture): replays with new inputs match eager calls (1e-9, as the FP64
+    form's graph test)."""
     import torch
 
     from bbhx_amd.likelihood import fused_param_rows
@@ -167,4 +168,4 @@ def test_ragged_fp32_bulk_graph_replays(direct_problem):
         got = out.cpu().numpy()
         print("fp32 replay", rep, got, eager, flush=True)
         assert int(flags.sum().item()) == 0
-        assert np.array_equal(got, eager)
+        assert np.max(np.abs(got - eager)) < 1e-9
