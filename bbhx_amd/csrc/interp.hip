@@ -8,6 +8,11 @@
 // the modes:
 //   h_ch(f) = sum_m conj(T_ch,m(f) A_m(f) e^{i phi_m(f)})  if t_start <= t_m(f) <= t_end
 // with every quantity a cubic in (f - f_k) on the segment k = inds[m][i].
+// A first pass repacks the 36 spline coefficients and the knot of each
+// (binary, mode, segment), spread over nine arrays each in the reference's
+// layout, into one 320-B record (library workspace): the lanes of a wavefront
+// mostly share a segment, and one base address lets the wavefront read it with
+// a few wide scalar loads instead of 37 separately addressed ones.
 
 #include <vector>
 
@@ -20,22 +25,45 @@ using namespace bbhx;
 
 namespace {
 
+// Repack the spline arrays into one InterpSeg per (binary, mode, segment):
+// lane t -> (part p, segment s) with s fastest, so the reads of each spline
+// array are unit-stride; part 9 is the knot.
+__global__ __launch_bounds__(256) void k_interp_pack(InterpSeg* __restrict__ R, const double* __restrict__ freqs,
+                                                     const double* __restrict__ y, const double* __restrict__ c1,
+                                                     const double* __restrict__ c2, const double* __restrict__ c3,
+                                                     long nper)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 10 * nper) return;
+    const int p = (int)(t / nper);
+    const long s = t - p * nper;
+    if (p == 9) {
+        R[s].x = freqs[s];
+        return;
+    }
+    const long q = p * nper + s;
+    R[s].c[p][0] = y[q];
+    R[s].c[p][1] = c1[q];
+    R[s].c[p][2] = c2[q];
+    R[s].c[p][3] = c3[q];
+}
+
 __global__ __launch_bounds__(INTERP_TILE) void k_interp_tdi(const InterpBin* __restrict__ bins, int numBinAll,
                                                             const double* __restrict__ dataFreqs,
-                                                            const double* __restrict__ freqs,
-                                                            const double* __restrict__ y,
-                                                            const double* __restrict__ c1,
-                                                            const double* __restrict__ c2,
-                                                            const double* __restrict__ c3, int length, int numModes,
-                                                            long nper)
+                                                            const InterpSeg* __restrict__ R, int length,
+                                                            int numModes)
 {
+    // e^{i phi} table in LDS: its lookups are per-lane gathers
+    __shared__ double2 s_tab[129];
+    fill_etab(s_tab, (int)threadIdx.x, INTERP_TILE);
+    __syncthreads();
     const int b = find_bin(bins, numBinAll, (int)blockIdx.x);
     const InterpBin& d = bins[b];
     const int i = ((int)blockIdx.x - d.tile_begin) * INTERP_TILE + threadIdx.x;
     if (i >= d.len) return;
     const double f = dataFreqs[d.start + i];
     cd h[3];
-    interp_point(h, f, i, d, b, freqs, y, c1, c2, c3, length, numModes, nper);
+    interp_point(h, f, i, d, b, R, length, numModes, s_tab);
     cd* out = reinterpret_cast<cd*>(d.tmpl);
     out[0 * (long)d.len + i] = h[0];
     out[1 * (long)d.len + i] = h[1];
@@ -103,7 +131,12 @@ extern "C" int bbhx_hip_InterpTDI(const int64_t* templateChannels_ptrs, const do
     if (tiles == 0) return OK;
     InterpBin* bins = upload_table(h, st);
     if (!bins) return ERR_ALLOC;
-    hipLaunchKernelGGL(k_interp_tdi, dim3((unsigned)tiles), dim3(INTERP_TILE), 0, st, bins, numBinAll, dataFreqs, freqs,
-                       propArrays, c1, c2, c3, length, numModes, (long)numBinAll * numModes * length);
+    const long nper = (long)numBinAll * numModes * length;
+    InterpSeg* R = static_cast<InterpSeg*>(workspace_get(WS_INTERP, sizeof(InterpSeg) * (size_t)nper));
+    if (!R) return ERR_ALLOC;
+    hipLaunchKernelGGL(k_interp_pack, dim3((unsigned)ceil_div(10 * nper, 256)), dim3(256), 0, st, R, freqs, propArrays,
+                       c1, c2, c3, nper);
+    hipLaunchKernelGGL(k_interp_tdi, dim3((unsigned)tiles), dim3(INTERP_TILE), 0, st, bins, numBinAll, dataFreqs, R,
+                       length, numModes);
     return hipGetLastError() == hipSuccess ? OK : ERR_LAUNCH;
 }

@@ -22,7 +22,8 @@ enum WsSlot : int {
     WS_GRAD = 5,
     WS_GRAD_AUX = 6,
     WS_PREP = 7,
-    WS_SLOTS = 8,
+    WS_INTERP = 8,
+    WS_SLOTS = 9,
 };
 
 // Returns a device pointer with at least 'bytes' bytes on the current device,

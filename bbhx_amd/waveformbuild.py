@@ -69,6 +69,7 @@ class TemplateInterpFD(BBHxParallelModule):
         data_freqs = to_dev(data_freqs, self.device)
         self.data_length = int(data_freqs.numel())
         self.num_channels = num_channels
+        self.t_start, self.t_end = as_np1d(t_start), as_np1d(t_end)
         (freqs, y, c1, c2, c3) = interp_container
         freqs_shaped = freqs.reshape(num_bin_all, num_modes, -1)
 

@@ -292,6 +292,29 @@ def bench_direct_ragged(args):
                    "hbm_frac": ubytes / (ums * 1e-3) / (HBM_PEAK_GBS * 1e9)}
         log(f"unfused inner product (k_direct_like): {nb} templates, {int(np.sum(iln))} dense bins, {ums:.3f} ms, "
             f"{unfused['GB_per_s']:.0f} GB/s ({unfused['hbm_frac']:.2f} of HBM peak)")
+        # the template build of the same templates (the reference's InterpTDI, a8:
+        # repack + k_interp_tdi), repeated on its own inputs: per dense bin f (8 B),
+        # one segment index per mode (4 B each) and the 3 complex channels out
+        # (48 B); per (template, mode, knot) the 36 coefficients and the knot once
+        ir, spl = like.waveform_gen.interp_response, like.waveform_gen.spline
+        tp = np.asarray([t.data_ptr() for t in ir.template_carrier], dtype=np.int64)
+        iargs = (tp, like.data_freqs, *spl.container, ir.t_start, ir.t_end, ir.length, ir.data_length, ir.num_bin_all,
+                 ir.num_modes, ir.ptrs, ir.start_inds, ir.lengths)
+        like.waveform_gen.backend.InterpTDI_wrap(*iargs)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(reps):
+            like.waveform_gen.backend.InterpTDI_wrap(*iargs)
+        e1.record()
+        torch.cuda.synchronize()
+        ims = e0.elapsed_time(e1) / reps
+        ibytes = (56.0 + 4.0 * len(MODES)) * float(np.sum(iln)) + 296.0 * nb * len(MODES) * Lu
+        unfused["template_build"] = {"kernels": "k_interp_pack + k_interp_tdi", "ms": ims, "bytes": ibytes,
+                                     "GB_per_s": ibytes / (ims * 1e-3) / 1e9,
+                                     "hbm_frac": ibytes / (ims * 1e-3) / (HBM_PEAK_GBS * 1e9)}
+        log(f"unfused template build (InterpTDI): {nb} templates, {ims:.3f} ms, "
+            f"{unfused['template_build']['GB_per_s']:.0f} GB/s ({unfused['template_build']['hbm_frac']:.2f} of HBM peak)")
+        del ir, spl, iargs
         del tc, tcs, dh_u, hh_u, args_u
 
     # gradient (11 parameters) of the same batch: the adjoint launch sequence
