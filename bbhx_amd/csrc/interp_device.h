@@ -83,8 +83,18 @@ __device__ inline void interp_point(cd h[3], double f, int i, const InterpBin& d
     h[0] = mk(0.0, 0.0);
     h[1] = mk(0.0, 0.0);
     h[2] = mk(0.0, 0.0);
+    // each mode's segment index is loaded one mode ahead, so that its latency
+    // overlaps the previous mode's work (the walker itself only waits on scalar
+    // loads); through a global-address-space pointer, since a generic (flat) load
+    // also counts against the scalar loads' counter and would be waited on with them
+    const __attribute__((address_space(1))) int* inds = (const __attribute__((address_space(1))) int*)d.inds;
+    int k_next = inds[i];
+    // f resident before the loop: a first wait for it inside the loop would be
+    // placed in every iteration and also wait for the prefetched index
+    asm volatile("" ::"v"(f));
     for (int m = 0; m < numModes; m++) {
-        const int k = d.inds[(long)m * d.len + i];
+        const int k = k_next;
+        if (m + 1 < numModes) k_next = inds[(long)(m + 1) * d.len + i];
         const long mbase = ((long)b * numModes + m) * length;
         bool pending = true;
 #pragma unroll 1

@@ -247,6 +247,69 @@ def test_interp_and_direct_like(backend):
     np.testing.assert_allclose(host(h_g), h_o, rtol=1e-9)
 
 
+def test_interp_arbitrary_segment_indices(backend):
+    """InterpTDI takes the per-bin segment indices as given (WaveformBuild.cu:80-100),
+    not only those of a sorted grid. The kernel reads a segment once per wavefront
+    when lanes share it (up to three distinct segments) and per lane otherwise, so
+    the indices here come in 64-bin blocks (one wavefront each) of: one segment;
+    2, 3 and 5 segments interleaved; a random index per bin in [-1, L - 1], which
+    includes the two out-of-range values (the mode is left out); and a sorted run.
+    Against the oracle, which loops over bins and modes one at a time; the phase
+    row is scaled down so that the cubics, evaluated away from their own
+    segment, stay well conditioned: 1e-9 relative."""
+    import torch
+
+    rng = np.random.default_rng(11)
+    B, M, L = 3, 3, 40
+    N = 3000
+    data_freqs = np.arange(1, N + 1) * 1e-5
+    # evenly spaced knots, jittered (no near-coincident knots, whose cubics blow up away from their segment)
+    x = np.linspace(2e-4, 3.2e-2, L)[None, None, :] + rng.uniform(-1e-4, 1e-4, (B, M, L))
+    y = rng.standard_normal(9 * B * M * L)
+    y[1 * B * M * L:2 * B * M * L] *= 1e-3
+    y[2 * B * M * L:3 * B * M * L] = rng.uniform(0, 1e7, B * M * L)
+    _, c1, c2, c3 = oracle.interpolate(x.ravel(), y, L, 9, M, B)
+    st = np.array([0, 100, 1000], dtype=np.int32)
+    n = np.array([2900, 1531, 700], dtype=np.int32)
+
+    def block(kind, size):
+        if kind == "one":
+            return np.full(size, rng.integers(0, L - 1))
+        if kind in (2, 3, 5):
+            segs = rng.choice(L - 1, kind, replace=False)
+            return segs[np.arange(size) % kind]
+        if kind == "random":
+            return rng.integers(-1, L, size)
+        return np.sort(rng.integers(0, L - 1, size))
+
+    kinds = ["one", 2, 3, 5, "random", "sorted"]
+    inds = []
+    for b in range(B):
+        rows = []
+        for m in range(M):
+            parts, done, j = [], 0, b + m
+            while done < n[b]:
+                size = min(64, n[b] - done)
+                parts.append(block(kinds[j % len(kinds)], size))
+                done += size
+                j += 1
+            rows.append(np.concatenate(parts))
+        inds.append(np.concatenate(rows).astype(np.int32))
+    assert any(np.any(a == -1) for a in inds) and any(np.any(a == L - 1) for a in inds)
+    # windows: none (t_end <= 0 means no end), a wide one, [0, 5e6] s
+    t_start = np.array([-1e30, -1e30, 0.0])
+    t_end = np.array([0.0, 1e30, 5e6])
+    o = oracle.interp_tdi(data_freqs, x.ravel(), y, c1, c2, c3, t_start, t_end, L, B, M, inds, st, n)
+    tm = [torch.zeros(3 * int(k), dtype=torch.complex128, device="cuda") for k in n]
+    ind_d = [dev(a, torch.int32) for a in inds]
+    backend.InterpTDI_wrap(np.array([t.data_ptr() for t in tm]), dev(data_freqs), dev(x.ravel()), dev(y), dev(c1),
+                           dev(c2), dev(c3), t_start, t_end, L, N, B, M, np.array([a.data_ptr() for a in ind_d]),
+                           st, n)
+    for gt, ot in zip(tm, o):
+        assert np.count_nonzero(ot) > 0
+        assert rel_err(host(gt).reshape(3, -1), ot) < 1e-9
+
+
 # ------------------------------------------------------------------ end to end
 
 
