@@ -25,31 +25,38 @@ using namespace bbhx;
 
 namespace {
 
-// Repack the spline arrays into one InterpSeg per (binary, mode, segment):
-// lane t -> (part p, segment s) with s fastest, so the reads of each spline
-// array are unit-stride; part 9 is the knot.
+// Segment records are built for a chunk of binaries at a time, at most this
+// many bytes (at least one binary), so that the workspace stays bounded for
+// batches whose spline arrays already fill much of the device.
+constexpr size_t INTERP_REC_CAP = size_t(256) << 20;
+
+// Repack the spline arrays of segments [s0, s0 + ns) into one InterpSeg each
+// (R[s - s0]): lane t -> (part p, segment s) with s fastest, so the reads of
+// each spline array are unit-stride; part 9 is the knot.
 __global__ __launch_bounds__(256) void k_interp_pack(InterpSeg* __restrict__ R, const double* __restrict__ freqs,
                                                      const double* __restrict__ y, const double* __restrict__ c1,
                                                      const double* __restrict__ c2, const double* __restrict__ c3,
-                                                     long nper)
+                                                     long nper, long s0, long ns)
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= 10 * nper) return;
-    const int p = (int)(t / nper);
-    const long s = t - p * nper;
+    if (t >= 10 * ns) return;
+    const int p = (int)(t / ns);
+    const long s = t - p * ns;
     if (p == 9) {
-        R[s].x = freqs[s];
+        R[s].x = freqs[s0 + s];
         return;
     }
-    const long q = p * nper + s;
+    const long q = p * nper + s0 + s;
     R[s].c[p][0] = y[q];
     R[s].c[p][1] = c1[q];
     R[s].c[p][2] = c2[q];
     R[s].c[p][3] = c3[q];
 }
 
-__global__ __launch_bounds__(INTERP_TILE) void k_interp_tdi(const InterpBin* __restrict__ bins, int numBinAll,
-                                                            const double* __restrict__ dataFreqs,
+// Tiles [tile0, tile0 + gridDim.x) of binaries [b0, b0 + nb), whose segment
+// records start at R.
+__global__ __launch_bounds__(INTERP_TILE) void k_interp_tdi(const InterpBin* __restrict__ bins, int b0, int nb,
+                                                            int tile0, const double* __restrict__ dataFreqs,
                                                             const InterpSeg* __restrict__ R, int length,
                                                             int numModes)
 {
@@ -57,13 +64,14 @@ __global__ __launch_bounds__(INTERP_TILE) void k_interp_tdi(const InterpBin* __r
     __shared__ double2 s_tab[129];
     fill_etab(s_tab, (int)threadIdx.x, INTERP_TILE);
     __syncthreads();
-    const int b = find_bin(bins, numBinAll, (int)blockIdx.x);
-    const InterpBin& d = bins[b];
-    const int i = ((int)blockIdx.x - d.tile_begin) * INTERP_TILE + threadIdx.x;
+    const int tile = tile0 + (int)blockIdx.x;
+    const int bl = find_bin(bins + b0, nb, tile);
+    const InterpBin& d = bins[b0 + bl];
+    const int i = (tile - d.tile_begin) * INTERP_TILE + threadIdx.x;
     if (i >= d.len) return;
     const double f = dataFreqs[d.start + i];
     cd h[3];
-    interp_point(h, f, i, d, b, R, length, numModes, s_tab);
+    interp_point(h, f, i, d, bl, R, length, numModes, s_tab);
     cd* out = reinterpret_cast<cd*>(d.tmpl);
     out[0 * (long)d.len + i] = h[0];
     out[1 * (long)d.len + i] = h[1];
@@ -132,11 +140,22 @@ extern "C" int bbhx_hip_InterpTDI(const int64_t* templateChannels_ptrs, const do
     InterpBin* bins = upload_table(h, st);
     if (!bins) return ERR_ALLOC;
     const long nper = (long)numBinAll * numModes * length;
-    InterpSeg* R = static_cast<InterpSeg*>(workspace_get(WS_INTERP, sizeof(InterpSeg) * (size_t)nper));
+    const long per_bin = (long)numModes * length;  // segment records per binary
+    long chunk = (long)(INTERP_REC_CAP / (sizeof(InterpSeg) * (size_t)per_bin));
+    chunk = chunk < 1 ? 1 : (chunk > numBinAll ? numBinAll : chunk);
+    InterpSeg* R = static_cast<InterpSeg*>(workspace_get(WS_INTERP, sizeof(InterpSeg) * (size_t)(per_bin * chunk)));
     if (!R) return ERR_ALLOC;
-    hipLaunchKernelGGL(k_interp_pack, dim3((unsigned)ceil_div(10 * nper, 256)), dim3(256), 0, st, R, freqs, propArrays,
-                       c1, c2, c3, nper);
-    hipLaunchKernelGGL(k_interp_tdi, dim3((unsigned)tiles), dim3(INTERP_TILE), 0, st, bins, numBinAll, dataFreqs, R,
-                       length, numModes);
-    return hipGetLastError() == hipSuccess ? OK : ERR_LAUNCH;
+    for (long b0 = 0; b0 < numBinAll; b0 += chunk) {
+        const long b1 = b0 + chunk < numBinAll ? b0 + chunk : numBinAll;
+        const int t0 = h[b0].tile_begin;
+        const int t1 = b1 < numBinAll ? h[b1].tile_begin : (int)tiles;
+        if (t1 == t0) continue;  // empty bands only
+        const long ns = (b1 - b0) * per_bin;
+        hipLaunchKernelGGL(k_interp_pack, dim3((unsigned)ceil_div(10 * ns, 256)), dim3(256), 0, st, R, freqs,
+                           propArrays, c1, c2, c3, nper, b0 * per_bin, ns);
+        hipLaunchKernelGGL(k_interp_tdi, dim3((unsigned)(t1 - t0)), dim3(INTERP_TILE), 0, st, bins, (int)b0,
+                           (int)(b1 - b0), t0, dataFreqs, R, length, numModes);
+        if (hipGetLastError() != hipSuccess) return ERR_LAUNCH;
+    }
+    return OK;
 }

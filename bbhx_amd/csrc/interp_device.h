@@ -67,7 +67,8 @@ __device__ inline void interp_add_segment(cd h[3], double f, const InterpSeg* __
     }
 }
 
-// Sum of the modes at dense frequency f (bin i of binary b):
+// Sum of the modes at dense frequency f (bin i of the binary d whose segment
+// records are R[(b numModes + m) length + k]):
 //   h_ch(f) = sum_m conj(T_ch,m A_m e^{i phi_m}) on segment k = inds[m][i]
 // (k <= -1 or k >= length - 1: the mode is left out). Dense bins are much finer
 // than the sparse grid over most of a band, so the lanes of a wavefront usually

@@ -310,6 +310,40 @@ def test_interp_arbitrary_segment_indices(backend):
         assert rel_err(host(gt).reshape(3, -1), ot) < 1e-9
 
 
+def test_interp_record_chunks(backend):
+    """InterpTDI packs its per-segment records for a chunk of binaries at a time
+    (at most 256 MiB of records, at least one binary). Two modes on 220 000-knot
+    grids make 134 MiB of records per binary, so three binaries run in three
+    chunks; each binary's template against the oracle, 1e-9 relative."""
+    import torch
+
+    rng = np.random.default_rng(12)
+    B, M, L = 3, 2, 220_000
+    N = 3000
+    data_freqs = np.arange(1, N + 1) * 1e-5
+    x = np.linspace(2e-4, 3.2e-2, L)[None, None, :] + rng.uniform(-3e-8, 3e-8, (B, M, L))
+    y = rng.standard_normal(9 * B * M * L)
+    y[1 * B * M * L:2 * B * M * L] *= 1e-3
+    y[2 * B * M * L:3 * B * M * L] = rng.uniform(0, 1e7, B * M * L)
+    assert M * L * 320 > (256 << 20) // 2
+    _, c1, c2, c3 = oracle.interpolate(x.ravel(), y, L, 9, M, B)
+    st = np.searchsorted(data_freqs, x.min(axis=(1, 2)), side="right").astype(np.int32)
+    et = np.searchsorted(data_freqs, x.max(axis=(1, 2)), side="right")
+    inds = [np.concatenate([np.searchsorted(x[i, j], data_freqs[st[i]:et[i]], side="right") - 1
+                            for j in range(M)]).astype(np.int32) for i in range(B)]
+    n = (et - st).astype(np.int32)
+    t_start, t_end = np.full(B, -1e30), np.zeros(B)
+    o = oracle.interp_tdi(data_freqs, x.ravel(), y, c1, c2, c3, t_start, t_end, L, B, M, inds, st, n)
+    tm = [torch.zeros(3 * int(k), dtype=torch.complex128, device="cuda") for k in n]
+    ind_d = [dev(a, torch.int32) for a in inds]
+    backend.InterpTDI_wrap(np.array([t.data_ptr() for t in tm]), dev(data_freqs), dev(x.ravel()), dev(y), dev(c1),
+                           dev(c2), dev(c3), t_start, t_end, L, N, B, M, np.array([a.data_ptr() for a in ind_d]),
+                           st, n)
+    for gt, ot in zip(tm, o):
+        assert np.count_nonzero(ot) == ot.size
+        assert rel_err(host(gt).reshape(3, -1), ot) < 1e-9
+
+
 # ------------------------------------------------------------------ end to end
 
 
