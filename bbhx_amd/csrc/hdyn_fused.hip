@@ -47,10 +47,11 @@ __global__ __launch_bounds__(64) void k_fused_setup(FusedRec* __restrict__ recs,
 // measured on MI355X at 4096 templates x 128 nodes, 0.23 ms against 0.30 ms for
 // one wavefront per SIMD without spills, and 0.38 ms for three per SIMD.
 __global__ __launch_bounds__(64 * MAX_RESIDUAL_WAVES) __attribute__((amdgpu_waves_per_eu(2))) void k_fused_eval(
-    double* __restrict__ d_h, double* __restrict__ h_h, const FusedRec* __restrict__ recs,
-    const double* __restrict__ params, const double* __restrict__ t_start, const double* __restrict__ t_end, int B,
-    int M, const double* __restrict__ freqs, const cd* __restrict__ h0, const cd* __restrict__ dc, int K,
-    resp::OrbitTab orb, int TDItag, int rescaled, int tdi2, int jend, double initial_t_val)
+    double* __restrict__ d_h, double* __restrict__ h_h, double* __restrict__ ll, double d_d,
+    const FusedRec* __restrict__ recs, const double* __restrict__ params, const double* __restrict__ t_start,
+    const double* __restrict__ t_end, int B, int M, const double* __restrict__ freqs, const cd* __restrict__ h0,
+    const cd* __restrict__ dc, int K, resp::OrbitTab orb, int TDItag, int rescaled, int tdi2, int jend,
+    double initial_t_val)
 {
     extern __shared__ cd rs[];  // [3][K]
     const int b = blockIdx.x;
@@ -68,6 +69,9 @@ __global__ __launch_bounds__(64 * MAX_RESIDUAL_WAVES) __attribute__((amdgpu_wave
         d_h[2 * b + 1] = sums[1];
         h_h[2 * b] = sums[2];
         h_h[2 * b + 1] = sums[3];
+        // log-likelihood (likelihood.py:640-648): 2 Re d_h is exact, so this is
+        // the value of the same expression formed from d_h, h_h afterwards
+        if (ll) ll[b] = -0.5 * ((d_d + sums[2]) - 2.0 * sums[0]);
     }
 }
 
@@ -154,7 +158,8 @@ extern "C" int bbhx_hip_hdyn_fused(double* d_h, double* h_h, const double* param
                                    int numModes, const double* freqs, const double* h0, const double* dataConstants,
                                    int K, const double* qnm, int table_len, double dspin, double initial_t_val,
                                    const bbhx_response* orbits, int TDItag, int rescaled, int tdi2,
-                                   int include_last_segment, int precision, void* stream)
+                                   int include_last_segment, int precision, double* logL, double d_d,
+                                   void* stream)
 {
     resp::OrbitTab orb;
     if (!response_orbit_tab(orbits, orb)) return ERR_ARGS;
@@ -182,7 +187,7 @@ extern "C" int bbhx_hip_hdyn_fused(double* d_h, double* h_h, const double* param
     const int jend = include_last_segment ? K : K - 1;
     const size_t lds = sizeof(cd) * 3 * (size_t)K;
     hipLaunchKernelGGL(k_fused_eval, dim3(numBinAll), dim3(64 * residual_waves(numBinAll, K)), lds, st, d_h, h_h,
-                       recs, params, t_start, t_end,
+                       logL, d_d, recs, params, t_start, t_end,
                        numBinAll, numModes, freqs, reinterpret_cast<const cd*>(h0),
                        reinterpret_cast<const cd*>(dataConstants), K, orb, TDItag, rescaled, tdi2, jend,
                        initial_t_val);

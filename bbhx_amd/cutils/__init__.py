@@ -71,7 +71,7 @@ def load_library():
         "bbhx_hip_direct_like": [_vp, _vp, _vp, _vp, _i64p, _ip, _ip, C.c_int, C.c_int, _vp],
         "bbhx_hip_hdyn_fused": [_vp, _vp, _vp, _vp, _vp, C.c_int, _ip, _ip, C.c_int, _vp, _vp, _vp, C.c_int, _vp,
                                 C.c_int, C.c_double, C.c_double, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
-                                _vp],
+                                _vp, C.c_double, _vp],
         "bbhx_hip_hdyn_fused_grad": [_vp, _vp, _vp, _vp, _vp, _vp, C.c_int, _ip, _ip, C.c_int, _vp, _vp, _vp,
                                      C.c_int, _vp, C.c_int, C.c_double, C.c_double, _vp, C.c_int, C.c_int, C.c_int,
                                      C.c_int, _vp],
@@ -313,8 +313,9 @@ def direct_like_wrap(d_h, h_h, dataChannels, noise_weight_times_df, templateChan
 
 
 def hdyn_fused(d_h, h_h, params, t_start, t_end, ells, mms, freqs, h0, data_constants, qnm, dspin, initial_t_val,
-               response, TDItag, rescaled, tdi2, include_last_segment=True, precision=0):
-    """Fused heterodyned likelihood (no reference counterpart; see include/bbhx_hip.h)."""
+               response, TDItag, rescaled, tdi2, include_last_segment=True, precision=0, logL=None, d_d=0.0):
+    """Fused heterodyned likelihood (no reference counterpart; see include/bbhx_hip.h).
+    ``logL`` (B,) float64 device, optional: also written, -1/2 (d_d + Re h_h - 2 Re d_h)."""
     torch = _torch()
     f64, c128 = torch.float64, torch.complex128
     B = int(params.shape[1])
@@ -326,7 +327,8 @@ def hdyn_fused(d_h, h_h, params, t_start, t_end, ells, mms, freqs, h0, data_cons
         _dev(d_h, c128), _dev(h_h, c128), _dev(params, f64, "params"), _dev(t_start, f64), _dev(t_end, f64), B,
         elp, mmp, int(el.size), _dev(freqs, f64), _dev(h0, c128, "h0"), _dev(data_constants, c128), K,
         _dev(qnm, f64), table_len, float(dspin), float(initial_t_val), response.handle, int(TDItag),
-        int(bool(rescaled)), int(bool(tdi2)), int(bool(include_last_segment)), int(precision), _stream())
+        int(bool(rescaled)), int(bool(tdi2)), int(bool(include_last_segment)), int(precision),
+        None if logL is None else _dev(logL, f64, "logL"), float(d_d), _stream())
     _check(st, "hdyn_fused")
 
 

@@ -354,11 +354,12 @@ class HeterodynedLikelihood(BBHxParallelModule):
     def get_ll_device(self, rows, t_start, t_end, precision=0, modes=None):
         """Fused log-likelihood of device-resident inputs (from ``fused_inputs``);
         returns a device tensor. No host transfer."""
-        d_h, h_h = self._fused_core(rows, t_start, t_end, modes, precision)
+        ll = self.xp.empty(rows.shape[1], dtype=self.xp.float64, device=self.device)
+        d_h, h_h = self._fused_core(rows, t_start, t_end, modes, precision, ll)
         self.hdyn_d_h, self.hdyn_h_h = d_h, h_h
-        return -1 / 2.0 * (self.reference_d_d + h_h - 2 * d_h).real
+        return ll
 
-    def _fused_core(self, rows, t_start, t_end, modes, precision):
+    def _fused_core(self, rows, t_start, t_end, modes, precision, ll=None):
         torch = self.xp
         gen = self.template_gen
         if modes is None:
@@ -374,7 +375,8 @@ class HeterodynedLikelihood(BBHxParallelModule):
         self.backend.hdyn_fused(d_h, h_h, rows, t_start, t_end, ells, mms, self.freqs, self._h0_flat,
                                 self.data_constants, gen.amp_phase_gen.qnm_packed, gen.amp_phase_gen.dspin,
                                 gen.amp_phase_gen.initial_t_val, resp.cpp_response, resp.TDItag_int, resp.rescaled,
-                                resp.tdi2, self.include_last_segment, precision)
+                                resp.tdi2, self.include_last_segment, precision, logL=ll,
+                                d_d=self.reference_d_d)
         gen.num_bin_all = B
         return d_h, h_h
 

@@ -156,16 +156,18 @@ int bbhx_hip_direct_like(double* d_h, double* h_h, const double* dataChannels, c
  * freqs (K,), h0 (3, K) complex, dataConstants (4, 3, K) complex, all device.
  * qnm: the PhenomD QNM table spline, 8 arrays of table_len entries packed
  *   contiguously (y_rd, c1_rd, c2_rd, c3_rd, y_dm, c1_dm, c2_dm, c3_dm).
- * precision: 0 = FP64 throughout; 1 = FP32 bulk / FP64 phase (phases,
- *   t(f) and the Doppler phase in FP64, transfer functions, amplitudes, the
- *   residuals and the heterodyne sums in FP32 with compensated summation).
- * Outputs d_h, h_h: (B,) complex. */
+ * precision: must be 0 (FP64 throughout; no reduced-precision form of the
+ *   heterodyned sums is built: other values return 1).
+ * Outputs d_h, h_h: (B,) complex; logL: (B,) device or null: when given, the
+ *   eval kernel also writes logL[b] = -1/2 (d_d + Re h_h - 2 Re d_h)
+ *   (likelihood.py:640-648, d_d = <d|d> of the data), so that a caller needs
+ *   no further kernel for it. */
 int bbhx_hip_hdyn_fused(double* d_h, double* h_h, const double* params, const double* t_start,
                         const double* t_end, int numBinAll, const int* ells_host, const int* mms_host,
                         int numModes, const double* freqs, const double* h0, const double* dataConstants,
                         int K, const double* qnm, int table_len, double dspin, double initial_t_val,
                         const bbhx_response* orbits, int TDItag, int rescaled, int tdi2,
-                        int include_last_segment, int precision, void* stream);
+                        int include_last_segment, int precision, double* logL, double d_d, void* stream);
 
 /* Gradient of the heterodyned log-likelihood (no counterpart in the reference,
  * which has no derivatives). Same inputs as bbhx_hip_hdyn_fused (precision is

@@ -416,3 +416,10 @@ def test_heterodyned_likelihood(backend, orbits, small_problem, K, last):
     assert np.all(np.isfinite(ll_f))
     assert np.max(np.abs(ll_p - ll_o)) < 1e-6, (ll_p, ll_o)
     assert np.max(np.abs(ll_f - ll_o)) < 1e-6, (ll_f, ll_o)
+    # the device entry point's logL, written by the eval kernel itself, is the
+    # same expression of the d_h, h_h it returns, bit for bit (2 Re d_h is exact)
+    rows, ts, te = het.fused_inputs(p)
+    ll_d = host(het.get_ll_device(rows, ts, te))
+    dh, hh = host(het.hdyn_d_h), host(het.hdyn_h_h)
+    assert np.array_equal(ll_d, -1 / 2.0 * (het.reference_d_d + hh - 2 * dh).real)
+    assert np.max(np.abs(ll_d - ll_f)) < 1e-9
