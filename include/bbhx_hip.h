@@ -212,7 +212,9 @@ int bbhx_hip_direct_plan_band(const bbhx_direct_plan* p, int* band_host);
  * cubics (and the reduction of the phase modulo 2 pi) and the noise-weighted
  * inner products stay FP64. The first switch to 1 allocates 192 B per (knot,
  * mode) of FP32-bulk segment records (call it outside graph capture). The
- * gradient entry point always computes in FP64. */
+ * gradient entry point always computes in FP64. In the FP32-bulk form the
+ * dense-band launch uses 3/2 x grid workgroups (its kernel holds 6 wavefronts
+ * per SIMD against 4), so a launch shape tuned for the FP64 form serves both. */
 int bbhx_hip_direct_plan_set_precision(bbhx_direct_plan* p, int precision);
 
 /* params (11, B) and t_start, t_end (B,) as for bbhx_hip_hdyn_fused; dataFreqs
