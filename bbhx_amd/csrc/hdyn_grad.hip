@@ -129,10 +129,11 @@ __device__ inline void grad_point(const GradRec& G, const resp::OrbitTab& o, dou
 // as the forward kernel computes them, and gw[(b * 3 + c) * K + j] = the
 // complex weight g of node j, channel c.
 __global__ __launch_bounds__(64 * MAX_RESIDUAL_WAVES) __attribute__((amdgpu_waves_per_eu(2))) void k_grad_weights(
-    double* __restrict__ d_h, double* __restrict__ h_h, cd* __restrict__ gw, const FusedRec* __restrict__ recs,
-    const double* __restrict__ params, const double* __restrict__ t_start, const double* __restrict__ t_end, int B,
-    int M, const double* __restrict__ freqs, const cd* __restrict__ h0, const cd* __restrict__ dc, int K,
-    resp::OrbitTab orb, int TDItag, int rescaled, int tdi2, int jend, double initial_t_val)
+    double* __restrict__ d_h, double* __restrict__ h_h, double* __restrict__ ll, double d_d, cd* __restrict__ gw,
+    const FusedRec* __restrict__ recs, const double* __restrict__ params, const double* __restrict__ t_start,
+    const double* __restrict__ t_end, int B, int M, const double* __restrict__ freqs, const cd* __restrict__ h0,
+    const cd* __restrict__ dc, int K, resp::OrbitTab orb, int TDItag, int rescaled, int tdi2, int jend,
+    double initial_t_val)
 {
     extern __shared__ cd rs[];  // residuals [3][K]
     const int b = blockIdx.x;
@@ -151,6 +152,8 @@ __global__ __launch_bounds__(64 * MAX_RESIDUAL_WAVES) __attribute__((amdgpu_wave
         d_h[2 * b + 1] = sums[1];
         h_h[2 * b] = sums[2];
         h_h[2 * b + 1] = sums[3];
+        // log-likelihood as the forward kernel writes it (likelihood.py:640-648)
+        if (ll) ll[b] = -0.5 * ((d_d + sums[2]) - 2.0 * sums[0]);
     }
 }
 
@@ -226,7 +229,7 @@ extern "C" int bbhx_hip_hdyn_fused_grad(double* d_h, double* h_h, double* grad, 
                                         const double* h0, const double* dataConstants, int K, const double* qnm,
                                         int table_len, double dspin, double initial_t_val,
                                         const bbhx_response* orbits, int TDItag, int rescaled, int tdi2,
-                                        int include_last_segment, void* stream)
+                                        int include_last_segment, double* logL, double d_d, void* stream)
 {
     resp::OrbitTab orb;
     if (!response_orbit_tab(orbits, orb)) return ERR_ARGS;
@@ -269,7 +272,7 @@ extern "C" int bbhx_hip_hdyn_fused_grad(double* d_h, double* h_h, double* grad, 
     if (timed) (void)hipEventRecord(ev[1], st);
     const int jend = include_last_segment ? K : K - 1;
     hipLaunchKernelGGL(k_grad_weights, dim3(numBinAll), dim3(64 * residual_waves(numBinAll, K)),
-                       sizeof(cd) * 3 * (size_t)K, st, d_h, h_h, gw, recs,
+                       sizeof(cd) * 3 * (size_t)K, st, d_h, h_h, logL, d_d, gw, recs,
                        params, t_start, t_end, numBinAll, numModes, freqs, reinterpret_cast<const cd*>(h0),
                        reinterpret_cast<const cd*>(dataConstants), K, orb, TDItag, rescaled, tdi2, jend,
                        initial_t_val);

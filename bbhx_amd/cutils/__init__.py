@@ -74,7 +74,7 @@ def load_library():
                                 _vp, C.c_double, _vp],
         "bbhx_hip_hdyn_fused_grad": [_vp, _vp, _vp, _vp, _vp, _vp, C.c_int, _ip, _ip, C.c_int, _vp, _vp, _vp,
                                      C.c_int, _vp, C.c_int, C.c_double, C.c_double, _vp, C.c_int, C.c_int, C.c_int,
-                                     C.c_int, _vp],
+                                     C.c_int, _vp, C.c_double, _vp],
         "bbhx_hip_direct_plan_create": [C.POINTER(_vp), C.c_int, _ip, C.c_int, C.c_int, C.c_int],
         "bbhx_hip_direct_plan_destroy": [_vp],
         "bbhx_hip_direct_plan_bytes": [_vp],
@@ -333,8 +333,9 @@ def hdyn_fused(d_h, h_h, params, t_start, t_end, ells, mms, freqs, h0, data_cons
 
 
 def hdyn_fused_grad(d_h, h_h, grad, params, t_start, t_end, ells, mms, freqs, h0, data_constants, qnm, dspin,
-                    initial_t_val, response, TDItag, rescaled, tdi2, include_last_segment=True):
-    """Heterodyned likelihood and its gradient (11, B) (no reference counterpart; see include/bbhx_hip.h)."""
+                    initial_t_val, response, TDItag, rescaled, tdi2, include_last_segment=True, logL=None, d_d=0.0):
+    """Heterodyned likelihood and its gradient (11, B) (no reference counterpart; see include/bbhx_hip.h).
+    ``logL`` (B,) float64 device, optional: also written, as by ``hdyn_fused``."""
     torch = _torch()
     f64, c128 = torch.float64, torch.complex128
     B = int(params.shape[1])
@@ -349,7 +350,7 @@ def hdyn_fused_grad(d_h, h_h, grad, params, t_start, t_end, ells, mms, freqs, h0
         _dev(t_end, f64), B, elp, mmp, int(el.size), _dev(freqs, f64), _dev(h0, c128, "h0"),
         _dev(data_constants, c128), K, _dev(qnm, f64), table_len, float(dspin), float(initial_t_val),
         response.handle, int(TDItag), int(bool(rescaled)), int(bool(tdi2)), int(bool(include_last_segment)),
-        _stream())
+        None if logL is None else _dev(logL, f64, "logL"), float(d_d), _stream())
     _check(st, "hdyn_fused_grad")
 
 

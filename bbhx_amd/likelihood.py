@@ -421,16 +421,18 @@ class HeterodynedLikelihood(BBHxParallelModule):
         d_h = torch.empty(B, dtype=torch.complex128, device=self.device)
         h_h = torch.empty(B, dtype=torch.complex128, device=self.device)
         grad = torch.empty((11, B), dtype=torch.float64, device=self.device)
+        ll = torch.empty(B, dtype=torch.float64, device=self.device)
         resp = gen.response_gen
         if not hasattr(self, "_h0_flat"):
             self._h0_flat = self.h0_sparse.reshape(3, -1).contiguous()
         self.backend.hdyn_fused_grad(d_h, h_h, grad, rows, t_start, t_end, ells, mms, self.freqs, self._h0_flat,
                                      self.data_constants, gen.amp_phase_gen.qnm_packed, gen.amp_phase_gen.dspin,
                                      gen.amp_phase_gen.initial_t_val, resp.cpp_response, resp.TDItag_int,
-                                     resp.rescaled, resp.tdi2, self.include_last_segment)
+                                     resp.rescaled, resp.tdi2, self.include_last_segment, logL=ll,
+                                     d_d=self.reference_d_d)
         gen.num_bin_all = B
         self.hdyn_d_h, self.hdyn_h_h = d_h, h_h
-        return -1 / 2.0 * (self.reference_d_d + h_h - 2 * d_h).real, grad
+        return ll, grad
 
     def get_ll_and_grad(self, x, **waveform_kwargs):
         """log-likelihood (B,) and gradient (B, 11) at sampling coordinates x

@@ -174,13 +174,14 @@ int bbhx_hip_hdyn_fused(double* d_h, double* h_h, const double* params, const do
  * FP64); computes d_h, h_h as it does and grad (11, B): the derivatives of
  * logL = Re d_h - 1/2 Re h_h (- 1/2 d_d) with respect to ln(m1 + m2), q = m1/m2,
  * chi1z, chi2z, distance [m], phi_ref, cos(inc), lam, sin(beta), psi, t_ref [s]
- * (row order). The observation window t_start/t_end is held fixed. K <= 1024. */
+ * (row order). The observation window t_start/t_end is held fixed. K <= 1024.
+ * logL (B,) device or null, d_d: as for bbhx_hip_hdyn_fused. */
 int bbhx_hip_hdyn_fused_grad(double* d_h, double* h_h, double* grad, const double* params, const double* t_start,
                              const double* t_end, int numBinAll, const int* ells_host, const int* mms_host,
                              int numModes, const double* freqs, const double* h0, const double* dataConstants, int K,
                              const double* qnm, int table_len, double dspin, double initial_t_val,
                              const bbhx_response* orbits, int TDItag, int rescaled, int tdi2,
-                             int include_last_segment, void* stream);
+                             int include_last_segment, double* logL, double d_d, void* stream);
 
 /* ---- fused direct likelihood of a ragged batch (no counterpart in the reference) ---- */
 

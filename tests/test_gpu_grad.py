@@ -82,6 +82,10 @@ def test_grad_value_matches_forward(grad_problem):
     assert np.all(np.isfinite(grad))
     assert np.max(np.abs(ll_g - ll_f)) < 1e-7
     assert np.max(np.abs(ll_g - ll_o)) < 1e-6
+    # the logL the gradient entry point writes is the same expression of its d_h, h_h
+    het.get_ll_and_grad(X)
+    dh, hh = het.hdyn_d_h.cpu().numpy(), het.hdyn_h_h.cpu().numpy()
+    assert np.array_equal(ll_g, -1 / 2.0 * (het.reference_d_d + hh - 2 * dh).real)
 
 
 def test_grad_against_oracle_finite_differences(grad_problem):
