@@ -179,6 +179,29 @@ def test_spline(backend):
     np.testing.assert_allclose(host(spl.c3_shaped[4, 2, 1, :-1]), cs.c[0], rtol=1e-7, atol=1e-7 * np.abs(cs.c[0]).max())
 
 
+def test_spline_chunked_launches(backend):
+    """The batched spline runs its systems in launches of at most 2.5 GiB of
+    arrays (x, y and three outputs, 40 B per knot): 2^20 knots allow 64
+    systems per launch, so 9 quantities x 8 binaries (72 systems, the
+    quantities sharing each binary's knots) take two launches. The systems on
+    both sides of the boundary (63: quantity 7 of binary 7; 64: quantity 8 of
+    binary 0) and the last one against the oracle, 1e-9 relative."""
+    import torch
+
+    from bbhx_amd.utils.interpolate import CubicSplineInterpolant
+
+    rng = np.random.default_rng(4)
+    B, M, P, L = 8, 1, 9, 2**20
+    x = np.cumsum(rng.uniform(0.5, 1.5, (B, M, L)), axis=-1) * 1e-6
+    y = np.sin(3e3 * x[None] * (1 + np.arange(P)[:, None, None, None])) + 0.1 * rng.standard_normal((P, B, M, L))
+    spl = CubicSplineInterpolant(dev(x), dev(y), force_backend=backend)
+    g = [host(c).reshape(P, B, M, L) for c in (spl.c1, spl.c2, spl.c3)]
+    for p, b in ((7, 7), (8, 0), (8, 7)):
+        _, c1o, c2o, c3o = oracle.interpolate(x[b].ravel(), y[p, b].ravel(), L, 1, 1, 1)
+        for gc, oc in zip(g, (c1o, c2o, c3o)):
+            assert rel_err(gc[p, b, 0, :-1], oc[:-1]) < 1e-9, (p, b)
+
+
 def test_direct_sum_and_hdyn(backend):
     import torch
 
