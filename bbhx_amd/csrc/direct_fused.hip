@@ -15,9 +15,10 @@
 //                 t = 0 and t = Tobs, the grid refitted to that window
 //                 (phenomhm.py:208-224, :509-614), then amplitude, phase, t(f)
 //                 and the TDI response on the refitted grid (phenomhm.py:487-507,
-//                 Response.cu)
+//                 Response.cu), written compact: nine values per (knot, mode)
 //   k_dir_spline  one wavefront per binary, one lane per (mode, quantity):
-//                 not-a-knot spline coefficients (Interpolate.cu:15-359)
+//                 not-a-knot spline coefficients (Interpolate.cu:15-359) from
+//                 the compact values, each record's (y, c1, c2, c3) written together
 //   k_dir_band    dense band [st, et) of each binary (waveformbuild.py:139-156)
 //                 and the exclusive prefix of its chunks of dense bins (chunks
 //                 aligned to absolute multiples of the chunk size)
@@ -58,6 +59,7 @@ struct bbhx_direct_plan {
     FusedRec* recs = nullptr;
     double* X = nullptr;   // (S, M)
     double* Cf = nullptr;  // (S, M, RS): segment records (bounds, value and cubic coefficients)
+    double* Yc = nullptr;  // (S, M, NQ): the grid pass's nine values per knot, compact
     double* grid_lo = nullptr;    // (B, M) log2 of the first refitted grid point
     double* grid_istep = nullptr; // (B, M) 1 / log2 step (0 for a degenerate grid)
     int* band = nullptr;          // (B, 3): st, et, chunk_begin
@@ -159,7 +161,7 @@ __device__ inline double wmax_d(double v)
 }
 
 // One 64-lane wavefront per (binary, mode). LDS: t(f) of the default grid (L_b doubles).
-__global__ __launch_bounds__(64) void k_dir_grid(double* __restrict__ X, double* __restrict__ Cf,
+__global__ __launch_bounds__(64) void k_dir_grid(double* __restrict__ X, double* __restrict__ Yc,
                                                  double* __restrict__ grid_lo, double* __restrict__ grid_istep,
                                                  int* __restrict__ flags, const FusedRec* __restrict__ recs,
                                                  const long* __restrict__ off, const int* __restrict__ len,
@@ -259,16 +261,16 @@ __global__ __launch_bounds__(64) void k_dir_grid(double* __restrict__ X, double*
         phase = phase + dphi;
         const long p = (base + i) * M + m;
         X[p] = f;
-        double* y = Cf + p * RS + RQ;
-        y[0 * 4] = amp;
-        y[1 * 4] = phase;
-        y[2 * 4] = tf;
-        y[3 * 4] = T[0].re;
-        y[4 * 4] = T[0].im;
-        y[5 * 4] = T[1].re;
-        y[6 * 4] = T[1].im;
-        y[7 * 4] = T[2].re;
-        y[8 * 4] = T[2].im;
+        double* y = Yc + p * NQ;
+        y[0] = amp;
+        y[1] = phase;
+        y[2] = tf;
+        y[3] = T[0].re;
+        y[4] = T[0].im;
+        y[5] = T[1].re;
+        y[6] = T[1].im;
+        y[7] = T[2].re;
+        y[8] = T[2].im;
     }
 }
 
@@ -277,7 +279,8 @@ constexpr int DSPL_BLOCK = 8;  // rows loaded ahead of the recurrence in each sw
 // Not-a-knot spline of one binary: lane s = mode * 9 + quantity runs the
 // Thomas algorithm over the binary's L knots (the system of spline.hip);
 // consecutive lanes read consecutive quantities of the same knot.
-__global__ __launch_bounds__(64) void k_dir_spline(const double* __restrict__ X, double* __restrict__ Cf,
+__global__ __launch_bounds__(64) void k_dir_spline(const double* __restrict__ X, const double* __restrict__ Yc,
+                                                   double* __restrict__ Cf,
                                                    SegRecF* __restrict__ CfF, const long* __restrict__ off,
                                                    const int* __restrict__ len, int M)
 {
@@ -291,7 +294,7 @@ __global__ __launch_bounds__(64) void k_dir_spline(const double* __restrict__ X,
     auto xi = [&](int i) { return X[(base + i) * M + m]; };
     const int q = s - m * NQ;
     auto at = [&](int i) { return ((base + i) * M + m) * RS + RQ + q * 4; };
-    auto yi = [&](int i) { return Cf[at(i)]; };
+    auto yi = [&](int i) { return Yc[((base + i) * M + m) * NQ + q]; };
     double* dp = Cf + 1;  // d' then the slopes (= c1)
     double* C2 = Cf + 2;
     double* cp = Cf + 3;  // c' then c3
@@ -375,6 +378,7 @@ __global__ __launch_bounds__(64) void k_dir_spline(const double* __restrict__ X,
     }
     double snext = dp[at(n - 1)];
     double xn = xi(n - 1), yn = yi(n - 1);
+    Cf[at(n - 1)] = yn;
     dp[at(n - 1)] = 0.0;
     C2[at(n - 1)] = 0.0;
     cp[at(n - 1)] = 0.0;
@@ -403,6 +407,7 @@ __global__ __launch_bounds__(64) void k_dir_spline(const double* __restrict__ X,
         const double slope = (yn - y_i) * idx;
         const double t = (si + snext - 2 * slope) * idx;
         const double c2 = (slope - si) * idx - t, c3 = t * idx;
+        Cf[at(i)] = y_i;
         dp[at(i)] = si;
         C2[at(i)] = c2;
         cp[at(i)] = c3;
@@ -1193,7 +1198,7 @@ T* dalloc(size_t n, bool& ok)
 
 void plan_free(bbhx_direct_plan* p)
 {
-    void* bufs[] = {p->off, p->len, p->recs, p->X, p->Cf, p->grid_lo, p->grid_istep, p->band,
+    void* bufs[] = {p->off, p->len, p->recs, p->X, p->Cf, p->Yc, p->grid_lo, p->grid_istep, p->band,
                     p->total, p->partial, p->perm, p->cnt, p->next, p->Yadj, p->grecs, p->gpart, p->CfF};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
@@ -1238,6 +1243,7 @@ extern "C" int bbhx_hip_direct_plan_create(bbhx_direct_plan** out, int numBinAll
     p->recs = dalloc<FusedRec>((size_t)numBinAll * numModes, ok);
     p->X = dalloc<double>(SM, ok);
     p->Cf = dalloc<double>(SM * RS, ok);
+    p->Yc = dalloc<double>(SM * NQ, ok);
     p->grid_lo = dalloc<double>((size_t)numBinAll * numModes, ok);
     p->grid_istep = dalloc<double>((size_t)numBinAll * numModes, ok);
     p->band = dalloc<int>((size_t)numBinAll * 3, ok);
@@ -1275,7 +1281,7 @@ extern "C" long bbhx_hip_direct_plan_bytes(const bbhx_direct_plan* p)
 {
     if (!p) return -1;
     const long SM = p->S * p->M;
-    long bytes = SM * 8 * (1 + RS) + (long)p->B * p->M * (sizeof(FusedRec) + 16) + p->max_items * 36 + p->B * 24 +
+    long bytes = SM * 8 * (1 + RS + NQ) + (long)p->B * p->M * (sizeof(FusedRec) + 16) + p->max_items * 36 + p->B * 24 +
                  4L * (p->max_chunks + 1);
     if (p->Yadj) bytes += SM * 8 * NA + (long)p->B * p->M * (sizeof(bbhx::gradk::GradRec) + 8 * bbhx::gradk::NGRAD);
     if (p->CfF) bytes += SM * (long)sizeof(SegRecF);
@@ -1329,12 +1335,12 @@ int direct_launch(bbhx_direct_plan* p, double* d_h, double* h_h, double* grad, i
     hipLaunchKernelGGL(k_dir_setup, dim3(SETUP_PARTS * ceil_div((long)B * M, 64)), dim3(64), 0, st, p->recs, params, B,
                        ml, M, qnm,
                        table_len, dspin);
-    hipLaunchKernelGGL(k_dir_grid, dim3(B * M), dim3(64), sizeof(double) * (size_t)p->Lmax, st, p->X, p->Cf,
+    hipLaunchKernelGGL(k_dir_grid, dim3(B * M), dim3(64), sizeof(double) * (size_t)p->Lmax, st, p->X, p->Yc,
                        p->grid_lo, p->grid_istep, flags, p->recs, p->off, p->len, params, t_start, t_end, B, M, ml,
                        orb, TDItag, rescaled, tdi2, initial_t_val, mf_min, mf_max);
     // the FP32-bulk form (not for the gradient, whose adjoint pass reads the FP64 records)
     const bool f32 = !grad && p->precision == 1;
-    hipLaunchKernelGGL(k_dir_spline, dim3(B), dim3(64), 0, st, p->X, p->Cf,
+    hipLaunchKernelGGL(k_dir_spline, dim3(B), dim3(64), 0, st, p->X, p->Yc, p->Cf,
                        f32 ? static_cast<SegRecF*>(p->CfF) : nullptr, p->off, p->len, M);
     hipLaunchKernelGGL(k_dir_band, dim3(1), dim3(1024), 0, st, p->band, p->total, p->X, p->off, p->len, dataFreqs,
                        p->N, B, M, chunk);
