@@ -16,7 +16,8 @@
 //   k_prep_bounds   one workgroup: segment starts (binary search), item counts,
 //                   exclusive scan
 //   k_prep_items    persistent: one item per workgroup pass, 27 partial sums
-//   k_prep_reduce   one lane per (slot, channel, constant) + the totals
+//   k_prep_reduce   one lane per (slot, channel, constant)
+//   k_prep_totals   one wavefront per total
 
 #include "bbhx_hip.h"
 #include "common.h"
@@ -155,26 +156,35 @@ __global__ __launch_bounds__(256) void k_prep_items(double* __restrict__ partial
 }
 
 // lanes [0, 24 K): constant slot s = lane / 24, component p = lane % 24 (slot 0 and
-// the unused imaginary parts of B0, B1 stay zero); lanes [24 K, 24 K + 3): totals.
-__global__ void k_prep_reduce(double* __restrict__ consts, double* __restrict__ totals,
-                              const double* __restrict__ partial, const int* __restrict__ item, int K)
+// the unused imaginary parts of B0, B1 stay zero); a segment holds a few items.
+__global__ void k_prep_reduce(double* __restrict__ consts, const double* __restrict__ partial,
+                              const int* __restrict__ item, int K)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < 24 * K) {
-        const int slot = t / 24, p = t % 24;
-        const int c = p >> 1, ri = p & 1;  // c = constant * 3 + channel
-        double s = 0.0;
-        if (slot > 0) {
-            const int j = slot - 1;
-            for (int it = item[j]; it < item[j + 1]; it++) s += partial[(long)it * NP + p];
-        }
-        consts[2 * ((long)c * K + slot) + ri] = s;
-    } else if (t < 24 * K + 3) {
-        const int p = 24 + (t - 24 * K);
-        double s = 0.0;
-        for (int it = 0; it < item[K]; it++) s += partial[(long)it * NP + p];
-        totals[p - 24] = s;
+    if (t >= 24 * K) return;
+    const int slot = t / 24, p = t % 24;
+    const int c = p >> 1, ri = p & 1;  // c = constant * 3 + channel
+    double s = 0.0;
+    if (slot > 0) {
+        const int j = slot - 1;
+        for (int it = item[j]; it < item[j + 1]; it++) s += partial[(long)it * NP + p];
     }
+    consts[2 * ((long)c * K + slot) + ri] = s;
+}
+
+// The three totals (d_d, h_h, Re d_h) sum every work item: one wavefront per
+// total, lane l taking items l, l + 64, ... and the lanes combined by a fixed
+// shuffle tree (deterministic; a single lane's dependent loop over the ~N/2048
+// items took 140 us at N = 2^20).
+__global__ __launch_bounds__(192) void k_prep_totals(double* __restrict__ totals, const double* __restrict__ partial,
+                                                     const int* __restrict__ item, int K)
+{
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int p = 24 + w;
+    double s = 0.0;
+    for (int it = lane; it < item[K]; it += 64) s += partial[(long)it * NP + p];
+    s = wsum64(s);
+    if (lane == 0) totals[w] = s;
 }
 
 }  // namespace
@@ -199,7 +209,7 @@ extern "C" int bbhx_hip_hdyn_prep(double* dataConstants, double* totals, const d
     hipLaunchKernelGGL(k_prep_items, dim3(1024), dim3(256), 0, st, partial, seg, item, dataFreqs,
                        reinterpret_cast<const cd*>(dataChannels), reinterpret_cast<const cd*>(h0), psd,
                        data_length, nodes, K, df);
-    hipLaunchKernelGGL(k_prep_reduce, dim3(ceil_div(24L * K + 3, 256)), dim3(256), 0, st, dataConstants, totals,
-                       partial, item, K);
+    hipLaunchKernelGGL(k_prep_reduce, dim3(ceil_div(24L * K, 256)), dim3(256), 0, st, dataConstants, partial, item, K);
+    hipLaunchKernelGGL(k_prep_totals, dim3(1), dim3(192), 0, st, totals, partial, item, K);
     return hipGetLastError() == hipSuccess ? OK : ERR_LAUNCH;
 }
