@@ -180,6 +180,21 @@ def lisa_response(buffer, ells, mms, freqs, phi_ref, inc, lam, beta, psi, TDItag
         _d(n_arr), _d(x_arr), _i(links), C.c_double(orbit["armlength"]))
 
 
+def sylm(l, m, theta, phi):
+    """Spin-weight -2 spherical harmonic used by the response (Response.cu:383-424)."""
+    out = np.zeros(2)
+    _load().orc_sylm(C.c_int(l), C.c_int(m), C.c_double(theta), C.c_double(phi), _d(out))
+    return complex(out[0], out[1])
+
+
+def polarization(lam, beta, psi):
+    """Wave vector k (3,) and polarisation tensors H+, Hx (3, 3) in the SSB frame
+    (Response.cu:503-568)."""
+    k, hp, hc = np.zeros(3), np.zeros(9), np.zeros(9)
+    _load().orc_polarization(C.c_double(lam), C.c_double(beta), C.c_double(psi), _d(k), _d(hp), _d(hc))
+    return k, hp.reshape(3, 3), hc.reshape(3, 3)
+
+
 def interpolate(x, y, length, num_interp_params, num_modes, num_bin_all):
     """Not-a-knot spline coefficients (Interpolate.cu:320-359). Returns (B, c1, c2, c3)."""
     x, y = _f64(x), _f64(y)

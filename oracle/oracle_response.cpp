@@ -186,6 +186,31 @@ void matmul3(double* out, const double* a, const double* b)
         }
 }
 
+// Wave vector and polarisation tensors in the SSB frame, Response.cu:503-568:
+// H+ = O1 diag(1, -1, 0) O1^-1, Hx = O1 offdiag(1, 1) O1^-1.
+void polarization(double lam, double beta, double psi, double k[3], double Hp[9], double Hc[9])
+{
+    const double HSplus[9] = {1., 0., 0., 0., -1., 0., 0., 0., 0.};
+    const double HScross[9] = {0., 1., 0., 1., 0., 0., 0., 0., 0.};
+    double cl = std::cos(lam), sl = std::sin(lam);
+    double cb = std::cos(beta), sb = std::sin(beta);
+    double cp = std::cos(psi), sp = std::sin(psi);
+    k[0] = -std::cos(beta) * std::cos(lam);
+    k[1] = -std::cos(beta) * std::sin(lam);
+    k[2] = -std::sin(beta);
+    double O1[9] = {cp * sl - cl * sb * sp, -cl * cp * sb - sl * sp, -cb * cl,
+                    -cl * cp - sb * sl * sp, -cp * sb * sl + cl * sp, -cb * sl,
+                    cb * sp, cb * cp, -sb};
+    double invO1[9] = {cp * sl - cl * sb * sp, -cl * cp - sb * sl * sp, cb * sp,
+                       -cl * cp * sb - sl * sp, -cp * sb * sl + cl * sp, cb * cp,
+                       -cb * cl, -cb * sl, -sb};
+    double tmp[9];
+    matmul3(tmp, HSplus, invO1);
+    matmul3(Hp, O1, tmp);
+    matmul3(tmp, HScross, invO1);
+    matmul3(Hc, O1, tmp);
+}
+
 }  // namespace
 }  // namespace orc
 
@@ -211,24 +236,8 @@ void orc_lisa_response(double* buffer, const int* ells, const int* mms, const do
 
 #pragma omp parallel for schedule(dynamic)
     for (int b = 0; b < numBinAll; b++) {
-        // polarisation tensors in the SSB frame, Response.cu:503-568
-        const double HSplus[9] = {1., 0., 0., 0., -1., 0., 0., 0., 0.};
-        const double HScross[9] = {0., 1., 0., 1., 0., 0., 0., 0., 0.};
-        double cl = std::cos(lam[b]), sl = std::sin(lam[b]);
-        double cb = std::cos(beta[b]), sb = std::sin(beta[b]);
-        double cp = std::cos(psi[b]), sp = std::sin(psi[b]);
-        double k[3] = {-std::cos(beta[b]) * std::cos(lam[b]), -std::cos(beta[b]) * std::sin(lam[b]), -std::sin(beta[b])};
-        double O1[9] = {cp * sl - cl * sb * sp, -cl * cp * sb - sl * sp, -cb * cl,
-                        -cl * cp - sb * sl * sp, -cp * sb * sl + cl * sp, -cb * sl,
-                        cb * sp, cb * cp, -sb};
-        double invO1[9] = {cp * sl - cl * sb * sp, -cl * cp - sb * sl * sp, cb * sp,
-                           -cl * cp * sb - sl * sp, -cp * sb * sl + cl * sp, cb * cp,
-                           -cb * cl, -cb * sl, -sb};
-        double tmp[9], Hp[9], Hc[9];
-        matmul3(tmp, HSplus, invO1);
-        matmul3(Hp, O1, tmp);
-        matmul3(tmp, HScross, invO1);
-        matmul3(Hc, O1, tmp);
+        double k[3], Hp[9], Hc[9];
+        polarization(lam[b], beta[b], psi[b], k, Hp, Hc);
 
         for (int mi = 0; mi < numModes; mi++) {
             int l = ells[mi], m = mms[mi];
@@ -271,4 +280,21 @@ void orc_lisa_response(double* buffer, const int* ells, const int* mms, const do
     }
 }
 
+// Spin-weight -2 spherical harmonic of the response (Response.cu:383-424);
+// out = (re, im). Exported for the oracle's own pins.
+void orc_sylm(int l, int m, double theta, double phi, double* out)
+{
+    const cplx y = sYlm(l, m, theta, phi);
+    out[0] = y.real();
+    out[1] = y.imag();
+}
+
+// Wave vector k (3) and polarisation tensors H+ (9), Hx (9), row-major
+// (Response.cu:503-568). Exported for the oracle's own pins.
+void orc_polarization(double lam, double beta, double psi, double* k, double* Hp, double* Hc)
+{
+    polarization(lam, beta, psi, k, Hp, Hc);
+}
+
 }  // extern "C"
+
