@@ -108,15 +108,24 @@ __device__ inline bool in_window(const Window& w, double tf)
 }
 
 // Residual pass of one template by one workgroup: thread t owns nodes t,
-// t + blockDim.x, ...; for each node the three channels are summed over the
-// modes (in mode order) and r = h / h0 is written to rs[c * K + j] (LDS).
+// t + blockDim.x, ...; r = h / h0 is written to rs[c * K + j] (LDS).
+// Per node the modes are summed before the TDI combination: with link terms
+// G_m and weights w_m = A_m e^{i phi_m} (0 outside the observation window),
+// sum_m conj(T_c(G_m) w_m) = conj(T_c(sum_m w_m G_m)) because the TDI
+// combination T_c is linear in the link terms with coefficients that depend
+// on the frequency alone (WaveformBuild.cu:48-60 sums conj(T A e^{i phi}) mode
+// by mode; the same sum, reassociated). 'tab' is the e^{i theta} table.
+template <class E>
 __device__ inline void residuals(cd* rs, const FusedRec* __restrict__ rb, int M, const Window& win,
                                  const double* __restrict__ freqs, const cd* __restrict__ h0, int K,
-                                 const resp::OrbitTab& orb, int TDItag, int rescaled, int tdi2)
+                                 const resp::OrbitTab& orb, int TDItag, int rescaled, int tdi2,
+                                 const E* __restrict__ tab)
 {
     for (int j = threadIdx.x; j < K; j += blockDim.x) {
         const double f = freqs[j];
-        cd h[3] = {mk(0.0, 0.0), mk(0.0, 0.0), mk(0.0, 0.0)};
+        const resp::NodeResp nr = resp::resp_node(orb, f);
+        cd S[6];
+        for (int k = 0; k < 6; k++) S[k] = mk(0.0, 0.0);
         for (int mi = 0; mi < M; mi++) {
             const FusedRec& r = rb[mi];
             double amp, phase, tf;
@@ -124,24 +133,32 @@ __device__ inline void residuals(cd* rs, const FusedRec* __restrict__ rb, int M,
             // phenomhm.py:487-507: phase += 2 pi f (t_ref - t0); tf += t_ref
             phase = phase + ((f * win.dt_shift) * 2) * PI;
             tf = tf + win.t_ref;
-            cd T[3];
+            cd G[6];
             double dphi;
-            resp::resp_eval(r.s, orb, f, tf, TDItag, rescaled, tdi2, T, dphi);
+            resp::resp_links(r.s, orb, nr, f, tf, G, dphi, tab);
             phase = phase + dphi;
             if (in_window(win, tf)) {
-                const cd ap = amp * cexpi_red(phase);
-                h[0] += conj(T[0] * ap);
-                h[1] += conj(T[1] * ap);
-                h[2] += conj(T[2] * ap);
+                const cd ap = amp * cexpi_tab(phase, tab);
+                for (int k = 0; k < 6; k++) S[k] += G[k] * ap;
             }
         }
-        // r = h / h0 (likelihood.py:603)
+        cd T[3];
+        resp::tdi_combine(S, nr, TDItag, rescaled, tdi2, T);
+        // r = h / h0 with h = conj(T) (likelihood.py:603)
         for (int c = 0; c < 3; c++) {
             const cd d = h0[c * K + j];
+            const cd h = conj(T[c]);
             const double iden = rcp_pos(d.re * d.re + d.im * d.im);
-            rs[c * K + j] = mk((h[c].re * d.re + h[c].im * d.im) * iden, (h[c].im * d.re - h[c].re * d.im) * iden);
+            rs[c * K + j] = mk((h.re * d.re + h.im * d.im) * iden, (h.im * d.re - h.re * d.im) * iden);
         }
     }
+}
+
+// The e^{i theta} table of cexpi_tab copied from constant memory into LDS
+// ('tab', 129 entries) by the threads of a workgroup; a barrier must follow.
+__device__ inline void etab_to_lds(double2* tab)
+{
+    for (int j = threadIdx.x; j < 129; j += blockDim.x) tab[j] = make_double2(ETAB[j].x, ETAB[j].y);
 }
 
 // Heterodyned sums (Likelihood.cu:52-111) over the segments j = t+1,

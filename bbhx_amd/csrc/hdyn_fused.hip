@@ -54,13 +54,16 @@ __global__ __launch_bounds__(64 * MAX_RESIDUAL_WAVES) __attribute__((amdgpu_wave
     double initial_t_val)
 {
     extern __shared__ cd rs[];  // [3][K]
+    __shared__ double2 tab[129];
+    etab_to_lds(tab);
     const int b = blockIdx.x;
     Window win;
     win.t_ref = params[P_TREF * B + b];
     win.ts = t_start[b];
     win.te = t_end[b];
     win.dt_shift = win.t_ref - initial_t_val;
-    residuals(rs, recs + (long)b * M, M, win, freqs, h0, K, orb, TDItag, rescaled, tdi2);
+    __syncthreads();
+    residuals(rs, recs + (long)b * M, M, win, freqs, h0, K, orb, TDItag, rescaled, tdi2, tab);
     __syncthreads();
     double sums[4];
     segment_sums(rs, freqs, dc, K, jend, sums);

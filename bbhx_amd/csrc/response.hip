@@ -20,6 +20,7 @@ using namespace bbhx::resp;
 struct bbhx_response {
     const double* x = nullptr;  // (N, 3, 3) device
     const double* n = nullptr;  // (N, 6, 3) device
+    double* packed = nullptr;   // (N, ORB_ROW) device, owned: packed rows of x and n
     double dt = 0.0;
     int N = 0;
     int li12 = -1, li23 = -1, li31 = -1;
@@ -69,15 +70,34 @@ __global__ __launch_bounds__(EVAL_THREADS) void k_resp_eval(double* __restrict__
 
 inline hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
+// Packed orbit rows (OrbitTab::p): one thread per (row, value).
+__global__ __launch_bounds__(256) void k_orbit_pack(double* __restrict__ p, const double* __restrict__ x,
+                                                    const double* __restrict__ n, int N, int li12, int li23, int li31)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)N * ORB_VALS) return;
+    const int w = (int)(i / ORB_VALS), q = (int)(i % ORB_VALS);
+    auto val = [&](int row) {
+        if (q < 9) return x[(long)row * 9 + q];
+        const int j = (q - 9) / 3, c = (q - 9) % 3;
+        const int li = j == 0 ? li12 : (j == 1 ? li23 : li31);
+        return n[(long)row * 18 + 3 * li + c];
+    };
+    const double v = val(w);
+    p[(long)w * ORB_ROW + q] = v;
+    p[(long)w * ORB_ROW + ORB_VALS + q] = w + 1 < N ? val(w + 1) - v : 0.0;
+}
+
 }  // namespace
 
 namespace bbhx {
 // Used by the fused kernels: orbit table view of a response object.
 bool response_orbit_tab(const bbhx_response* r, OrbitTab& o)
 {
-    if (!r || !r->x || !r->n || r->N < 2 || r->li12 < 0 || r->li23 < 0 || r->li31 < 0) return false;
+    if (!r || !r->x || !r->n || !r->packed || r->N < 2 || r->li12 < 0 || r->li23 < 0 || r->li31 < 0) return false;
     o.x = r->x;
     o.n = r->n;
+    o.p = r->packed;
     o.dt = r->dt;
     o.inv_dt = 1.0 / r->dt;
     o.N = r->N;
@@ -101,6 +121,10 @@ int bbhx_hip_response_create(bbhx_response** out)
 
 int bbhx_hip_response_destroy(bbhx_response* r)
 {
+    if (r && r->packed) {
+        (void)hipDeviceSynchronize();
+        (void)hipFree(r->packed);
+    }
     delete r;
     return OK;
 }
@@ -120,6 +144,23 @@ int bbhx_hip_response_add_orbits(bbhx_response* r, double dt, int N, const doubl
         if (links[j] == 31) li31 = j;
     }
     if (li12 < 0 || li23 < 0 || li31 < 0) return ERR_ARGS;
+    // packed rows, built once here (device-wide synchronisation on both sides:
+    // the caller's arrays may come from any stream, and kernels of an earlier
+    // table may still read the old rows)
+    if (hipDeviceSynchronize() != hipSuccess) return ERR_LAUNCH;
+    if (r->packed) {
+        (void)hipFree(r->packed);
+        r->packed = nullptr;
+    }
+    double* packed = nullptr;
+    if (hipMalloc(&packed, sizeof(double) * ORB_ROW * (size_t)N) != hipSuccess) return ERR_ALLOC;
+    hipLaunchKernelGGL(k_orbit_pack, dim3(ceil_div(N * ORB_VALS, 256)), dim3(256), 0, nullptr, packed, x_arr, n_arr, N,
+                       li12, li23, li31);
+    if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+        (void)hipFree(packed);
+        return ERR_LAUNCH;
+    }
+    r->packed = packed;
     r->x = x_arr;
     r->n = n_arr;
     r->dt = dt;

@@ -37,9 +37,18 @@ struct RespRec {
     cd Yp, Yc;          // (Ylm + (-1)^l Yl-m*)/2 and i (Ylm - (-1)^l Yl-m*)/2
 };
 
+// Packed orbit rows (built once per orbit table, bbhx_hip_response_add_orbits):
+// row w holds the 18 values P_w = (x of spacecraft 1, 2, 3; unit vectors of
+// links 12, 23, 31) at table time w dt, then the 18 differences P_{w+1} - P_w
+// (zero in the last row), so that one contiguous 288-B row gives the linear
+// interpolation at any t in [w dt, (w + 1) dt).
+constexpr int ORB_VALS = 18;
+constexpr int ORB_ROW = 2 * ORB_VALS;
+
 struct OrbitTab {
     const double* x;  // (N, 3, 3)
     const double* n;  // (N, 6, 3)
+    const double* p;  // (N, ORB_ROW) packed rows
     double dt, inv_dt;
     int N;
     int li12, li23, li31;  // row of links 12, 23, 31 in n
@@ -104,57 +113,66 @@ __device__ inline double dot3(const double* a, const double* b) { return a[0] * 
 // Orbit quantities at time t that the response needs:
 //   kp[i] = k . p_i (i = spacecraft 1..3), and for links 12, 23, 31:
 //   kn[j] = k . n_j, un[j] = u . n_j, vn[j] = v . n_j
+// from the positions and link vectors linearly interpolated in time (the
+// table's lisatools linear interpolation), read as one packed row.
 __device__ inline void orbit_eval(const OrbitTab& o, const RespRec& r, double t, double kp[3], double kn[3],
                                   double un[3], double vn[3])
 {
     int w = int(floor(t * o.inv_dt));
     w = w < 0 ? 0 : (w > o.N - 2 ? o.N - 2 : w);
     const double a = (t - o.dt * w) * o.inv_dt;  // interpolation weight in [0, 1) inside the table
-    const double* x0 = o.x + (long)w * 9;
-    for (int i = 0; i < 3; i++) {
-        const double lo = dot3(r.k, x0 + 3 * i), hi = dot3(r.k, x0 + 9 + 3 * i);
-        kp[i] = fma(hi - lo, a, lo);
+    const double2* row = reinterpret_cast<const double2*>(o.p + (long)w * ORB_ROW);
+    double v[ORB_VALS];
+    for (int q = 0; q < ORB_VALS / 2; q++) {
+        const double2 lo = row[q], d = row[ORB_VALS / 2 + q];
+        v[2 * q] = fma(d.x, a, lo.x);
+        v[2 * q + 1] = fma(d.y, a, lo.y);
     }
-    const double* n0 = o.n + (long)w * 18;
-    const int li[3] = {o.li12, o.li23, o.li31};
+    for (int i = 0; i < 3; i++) kp[i] = dot3(r.k, v + 3 * i);
     for (int j = 0; j < 3; j++) {
-        const double* lo = n0 + 3 * li[j];
-        const double* hi = lo + 18;
-        double a0 = dot3(r.k, lo), a1 = dot3(r.k, hi);
-        kn[j] = fma(a1 - a0, a, a0);
-        a0 = dot3(r.u, lo);
-        a1 = dot3(r.u, hi);
-        un[j] = fma(a1 - a0, a, a0);
-        a0 = dot3(r.v, lo);
-        a1 = dot3(r.v, hi);
-        vn[j] = fma(a1 - a0, a, a0);
+        const double* nj = v + 9 + 3 * j;
+        kn[j] = dot3(r.k, nj);
+        un[j] = dot3(r.u, nj);
+        vn[j] = dot3(r.v, nj);
     }
 }
 
-// Transfer functions of TDI channels 1..3 at frequency f and time t; returns
-// the orbital Doppler phase in 'phaseRdelay' (Response.cu:54-314).
-__device__ inline void resp_eval(const RespRec& r, const OrbitTab& o, double f, double t, int TDItag, int rescaled,
-                                 int tdi2, cd out[3], double& phaseRdelay)
+// Quantities of one frequency that every (binary, mode) shares: x = pi f L / c
+// (the reference's 'prefactor') and its sine and cosine.
+struct NodeResp {
+    double x, sx, cx;
+};
+
+__device__ inline NodeResp resp_node(const OrbitTab& o, double f)
+{
+    NodeResp n;
+    n.x = PI * f * o.L / C_SI;
+    sincos_red(n.x, &n.sx, &n.cx);
+    return n;
+}
+
+// The six single-link terms of one (binary, mode) at frequency f and time t,
+// G[0..5] = G12, G21, G23, G32, G31, G13 (Response.cu:54-194), and the
+// orbital Doppler phase in 'phaseRdelay' (Response.cu:278-309); e^{i theta}
+// from the table 'tab' (cexpi_tab).
+template <class E>
+__device__ inline void resp_links(const RespRec& r, const OrbitTab& o, const NodeResp& nr, double f, double t,
+                                  cd G[6], double& phaseRdelay, const E* __restrict__ tab)
 {
     double kp[3], kn[3], un[3], vn[3];
     orbit_eval(o, r, t, kp, kn, un, vn);
-    double sa, ca;
-    sincos_red(OMEGA0 * t, &sa, &ca);
-    const double kR = r.k[0] * (AU_SI * ca) + r.k[1] * (AU_SI * sa);
+    const cd eo = cexpi_tab(OMEGA0 * t, tab);
+    const double kR = r.k[0] * (AU_SI * eo.re) + r.k[1] * (AU_SI * eo.im);
     phaseRdelay = 2. * PI / C_SI * f * kR;
 
-    const double L = o.L;
-    const double x = PI * f * L / C_SI;  // = the reference's 'prefactor'
-    double sx, cx;
-    sincos_red(x, &sx, &cx);
-
+    const double x = nr.x, sx = nr.sx, cx = nr.cx;
     // link j = 0, 1, 2 is n1 (12), n2 (23), n3 (31); n_j^T H n_j
     cd nHn[3];
     double sp[3], sm[3];  // sm: sinc(x (1 - kn)), sp: sinc(x (1 + kn))
     for (int j = 0; j < 3; j++) {
         nHn[j] = r.Yp * (un[j] * un[j] - vn[j] * vn[j]) + r.Yc * (2.0 * un[j] * vn[j]);
-        double s, c;
-        sincos_red(x * kn[j], &s, &c);
+        const cd e = cexpi_tab(x * kn[j], tab);
+        const double s = e.im, c = e.re;
         // 1/am and 1/ap from one reciprocal of their product
         const double am = x * (1. - kn[j]), ap = x * (1. + kn[j]);
         const double prod = am * ap;
@@ -168,18 +186,25 @@ __device__ inline void resp_eval(const RespRec& r, const OrbitTab& o, double f, 
         }
     }
     // exp(i x (1 + k.(p_i + p_j)/L)) for the pairs 12, 23, 31 (Response.cu:171-173)
-    const cd e12 = cexpi_red(x * (1. + (kp[0] + kp[1]) * o.inv_L));
-    const cd e23 = cexpi_red(x * (1. + (kp[1] + kp[2]) * o.inv_L));
-    const cd e31 = cexpi_red(x * (1. + (kp[2] + kp[0]) * o.inv_L));
+    const cd e12 = cexpi_tab(x * (1. + (kp[0] + kp[1]) * o.inv_L), tab);
+    const cd e23 = cexpi_tab(x * (1. + (kp[1] + kp[2]) * o.inv_L), tab);
+    const cd e31 = cexpi_tab(x * (1. + (kp[2] + kp[0]) * o.inv_L), tab);
     const cd g3 = mul_i(x * nHn[2]), g1 = mul_i(x * nHn[0]), g2 = mul_i(x * nHn[1]);
-    const cd G12 = (g3 * sm[2]) * e12;
-    const cd G21 = (g3 * sp[2]) * e12;
-    const cd G23 = (g1 * sm[0]) * e23;
-    const cd G32 = (g1 * sp[0]) * e23;
-    const cd G31 = (g2 * sm[1]) * e31;
-    const cd G13 = (g2 * sp[1]) * e31;
+    G[0] = (g3 * sm[2]) * e12;
+    G[1] = (g3 * sp[2]) * e12;
+    G[2] = (g1 * sm[0]) * e23;
+    G[3] = (g1 * sp[0]) * e23;
+    G[4] = (g2 * sm[1]) * e31;
+    G[5] = (g2 * sp[1]) * e31;
+}
 
-    // TDI combinations (Response.cu:197-261)
+// TDI channels 1..3 from link terms G[0..5] (order of resp_links) at the
+// frequency of 'nr' (Response.cu:197-261). Linear in G, so it may be applied
+// to a weighted sum of several modes' link terms at one frequency.
+__device__ inline void tdi_combine(const cd G[6], const NodeResp& nr, int TDItag, int rescaled, int tdi2, cd out[3])
+{
+    const cd G12 = G[0], G21 = G[1], G23 = G[2], G32 = G[3], G31 = G[4], G13 = G[5];
+    const double sx = nr.sx, cx = nr.cx;
     const double s2x = 2.0 * sx * cx, c2x = cx * cx - sx * sx;
     const cd z = mk(c2x, s2x);
     if (TDItag == 1) {
@@ -209,6 +234,17 @@ __device__ inline void resp_eval(const RespRec& r, const OrbitTab& o, double f, 
     out[0] = fa * A;
     out[1] = fa * E;
     out[2] = ft * T;
+}
+
+// Transfer functions of TDI channels 1..3 at frequency f and time t; returns
+// the orbital Doppler phase in 'phaseRdelay' (Response.cu:54-314).
+__device__ inline void resp_eval(const RespRec& r, const OrbitTab& o, double f, double t, int TDItag, int rescaled,
+                                 int tdi2, cd out[3], double& phaseRdelay)
+{
+    const NodeResp nr = resp_node(o, f);
+    cd G[6];
+    resp_links(r, o, nr, f, t, G, phaseRdelay, ETAB);
+    tdi_combine(G, nr, TDItag, rescaled, tdi2, out);
 }
 
 }  // namespace resp

@@ -90,18 +90,37 @@ def build_problem(batch, nodes, dense_bits, seed=1000):
     return dict(ref=ref, templates=templates, f=f, data=data, psd=psd, het=het, wave=wave, orbits=orbits)
 
 
-def cpu_baseline(prob, sample, nodes, min_seconds=10.0):
+def _omp_threads(n):
+    """Set the OpenMP thread count of the oracle's runtime (libgomp) in this process."""
+    import ctypes
+
+    try:
+        ctypes.CDLL("libgomp.so.1").omp_set_num_threads(int(n))
+        return True
+    except OSError:
+        return False
+
+
+def cpu_baseline(prob, sample, nodes, min_seconds=10.0, one_thread_seconds=4.0):
     """The FP64 CPU oracle (OpenMP over templates on the host cores) on a bounded
     sample: the first `sample` templates of the batch, repeated for at least
-    `min_seconds` of CPU time."""
+    `min_seconds` of CPU time; then the same on one thread (the reference's own
+    CPU build is single-threaded) for at least `one_thread_seconds`; then the
+    oracle built with FMA contraction on the same sample (the FP64 rounding
+    spread of the algorithm itself)."""
+    import oracle
     from bbhx_amd.utils.sensitivity import AET1SensitivityMatrix
     from oracle.pipeline import OracleHeterodyne, OracleWaveform
     from tests.params import MODES
 
-    owave = OracleWaveform(prob["orbits"].as_host_dict(), tdi2=True)
     psd_fn = lambda x: [AET1SensitivityMatrix(x)[i] for i in range(3)]  # noqa: E731
-    ohet = OracleHeterodyne(owave, prob["f"], prob["data"], prob["ref"], nodes, psd_fn, reference_length=8096,
-                            modes=MODES)
+
+    def make():
+        owave = OracleWaveform(prob["orbits"].as_host_dict(), tdi2=True)
+        return OracleHeterodyne(owave, prob["f"], prob["data"], prob["ref"], nodes, psd_fn, reference_length=8096,
+                                modes=MODES)
+
+    ohet = make()
     p = prob["templates"][:, :sample]
     ohet.get_ll(p[:, :8])  # warm-up
     t0 = time.perf_counter()
@@ -114,7 +133,22 @@ def cpu_baseline(prob, sample, nodes, min_seconds=10.0):
             break
     dt = (time.perf_counter() - t0) / reps
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return dict(value=sample / dt, ll=ll, cores=cores, seconds_per_batch=dt, reps=reps)
+    one = None
+    if _omp_threads(1):
+        n1 = min(256, sample)
+        t0 = time.perf_counter()
+        r1 = 0
+        while True:
+            ohet.get_ll(p[:, :n1])
+            r1 += 1
+            if time.perf_counter() - t0 > one_thread_seconds:
+                break
+        one = n1 * r1 / (time.perf_counter() - t0)
+        _omp_threads(cores)
+    with oracle.fma_rounding():
+        ll_fma = make().get_ll(p)[0]
+    return dict(value=sample / dt, ll=ll, ll_fma=ll_fma, cores=cores, seconds_per_batch=dt, reps=reps,
+                value_1thread=one)
 
 
 def ragged_lengths(p, t_data):
@@ -354,7 +388,7 @@ def bench_direct_ragged(args):
         tfile = ROOT / "profiles" / "k_dir_like.direct_ragged_1024.traffic.json"
         if tfile.exists() and B == 1024 and not args.uniform_length and args.direct_years == 1.0:
             traffic = json.loads(tfile.read_text())["hbm_bytes_per_launch"]
-        roofline = {"bound": "mfma", "compute": "FP64 VALU (no dense contraction, no MFMA; peak = FP64 vector peak)",
+        roofline = {"bound": "fp64-valu", "compute": "FP64 VALU (no dense contraction, no MFMA; peak = FP64 vector peak)",
                     "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic, "kernel": "k_dir_like",
                     "kernel_ms": split["dense_ms"], "flops_per_launch": flops,
@@ -547,22 +581,44 @@ def main():
     tfile = ROOT / "profiles" / "k_fused_eval.hdyn_4096x128.traffic.json"
     if tfile.exists() and B == 4096 and args.nodes == 128:
         traffic = json.loads(tfile.read_text())["hbm_bytes_per_launch"]
-    roofline = {"bound": "mfma", "compute": "FP64 VALU (no dense contraction, no MFMA; peak = FP64 vector peak)",
+    roofline = {"bound": "fp64-valu", "compute": "FP64 VALU (no dense contraction, no MFMA; peak = FP64 vector peak)",
                 "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": (achieved / FP64_PEAK_TFLOPS) if achieved else None, "traffic": traffic,
                 "kernel": "k_fused_eval", "kernel_ms": ms_eval / max(ncalls, 1), "flops_per_launch": flops,
                 "flop_rule": flop_model["counting_rules"]}
 
+    # end to end: host parameters in, host logL out (parameter rows built and
+    # uploaded, the fused launches, the logL copied back), as get_ll(params) runs
+    het.get_ll(prob["templates"])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    n_e2e = max(3, args.steps // 4)
+    for _ in range(n_e2e):
+        het.get_ll(prob["templates"])
+    e2e_ms = (time.perf_counter() - t0) * 1e3 / n_e2e
+    log(f"end to end get_ll(params) (host in, host out): {e2e_ms:.3f} ms per call of {B} templates")
+
     cpu = None
-    dll = None
+    acc = None
     if not args.no_cpu:
         log("timing the CPU oracle")
         c = cpu_baseline(prob, min(args.cpu_sample, B), args.nodes, args.cpu_seconds)
-        dll = float(np.max(np.abs(ll_fused[:len(c["ll"])] - c["ll"])))
+        n = len(c["ll"])
+        d = np.abs(ll_fused[:n] - c["ll"])
+        floor = np.abs(c["ll_fma"] - c["ll"])
+        acc = {"max_abs_dlogL_vs_cpu_oracle": float(d.max()),
+               "max_rel_dlogL_vs_cpu_oracle": float(np.max(d / np.abs(c["ll"]))),
+               "templates_within_1e-6": int(np.sum(d <= 1e-6)), "templates_checked": n,
+               "oracle_fma_rounding_spread": float(floor.max()),
+               "note": "oracle_fma_rounding_spread: max |dlogL| between the FP64 oracle and the same sources "
+                       "compiled with FMA contraction (FP64 rounding alone)"}
         cpu = {"value": c["value"], "unit": "logL evals/s", "cores": c["cores"], "kind": "port",
-               "sample": f"FP64 CPU oracle (OpenMP over templates), first {len(c['ll'])} templates of the batch, "
-                         f"{c['reps']} passes of {c['seconds_per_batch']:.3f} s"}
-        log(f"cpu oracle: {c['value']:.4g} evals/s on {c['cores']} threads; max |dlogL| vs GPU = {dll:.3e}")
+               "sample": f"FP64 CPU oracle (OpenMP over templates), first {n} templates of the batch, "
+                         f"{c['reps']} passes of {c['seconds_per_batch']:.3f} s",
+               "value_1thread": c["value_1thread"]}
+        log(f"cpu oracle: {c['value']:.4g} evals/s on {c['cores']} threads, {c['value_1thread']} on 1 thread; "
+            f"max |dlogL| vs GPU = {d.max():.3e} ({int(np.sum(d <= 1e-6))}/{n} within 1e-6); oracle FMA "
+            f"rounding spread {floor.max():.3e}")
 
     out = {
         "metric": "heterodyned logL evals/sec + grad evals/sec at 1 GPU, |dlogL|<1e-3 vs FP64",
@@ -580,7 +636,8 @@ def main():
         "config": {"workload": "heterodyned logL, fused FP64, hipGraph replay", "templates": B,
                    "sparse_nodes": args.nodes, "eager_ms_per_step": ms_eager, "graph_vs_eager": graph_diff,
                    "dense_bins": 2**args.dense_bits, "modes": 6, "tdi": "AET tdi2",
-                   "plain_fp64_path_evals_per_s": plain_rate, "max_abs_dlogL_vs_cpu_oracle": dll,
+                   "plain_fp64_path_evals_per_s": plain_rate, "end_to_end_get_ll_ms": e2e_ms,
+                   "end_to_end_evals_per_s": B / (e2e_ms * 1e-3), "accuracy_vs_cpu_oracle": acc,
                    "grad_evals_per_s": grad_rate, "grad_cost_vs_forward": value / grad_rate,
                    "grad_params": "lnM, q, chi1, chi2, dist, phi_ref, cos inc, lam, sin beta, psi, t_ref",
                    "grad_1024x256": grad_1024},

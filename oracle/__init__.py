@@ -20,6 +20,7 @@ reference's flat layouts require.
 
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 from pathlib import Path
 
@@ -28,6 +29,7 @@ import numpy as np
 _HERE = Path(__file__).resolve().parent
 _LIB_PATH = _HERE / "liboracle.so"
 _REF_PATH = _HERE / "_ref" / "libbbhx_ref.so"
+_FMA_PATH = _HERE / "liboracle_fma.so"
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -44,6 +46,24 @@ def _load():
             raise RuntimeError(f"{_LIB_PATH} is not built; run `make -C oracle` (or __graft_entry__.build())")
         _lib = C.CDLL(str(_LIB_PATH))
     return _lib
+
+
+@contextlib.contextmanager
+def fma_rounding():
+    """Inside the block the oracle runs from ``liboracle_fma.so``: the same
+    sources compiled with every multiply-add contracted into an FMA. The two
+    builds compute the same algorithm with different FP64 rounding, so their
+    difference is the spread that rounding alone gives a result (the floor
+    below which two FP64 implementations cannot be expected to agree)."""
+    global _lib
+    saved = _load()
+    if not _FMA_PATH.exists():
+        raise RuntimeError(f"{_FMA_PATH} is not built; run `make -C oracle`")
+    _lib = C.CDLL(str(_FMA_PATH))
+    try:
+        yield
+    finally:
+        _lib = saved
 
 
 def ref_available() -> bool:

@@ -24,6 +24,8 @@ against the oracle run template by template: |delta logL| <= 1e-6 + 1e-12
 |logL| (a 4-knot grid interpolates badly, |logL| reaches 5e7).
 """
 
+import contextlib
+
 import numpy as np
 import pytest
 
@@ -50,12 +52,52 @@ def _heterodyned_problem():
     data = inj + np.sqrt(psd / (4 * df)) * (rng.standard_normal(psd.shape) + 1j * rng.standard_normal(psd.shape))
     het = HeterodynedLikelihood(wave, f, data, ref, 128, sens_mat=sens, template_gen_kwargs=dict(modes=MODES),
                                 reference_gen_kwargs=dict(length=8096, modes=MODES))
+    het._problem = dict(f=f, data=data, orbits=orbits)
     return het, ref
 
 
 @pytest.fixture(scope="module")
 def het_full():
     return _heterodyned_problem()
+
+
+def test_heterodyned_all_templates_vs_oracle_full_size(het_full):
+    """The heterodyned benchmark itself: 4096 templates around the reference,
+    K = 128, 2^20 dense bins, fused and kernel-by-kernel HIP logL against the
+    FP64 CPU oracle on every template.
+
+    Bound: |dlogL| <= 1e-6 + 1e-12 |logL|. At this shape |logL| is 3.2e6-3.6e6
+    and the PhenomHM phases reach 1e4-1e6 rad, so FP64 rounding alone moves
+    logL by about 1e-6: the oracle built with FMA contraction (the same sources,
+    a different rounding) differs from itself by up to ~1.5e-6. The test also
+    holds the HIP paths to that floor (at most 3x the oracle's own FMA spread)
+    and asks for 99 % of templates inside the plain 1e-6."""
+    import oracle
+    from bbhx_amd.utils.sensitivity import AET1SensitivityMatrix
+    from oracle.pipeline import OracleHeterodyne, OracleWaveform
+
+    het, ref = het_full
+    pb = het._problem
+    p = perturb(ref, 4096, 1001, scale=1e-3)
+    ll_fused = het.get_ll(p, fused=True)
+    ll_plain = het.get_ll(p, fused=False)
+    psd_fn = lambda x: [AET1SensitivityMatrix(x)[i] for i in range(3)]  # noqa: E731
+    lls = []
+    for variant in (False, True):
+        with oracle.fma_rounding() if variant else contextlib.nullcontext():
+            owave = OracleWaveform(pb["orbits"].as_host_dict(), tdi2=True)
+            ohet = OracleHeterodyne(owave, pb["f"], pb["data"], ref, 128, psd_fn, reference_length=8096,
+                                    modes=MODES)
+            lls.append(ohet.get_ll(p)[0])
+    ll_o, ll_fma = lls
+    floor = float(np.max(np.abs(ll_fma - ll_o)))
+    for name, ll in (("fused", ll_fused), ("kernel-by-kernel", ll_plain)):
+        d = np.abs(ll - ll_o)
+        print(f"{name}: max |dlogL| {d.max():.3e}, {np.sum(d > 1e-6)} of {d.size} above 1e-6; oracle FMA spread "
+              f"{floor:.3e}; |logL| in [{np.abs(ll_o).min():.3e}, {np.abs(ll_o).max():.3e}]", flush=True)
+        assert np.all(d <= 1e-6 + 1e-12 * np.abs(ll_o)), name
+        assert d.max() <= 3.0 * max(floor, 1e-7), name
+        assert np.mean(d <= 1e-6) >= 0.99, name
 
 
 def test_heterodyned_reference_identity_full_size(het_full):
