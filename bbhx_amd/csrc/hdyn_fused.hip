@@ -156,35 +156,25 @@ extern "C" int bbhx_hip_timing_read(double* ms_setup, double* ms_eval, int* ncal
     return OK;
 }
 
-extern "C" int bbhx_hip_hdyn_fused(double* d_h, double* h_h, const double* params, const double* t_start,
-                                   const double* t_end, int numBinAll, const int* ells_host, const int* mms_host,
-                                   int numModes, const double* freqs, const double* h0, const double* dataConstants,
-                                   int K, const double* qnm, int table_len, double dspin, double initial_t_val,
-                                   const bbhx_response* orbits, int TDItag, int rescaled, int tdi2,
-                                   int include_last_segment, int precision, double* logL, double d_d,
-                                   void* stream)
+namespace {
+
+// The record set-up and evaluation launches of the fused forward, on records
+// 'recs' (numBinAll * numModes of them) that the caller provides; arguments
+// as bbhx_hip_hdyn_fused, already checked. 'timing' allows the optional event
+// timing (which may create events).
+int hdyn_fused_launch(FusedRec* recs, double* d_h, double* h_h, const double* params, const double* t_start,
+                      const double* t_end, int numBinAll, const ModeList& ml, int numModes, const double* freqs,
+                      const double* h0, const double* dataConstants, int K, const double* qnm, int table_len,
+                      double dspin, double initial_t_val, const resp::OrbitTab& orb, int TDItag, int rescaled,
+                      int tdi2, int include_last_segment, double* logL, double d_d, hipStream_t st, bool timing)
 {
-    resp::OrbitTab orb;
-    if (!response_orbit_tab(orbits, orb)) return ERR_ARGS;
-    if (numBinAll < 0 || numModes < 1 || numModes > MAX_MODES || K < 2 || K > 1024 || table_len < 2) return ERR_ARGS;
-    if (precision != 0) return ERR_ARGS;
-    if (numBinAll == 0) return OK;
-    if (!d_h || !h_h || !params || !t_start || !t_end || !ells_host || !mms_host || !freqs || !h0 ||
-        !dataConstants || !qnm)
-        return ERR_ARGS;
-    ModeList ml;
-    const int mst = check_modes(ells_host, mms_host, numModes, ml);
-    if (mst != OK) return mst;
-    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     const int nrec = numBinAll * numModes;
-    FusedRec* recs = static_cast<FusedRec*>(workspace_get(WS_FUSED, sizeof(FusedRec) * (size_t)nrec));
-    if (!recs) return ERR_ALLOC;
     hipEvent_t ev[3];
-    const bool timed = timer_take3(ev);
-    if (g_timer.on && !timed) return ERR_LAUNCH;
+    const bool timed = timing && timer_take3(ev);
+    if (timing && g_timer.on && !timed) return ERR_LAUNCH;
     if (timed) (void)hipEventRecord(ev[0], st);
-    hipLaunchKernelGGL(k_fused_setup, dim3(SETUP_PARTS * ceil_div(nrec, 64)), dim3(64), 0, st, recs, params, numBinAll, ml,
-                       numModes, qnm, table_len, dspin);
+    hipLaunchKernelGGL(k_fused_setup, dim3(SETUP_PARTS * ceil_div(nrec, 64)), dim3(64), 0, st, recs, params,
+                       numBinAll, ml, numModes, qnm, table_len, dspin);
     if (hipGetLastError() != hipSuccess) return ERR_LAUNCH;
     if (timed) (void)hipEventRecord(ev[1], st);
     const int jend = include_last_segment ? K : K - 1;
@@ -196,6 +186,47 @@ extern "C" int bbhx_hip_hdyn_fused(double* d_h, double* h_h, const double* param
                        initial_t_val);
     if (timed) (void)hipEventRecord(ev[2], st);
     return hipGetLastError() == hipSuccess ? OK : ERR_LAUNCH;
+}
+
+// Argument checks shared by the fused forward entry points. Returns OK (and
+// fills orb, ml), a status, or -1 for an empty batch (nothing to do).
+int hdyn_fused_check(double* d_h, double* h_h, const double* params, const double* t_start, const double* t_end,
+                     int numBinAll, const int* ells_host, const int* mms_host, int numModes, const double* freqs,
+                     const double* h0, const double* dataConstants, int K, const double* qnm, int table_len,
+                     const bbhx_response* orbits, int precision, resp::OrbitTab& orb, ModeList& ml)
+{
+    if (!response_orbit_tab(orbits, orb)) return ERR_ARGS;
+    if (numBinAll < 0 || numModes < 1 || numModes > MAX_MODES || K < 2 || K > 1024 || table_len < 2) return ERR_ARGS;
+    if (precision != 0) return ERR_ARGS;
+    if (numBinAll == 0) return -1;
+    if (!d_h || !h_h || !params || !t_start || !t_end || !ells_host || !mms_host || !freqs || !h0 ||
+        !dataConstants || !qnm)
+        return ERR_ARGS;
+    return check_modes(ells_host, mms_host, numModes, ml);
+}
+
+}  // namespace
+
+extern "C" int bbhx_hip_hdyn_fused(double* d_h, double* h_h, const double* params, const double* t_start,
+                                   const double* t_end, int numBinAll, const int* ells_host, const int* mms_host,
+                                   int numModes, const double* freqs, const double* h0, const double* dataConstants,
+                                   int K, const double* qnm, int table_len, double dspin, double initial_t_val,
+                                   const bbhx_response* orbits, int TDItag, int rescaled, int tdi2,
+                                   int include_last_segment, int precision, double* logL, double d_d,
+                                   void* stream)
+{
+    resp::OrbitTab orb;
+    ModeList ml;
+    const int chk = hdyn_fused_check(d_h, h_h, params, t_start, t_end, numBinAll, ells_host, mms_host, numModes,
+                                     freqs, h0, dataConstants, K, qnm, table_len, orbits, precision, orb, ml);
+    if (chk != OK) return chk < 0 ? OK : chk;
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const int nrec = numBinAll * numModes;
+    FusedRec* recs = static_cast<FusedRec*>(workspace_get(WS_FUSED, sizeof(FusedRec) * (size_t)nrec, st));
+    if (!recs) return ERR_ALLOC;
+    return hdyn_fused_launch(recs, d_h, h_h, params, t_start, t_end, numBinAll, ml, numModes, freqs, h0,
+                             dataConstants, K, qnm, table_len, dspin, initial_t_val, orb, TDItag, rescaled, tdi2,
+                             include_last_segment, logL, d_d, st, true);
 }
 
 extern "C" int bbhx_hip_release_workspace(void)

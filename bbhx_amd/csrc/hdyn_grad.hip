@@ -245,12 +245,12 @@ extern "C" int bbhx_hip_hdyn_fused_grad(double* d_h, double* h_h, double* grad, 
         return ERR_ARGS;
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     const int nrec = numBinAll * numModes;
-    FusedRec* recs = static_cast<FusedRec*>(workspace_get(WS_FUSED, sizeof(FusedRec) * (size_t)nrec));
-    GradRec* grecs = static_cast<GradRec*>(workspace_get(WS_GRAD, sizeof(GradRec) * (size_t)nrec));
+    FusedRec* recs = static_cast<FusedRec*>(workspace_get(WS_FUSED, sizeof(FusedRec) * (size_t)nrec, st));
+    GradRec* grecs = static_cast<GradRec*>(workspace_get(WS_GRAD, sizeof(GradRec) * (size_t)nrec, st));
     // adjoint weights (B, 3, K) complex, then partial gradients (B, M, 11)
     const size_t gw_bytes = sizeof(cd) * 3 * (size_t)K * numBinAll;
     char* scratch = static_cast<char*>(
-        workspace_get(WS_GRAD_AUX, gw_bytes + sizeof(double) * NGRAD * (size_t)nrec));
+        workspace_get(WS_GRAD_AUX, gw_bytes + sizeof(double) * NGRAD * (size_t)nrec, st));
     if (!recs || !grecs || !scratch) return ERR_ALLOC;
     cd* gw = reinterpret_cast<cd*>(scratch);
     double* partial = reinterpret_cast<double*>(scratch + gw_bytes);

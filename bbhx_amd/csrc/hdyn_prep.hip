@@ -198,12 +198,12 @@ extern "C" int bbhx_hip_hdyn_prep(double* dataConstants, double* totals, const d
     const long max_items = (long)(data_length + PREP_CHUNK - 1) / PREP_CHUNK + K;
     const size_t ints = 2 * ((size_t)K + 1);
     const size_t bytes = sizeof(double) * NP * (size_t)max_items + sizeof(int) * ints + 256;
-    char* ws = static_cast<char*>(workspace_get(WS_PREP, bytes));
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    char* ws = static_cast<char*>(workspace_get(WS_PREP, bytes, st));
     if (!ws) return ERR_ALLOC;
     double* partial = reinterpret_cast<double*>(ws);
     int* seg = reinterpret_cast<int*>(ws + sizeof(double) * NP * (size_t)max_items);
     int* item = seg + (K + 1);
-    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     hipLaunchKernelGGL(k_prep_bounds, dim3(1), dim3(1024), 0, st, seg, item, dataFreqs, data_length, nodes, K);
     // persistent: enough workgroups to fill the chip, each looping over items
     hipLaunchKernelGGL(k_prep_items, dim3(1024), dim3(256), 0, st, partial, seg, item, dataFreqs,

@@ -85,7 +85,7 @@ namespace bbhx {
 // Synchronises the stream so the host vector may be released on return.
 InterpBin* upload_table(const std::vector<InterpBin>& h, hipStream_t stream)
 {
-    InterpBin* d = static_cast<InterpBin*>(workspace_get(WS_TABLE, sizeof(InterpBin) * (h.size() + 1)));
+    InterpBin* d = static_cast<InterpBin*>(workspace_get(WS_TABLE, sizeof(InterpBin) * (h.size() + 1), stream));
     if (!d) return nullptr;
     if (hipMemcpyAsync(d, h.data(), sizeof(InterpBin) * h.size(), hipMemcpyHostToDevice, stream) != hipSuccess)
         return nullptr;
@@ -143,7 +143,7 @@ extern "C" int bbhx_hip_InterpTDI(const int64_t* templateChannels_ptrs, const do
     const long per_bin = (long)numModes * length;  // segment records per binary
     long chunk = (long)(INTERP_REC_CAP / (sizeof(InterpSeg) * (size_t)per_bin));
     chunk = chunk < 1 ? 1 : (chunk > numBinAll ? numBinAll : chunk);
-    InterpSeg* R = static_cast<InterpSeg*>(workspace_get(WS_INTERP, sizeof(InterpSeg) * (size_t)(per_bin * chunk)));
+    InterpSeg* R = static_cast<InterpSeg*>(workspace_get(WS_INTERP, sizeof(InterpSeg) * (size_t)(per_bin * chunk), st));
     if (!R) return ERR_ALLOC;
     for (long b0 = 0; b0 < numBinAll; b0 += chunk) {
         const long b1 = b0 + chunk < numBinAll ? b0 + chunk : numBinAll;

@@ -230,7 +230,7 @@ int bbhx_hip_hdyn(double* likeOut1, double* likeOut2, const double* templateChan
     if (nchunk > 64) nchunk = 64;
     int chunk = ceil_div(jend - 1 > 0 ? jend - 1 : 1, nchunk);
     nchunk = ceil_div(jend - 1 > 0 ? jend - 1 : 1, chunk);
-    cd* partial = static_cast<cd*>(workspace_get(WS_PARTIAL, sizeof(cd) * 2 * (size_t)numBinAll * nchunk));
+    cd* partial = static_cast<cd*>(workspace_get(WS_PARTIAL, sizeof(cd) * 2 * (size_t)numBinAll * nchunk, S(stream)));
     if (!partial) return ERR_ALLOC;
     hipLaunchKernelGGL(k_hdyn, dim3(waves, nchunk), dim3(64), 0, S(stream), partial,
                        reinterpret_cast<const cd*>(templateChannels), reinterpret_cast<const cd*>(dataConstants),
@@ -261,7 +261,7 @@ int bbhx_hip_direct_like(double* d_h, double* h_h, const double* dataChannels, c
     if (tiles == 0) return OK;
     InterpBin* bins = upload_table(h, st);
     if (!bins) return ERR_ALLOC;
-    double* partial = static_cast<double*>(workspace_get(WS_PARTIAL, sizeof(double) * 3 * (size_t)tiles));
+    double* partial = static_cast<double*>(workspace_get(WS_PARTIAL, sizeof(double) * 3 * (size_t)tiles, st));
     if (!partial) return ERR_ALLOC;
     hipLaunchKernelGGL(k_direct_like, dim3((unsigned)tiles), dim3(DL_THREADS), 0, st, partial, bins, numBinAll,
                        reinterpret_cast<const cd*>(dataChannels), noise_weight_times_df, data_stream_length);

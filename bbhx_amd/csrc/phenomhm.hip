@@ -154,7 +154,7 @@ int bbhx_hip_waveform_amp_phase(double* waveformOut, const int* ells, const int*
         return ERR_ARGS;
     if ((long)numBinAll * numModes > 65535L * 65535L) return ERR_ARGS;
     const int nrec = numBinAll * numModes;
-    PhmRec* recs = static_cast<PhmRec*>(workspace_get(WS_PHM_REC, sizeof(PhmRec) * (size_t)nrec));
+    PhmRec* recs = static_cast<PhmRec*>(workspace_get(WS_PHM_REC, sizeof(PhmRec) * (size_t)nrec, S(stream)));
     if (!recs) return ERR_ALLOC;
     hipLaunchKernelGGL(k_phm_setup, dim3(ceil_div(nrec, 64)), dim3(64), 0, S(stream), recs, ells, mms, m1_SI, m2_SI,
                        chi1z, chi2z, distance, numModes, numBinAll, Mf_RD_lm_all, Mf_DM_lm_all, run_phenomd);

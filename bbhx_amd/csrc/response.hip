@@ -190,7 +190,7 @@ int bbhx_hip_response_compute(bbhx_response* r, double* response_out, const int*
     const double* tf = response_out + (includesAmps + 1) * nper;
     double* resp = response_out + (includesAmps + 2) * nper;
     const int nrec = numBinAll * numModes;
-    RespRec* recs = static_cast<RespRec*>(workspace_get(WS_RESP_REC, sizeof(RespRec) * (size_t)nrec));
+    RespRec* recs = static_cast<RespRec*>(workspace_get(WS_RESP_REC, sizeof(RespRec) * (size_t)nrec, S(stream)));
     if (!recs) return ERR_ALLOC;
     hipLaunchKernelGGL(k_resp_setup, dim3(ceil_div(nrec, 64)), dim3(64), 0, S(stream), recs, ells, mms, phi_ref, inc,
                        lam, beta, psi, numModes, numBinAll);

@@ -1,9 +1,17 @@
-// Per-device, grow-only scratch buffers owned by the library.
+// Scratch buffers owned by the library, one set per (device, stream).
 //
 // Kernels that need per-(binary, mode) coefficient records or per-binary
-// descriptor tables take them from here. A buffer only grows; growing waits
-// for the device to go idle before releasing the old one, so in steady state
-// (same or smaller problem sizes) no allocation happens on any call.
+// descriptor tables take them from here. Work queued on different streams
+// gets different buffers, so two callers driving the library from two streams
+// never share scratch memory. A buffer only grows; a buffer that is replaced by
+// a larger one is retired, not freed, until workspace_release_all(), so kernels
+// still queued on the old buffer, and launch sequences captured in a graph
+// before the growth, keep valid memory (no device synchronisation and no free
+// on the growth path, so growing inside a graph capture is also safe). In
+// steady state (same or smaller problem sizes) no allocation happens on any call.
+// Calls that overlap on one stream cannot overlap in time (a stream runs its
+// work in order); replays of one graph that run concurrently with each other
+// would share its buffers and must not be issued.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -27,10 +35,15 @@ enum WsSlot : int {
 };
 
 // Returns a device pointer with at least 'bytes' bytes on the current device,
-// or nullptr on allocation failure.
-void* workspace_get(int slot, size_t bytes);
+// private to work queued on 'stream', or nullptr on allocation failure.
+void* workspace_get(int slot, size_t bytes, hipStream_t stream);
 
-// Releases every buffer of the current device (after a device synchronize).
+// Frees every buffer of the current device, retired ones included (after a
+// device synchronize). Graphs captured before the call must not be replayed
+// after it.
 void workspace_release_all();
+
+// Bytes held on the current device (live and retired buffers).
+size_t workspace_bytes();
 
 }  // namespace bbhx

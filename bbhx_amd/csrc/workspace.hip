@@ -1,42 +1,46 @@
 #include "workspace.h"
 
+#include <map>
 #include <mutex>
+#include <utility>
+#include <vector>
 
 namespace bbhx {
 
 namespace {
-constexpr int MAX_DEVICES = 64;
 struct Buf {
     void* ptr = nullptr;
     size_t bytes = 0;
 };
-Buf g_bufs[MAX_DEVICES][WS_SLOTS];
+struct StreamSet {
+    Buf slot[WS_SLOTS];
+};
+struct DeviceWs {
+    std::map<hipStream_t, StreamSet> sets;
+    std::vector<void*> retired;  // replaced buffers, freed by workspace_release_all
+};
+std::map<int, DeviceWs> g_ws;
 std::mutex g_mutex;
 }  // namespace
 
-void* workspace_get(int slot, size_t bytes)
+void* workspace_get(int slot, size_t bytes, hipStream_t stream)
 {
     if (slot < 0 || slot >= WS_SLOTS) return nullptr;
     int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEVICES) return nullptr;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0) return nullptr;
     std::lock_guard<std::mutex> lock(g_mutex);
-    Buf& b = g_bufs[dev][slot];
+    DeviceWs& d = g_ws[dev];
+    Buf& b = d.sets[stream].slot[slot];
     if (b.bytes >= bytes && b.ptr) return b.ptr;
     // grow by at least 1.5x to avoid repeated reallocation on slowly growing sizes
     size_t want = bytes < 4096 ? 4096 : bytes;
     if (b.bytes && want < b.bytes + b.bytes / 2) want = b.bytes + b.bytes / 2;
-    if (b.ptr) {
-        // the old buffer may still be read by queued kernels
-        if (hipDeviceSynchronize() != hipSuccess) return nullptr;
-        (void)hipFree(b.ptr);
-        b.ptr = nullptr;
-        b.bytes = 0;
-    }
     void* p = nullptr;
     if (hipMalloc(&p, want) != hipSuccess) {
         (void)hipGetLastError();
         return nullptr;
     }
+    if (b.ptr) d.retired.push_back(b.ptr);
     b.ptr = p;
     b.bytes = want;
     return p;
@@ -45,13 +49,29 @@ void* workspace_get(int slot, size_t bytes)
 void workspace_release_all()
 {
     int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= MAX_DEVICES) return;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0) return;
     std::lock_guard<std::mutex> lock(g_mutex);
+    auto it = g_ws.find(dev);
+    if (it == g_ws.end()) return;
     (void)hipDeviceSynchronize();
-    for (int s = 0; s < WS_SLOTS; s++) {
-        if (g_bufs[dev][s].ptr) (void)hipFree(g_bufs[dev][s].ptr);
-        g_bufs[dev][s] = Buf{};
-    }
+    for (auto& kv : it->second.sets)
+        for (Buf& b : kv.second.slot)
+            if (b.ptr) (void)hipFree(b.ptr);
+    for (void* p : it->second.retired) (void)hipFree(p);
+    g_ws.erase(it);
+}
+
+size_t workspace_bytes()
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0) return 0;
+    std::lock_guard<std::mutex> lock(g_mutex);
+    auto it = g_ws.find(dev);
+    if (it == g_ws.end()) return 0;
+    size_t n = 0;
+    for (auto& kv : it->second.sets)
+        for (Buf& b : kv.second.slot) n += b.bytes;
+    return n;
 }
 
 }  // namespace bbhx

@@ -312,14 +312,38 @@ def direct_like_wrap(d_h, h_h, dataChannels, noise_weight_times_df, templateChan
     _check(st, "direct_like")
 
 
+def _need(t, n, name):
+    if int(t.numel()) != int(n):
+        raise ValueError(f"{name} must hold {int(n)} elements, got {int(t.numel())}")
+
+
+def _check_fused_sizes(params, B, d_h, h_h, t_start, t_end, logL, qnm):
+    """Host-side size checks shared by the fused entry points (before any device call)."""
+    if params.dim() != 2 or int(params.shape[0]) != 11:
+        raise ValueError(f"params must be (11, B), got {tuple(params.shape)}")
+    for t, name in ((d_h, "d_h"), (h_h, "h_h"), (t_start, "t_start"), (t_end, "t_end")):
+        _need(t, B, name)
+    if logL is not None:
+        _need(logL, B, "logL")
+    if int(qnm.numel()) < 16 or int(qnm.numel()) % 8:
+        raise ValueError("qnm must hold the 8 rows of the QNM table")
+
+
+def _check_hdyn_sizes(params, d_h, h_h, t_start, t_end, freqs, h0, data_constants, qnm, logL):
+    B, K = int(params.shape[1]) if params.dim() == 2 else -1, int(freqs.numel())
+    _check_fused_sizes(params, B, d_h, h_h, t_start, t_end, logL, qnm)
+    _need(h0, 3 * K, "h0 (3, K)")
+    _need(data_constants, 12 * K, "data_constants (4, 3, K)")
+    return B, K
+
+
 def hdyn_fused(d_h, h_h, params, t_start, t_end, ells, mms, freqs, h0, data_constants, qnm, dspin, initial_t_val,
                response, TDItag, rescaled, tdi2, include_last_segment=True, precision=0, logL=None, d_d=0.0):
     """Fused heterodyned likelihood (no reference counterpart; see include/bbhx_hip.h).
     ``logL`` (B,) float64 device, optional: also written, -1/2 (d_d + Re h_h - 2 Re d_h)."""
     torch = _torch()
     f64, c128 = torch.float64, torch.complex128
-    B = int(params.shape[1])
-    K = int(freqs.numel())
+    B, K = _check_hdyn_sizes(params, d_h, h_h, t_start, t_end, freqs, h0, data_constants, qnm, logL)
     el, elp = _host_i32(ells)
     mm, mmp = _host_i32(mms)
     table_len = int(qnm.numel()) // 8
@@ -338,8 +362,7 @@ def hdyn_fused_grad(d_h, h_h, grad, params, t_start, t_end, ells, mms, freqs, h0
     ``logL`` (B,) float64 device, optional: also written, as by ``hdyn_fused``."""
     torch = _torch()
     f64, c128 = torch.float64, torch.complex128
-    B = int(params.shape[1])
-    K = int(freqs.numel())
+    B, K = _check_hdyn_sizes(params, d_h, h_h, t_start, t_end, freqs, h0, data_constants, qnm, logL)
     el, elp = _host_i32(ells)
     mm, mmp = _host_i32(mms)
     table_len = int(qnm.numel()) // 8
@@ -431,6 +454,12 @@ def direct_fused(plan, d_h, h_h, flags, params, t_start, t_end, ells, mms, data_
     B = plan.num_bin_all
     if tuple(params.shape) != (11, B):
         raise ValueError("params must be (11, B) for this plan")
+    _check_fused_sizes(params, B, d_h, h_h, t_start, t_end, None, qnm)
+    _need(flags, B, "flags")
+    N = plan.data_length
+    _need(data_freqs, N, "data_freqs")
+    _need(data_channels, 3 * N, "data_channels (3, N)")
+    _need(noise_factors, 3 * N, "noise_factors (3, N)")
     st = load_library().bbhx_hip_direct_fused(
         plan.handle, _dev(d_h, c128), _dev(h_h, c128), _dev(flags, torch.int32, "flags"), _dev(params, f64, "params"),
         _dev(t_start, f64), _dev(t_end, f64), elp, mmp, int(el.size), _dev(data_freqs, f64),
@@ -450,6 +479,12 @@ def direct_fused_grad(plan, d_h, h_h, grad, flags, params, t_start, t_end, ells,
     B = plan.num_bin_all
     if tuple(params.shape) != (11, B):
         raise ValueError("params must be (11, B) for this plan")
+    _check_fused_sizes(params, B, d_h, h_h, t_start, t_end, None, qnm)
+    _need(flags, B, "flags")
+    N = plan.data_length
+    _need(data_freqs, N, "data_freqs")
+    _need(data_channels, 3 * N, "data_channels (3, N)")
+    _need(noise_factors, 3 * N, "noise_factors (3, N)")
     st = load_library().bbhx_hip_direct_fused_grad(
         plan.handle, _dev(d_h, c128), _dev(h_h, c128), _dev(grad, f64, "grad"), _dev(flags, torch.int32, "flags"),
         _dev(params, f64, "params"),

@@ -276,6 +276,8 @@ class HeterodynedLikelihood(BBHxParallelModule):
         freqs_keep = freqs[keep]
         freqs = np.logspace(np.log10(freqs_keep[0]), np.log10(freqs_keep[-1]), self.length_f_het)
         self.h0_sparse = self.template_gen(*reference_template_params, freqs=freqs, **template_gen_kwargs)[None, :, :]
+        # the fused kernels read h0 as (3, K) contiguous; rebuilt whenever the reference changes
+        self._h0_flat = self.h0_sparse.reshape(3, -1).contiguous()
 
         f_dense_h = self.f_dense.cpu().numpy()
         inds = (f_dense_h >= freqs[0]) & (f_dense_h <= freqs[-1])
@@ -370,8 +372,6 @@ class HeterodynedLikelihood(BBHxParallelModule):
         d_h = torch.empty(B, dtype=torch.complex128, device=self.device)
         h_h = torch.empty(B, dtype=torch.complex128, device=self.device)
         resp = gen.response_gen
-        if not hasattr(self, "_h0_flat"):
-            self._h0_flat = self.h0_sparse.reshape(3, -1).contiguous()
         self.backend.hdyn_fused(d_h, h_h, rows, t_start, t_end, ells, mms, self.freqs, self._h0_flat,
                                 self.data_constants, gen.amp_phase_gen.qnm_packed, gen.amp_phase_gen.dspin,
                                 gen.amp_phase_gen.initial_t_val, resp.cpp_response, resp.TDItag_int, resp.rescaled,
@@ -423,8 +423,6 @@ class HeterodynedLikelihood(BBHxParallelModule):
         grad = torch.empty((11, B), dtype=torch.float64, device=self.device)
         ll = torch.empty(B, dtype=torch.float64, device=self.device)
         resp = gen.response_gen
-        if not hasattr(self, "_h0_flat"):
-            self._h0_flat = self.h0_sparse.reshape(3, -1).contiguous()
         self.backend.hdyn_fused_grad(d_h, h_h, grad, rows, t_start, t_end, ells, mms, self.freqs, self._h0_flat,
                                      self.data_constants, gen.amp_phase_gen.qnm_packed, gen.amp_phase_gen.dspin,
                                      gen.amp_phase_gen.initial_t_val, resp.cpp_response, resp.TDItag_int,

@@ -136,95 +136,104 @@ class LISATDIResponse(BBHxParallelModule):
     def tf(self):
         return self._slab(1)
 
+    # ---- host-side checks (no device work; the reference's raise points)
+
+    @staticmethod
+    def buffer_layout(numel, num_bin_all, num_modes, length):
+        """(nparams, includes_amps) of a caller's out_buffer of ``numel`` doubles:
+        8 slabs (phase, tf, 6 transfer parts) or 9 (amp first) of B * M * L each
+        (fastfdresponse.py:389-421); anything else is a ValueError."""
+        per = num_bin_all * num_modes * length
+        slabs, rem = divmod(int(numel), per) if per > 0 else (0, 1)
+        if rem or slabs not in (8, 9):
+            raise ValueError(f"out_buffer incorrect length. The length should be equivalent to (8 or 9) * {per}. "
+                             f"Given length is {int(numel)}.")
+        return slabs, int(slabs == 9)
+
+    @staticmethod
+    def check_freqs(shape, numel, num_bin_all, num_modes, length):
+        """Frequencies are either (B, M, L) or flat with B * M * L entries."""
+        want = (num_bin_all, num_modes, length)
+        if len(shape) > 1:
+            if tuple(shape) != want:
+                raise ValueError(f"freqs have incorrect shape. Shape should be {want}. Current shape is {tuple(shape)}.")
+        elif int(numel) != num_bin_all * num_modes * length:
+            raise ValueError(f"freqs incorrect length. The length should be equivalent to "
+                             f"{num_bin_all * num_modes * length}. Given length is {int(numel)}.")
+
+    @staticmethod
+    def check_orbit_coverage(tf_min, tf_max, t_lo, t_hi):
+        """Every t(f) must lie inside the orbit table (fastfdresponse.py:486-492,
+        where the reference stops in a debugger; here it is a ValueError)."""
+        if tf_min < t_lo or tf_max > t_hi:
+            raise ValueError(f"Orbital information does not cover minimum ({tf_min}) and maximum ({tf_max}) tf. "
+                             f"Orbital information begins at {t_lo} and ends at {t_hi}.")
+
+    def _mode_arrays(self, modes):
+        torch = self.xp
+        if modes is None:
+            return self.ells_default, self.mms_default
+        ells_h = np.asarray([ell for ell, mm in modes], dtype=np.int32)
+        mms_h = np.asarray([mm for ell, mm in modes], dtype=np.int32)
+        self._sanity_check_modes(ells_h, mms_h)
+        return torch.as_tensor(ells_h, device=self.device), torch.as_tensor(mms_h, device=self.device)
+
     def __call__(self, freqs, inc, lam, beta, psi, phi_ref, length, modes=None, phase=None, tf=None,
                  out_buffer=None, adjust_phase=True, direct=False):
-        """Evaluate the response (fastfdresponse.py:278-531)."""
-        torch = self.xp
+        """Transfer functions and Doppler-shifted phase of every (binary, mode,
+        frequency) (fastfdresponse.py:278-531). Either ``out_buffer`` (the
+        waveform buffer, 8 or 9 slabs, phase and tf already in place) or both
+        ``phase`` and ``tf`` are given; with the latter the shifted phase is
+        written back into ``phase`` when ``adjust_phase``."""
         dev = self.device
         inc, lam, beta, psi, phi_ref = (to_dev(as_np1d(v), dev) for v in (inc, lam, beta, psi, phi_ref))
-
-        if modes is not None:
-            ells_h = np.asarray([ell for ell, mm in modes], dtype=np.int32)
-            mms_h = np.asarray([mm for ell, mm in modes], dtype=np.int32)
-            self._sanity_check_modes(ells_h, mms_h)
-            ells = torch.as_tensor(ells_h, device=dev)
-            mms = torch.as_tensor(mms_h, device=dev)
-        else:
-            ells, mms = self.ells_default, self.mms_default
+        ells, mms = self._mode_arrays(modes)
         self.modes = [(int(ell), int(mm)) for ell, mm in zip(ells.cpu().tolist(), mms.cpu().tolist())]
-
-        num_modes = int(ells.numel())
-        num_bin_all = int(inc.numel())
-        self.length = int(length)
-        self.num_modes = num_modes
-        self.num_bin_all = num_bin_all
-        self.num_per_param = self.length * num_modes * num_bin_all
-        self.num_per_bin = self.length * num_modes
+        B, M, L = int(inc.numel()), int(ells.numel()), int(length)
+        self.length, self.num_modes, self.num_bin_all = L, M, B
+        self.num_per_bin = L * M
+        self.num_per_param = n = B * M * L
         self.nresponse_params = 6
 
-        if out_buffer is None and phase is None and tf is None:
+        given_pt = phase is not None or tf is not None
+        if given_pt and (phase is None or tf is None):
+            raise ValueError("If provided phase or tf, need to provide both.")
+        if out_buffer is None and not given_pt:
             raise ValueError("Must provide either out_buffer or both phase and tf.")
-
         if out_buffer is None:
-            includes_amps = 0
-            self.nparams = includes_amps + 2 + self.nresponse_params
+            self.nparams, self.includes_amps = 8, 0
             self._initialize_response_container()
         else:
-            nparams_empirical = out_buffer.numel() / (self.num_bin_all * self.num_modes * self.length)
-            if np.allclose(nparams_empirical, np.round(nparams_empirical)) and int(nparams_empirical) in [8, 9]:
-                self.nparams = int(nparams_empirical)
-            else:
-                raise ValueError(
-                    f"out_buffer incorrect length. The length should be equivalent to (8 or 9) * "
-                    f"{self.num_bin_all * self.num_modes * self.length}. Given length is {out_buffer.numel()}.")
-            includes_amps = 1 if self.nparams == 9 else 0
+            self.nparams, self.includes_amps = self.buffer_layout(out_buffer.numel(), B, M, L)
             self.response_carrier = out_buffer
-        self.includes_amps = includes_amps
+        ia = self.includes_amps
 
         freqs = to_dev(freqs, dev)
-        if freqs.ndim > 1:
-            if tuple(freqs.shape) != (self.num_bin_all, self.num_modes, self.length):
-                raise ValueError(f"freqs have incorrect shape. Shape should be "
-                                 f"{(self.num_bin_all, self.num_modes, self.length)}. Current shape is {tuple(freqs.shape)}.")
-            freqs = freqs.flatten()
-        elif freqs.numel() != self.num_bin_all * self.num_modes * self.length:
-            raise ValueError(f"freqs incorrect length. The length should be equivalent to "
-                             f"{self.num_bin_all * self.num_modes * self.length}. Given length is {freqs.numel()}.")
-        self.freqs = freqs
+        self.check_freqs(tuple(freqs.shape), freqs.numel(), B, M, L)
+        self.freqs = freqs.flatten() if freqs.ndim > 1 else freqs
 
         t_lo, t_hi = float(self.orbits.t_base.min()), float(self.orbits.t_base.max())
-        n = self.num_per_param
-        if phase is not None and tf is not None:
-            use_phase_tf = True
+        if given_pt:
             if tuple(phase.shape) != tuple(tf.shape):
                 raise ValueError("Shape of phase array and tf array need to be the same shape.")
             phase_d, tf_d = to_dev(phase, dev), to_dev(tf, dev)
-            if not direct and (float(tf_d.min()) < t_lo or float(tf_d.max()) > t_hi):
-                raise ValueError(f"Orbital information does not cover minimum ({float(tf_d.min())}) and maximum "
-                                 f"({float(tf_d.max())}) tf. Orbital information begins at {t_lo} and ends at {t_hi}.")
             if phase_d.numel() != n:
                 raise ValueError(f"phase incorrect length. The length should be equivalent to {n}. "
                                  f"Given length is {phase_d.numel()}.")
-            self.response_carrier[includes_amps * n:(includes_amps + 1) * n] = phase_d.flatten()
-            self.response_carrier[(includes_amps + 1) * n:(includes_amps + 2) * n] = tf_d.flatten()
-        elif phase is not None or tf is not None:
-            raise ValueError("If provided phase or tf, need to provide both.")
-        else:
-            use_phase_tf = False
             if not direct:
-                tfv = self.tf
-                lo, hi = float(tfv.min()), float(tfv.max())
-                if lo < t_lo or hi > t_hi:
-                    raise ValueError(f"Orbital information does not cover minimum ({lo}) and maximum ({hi}) tf. "
-                                     f"Orbital information begins at {t_lo} and ends at {t_hi}.")
+                self.check_orbit_coverage(float(tf_d.min()), float(tf_d.max()), t_lo, t_hi)
+            self.response_carrier[ia * n:(ia + 1) * n] = phase_d.flatten()
+            self.response_carrier[(ia + 1) * n:(ia + 2) * n] = tf_d.flatten()
+        elif not direct:
+            tfv = self.tf
+            self.check_orbit_coverage(float(tfv.min()), float(tfv.max()), t_lo, t_hi)
 
         self.response_gen(self.response_carrier, ells, mms, self.freqs, phi_ref, inc, lam, beta, psi,
-                          self.TDItag_int, self.rescaled, self.tdi2, self.order_fresnel_stencil, num_modes,
-                          self.length, num_bin_all, includes_amps)
+                          self.TDItag_int, self.rescaled, self.tdi2, self.order_fresnel_stencil, M, L, B, ia)
 
-        if use_phase_tf and adjust_phase:
-            output = self.response_carrier[includes_amps * n:(includes_amps + 1) * n]
-            if isinstance(phase, torch.Tensor):
-                phase.copy_(output.reshape(phase.shape).to(phase.device))
+        if given_pt and adjust_phase:
+            shifted = self.response_carrier[ia * n:(ia + 1) * n].reshape(phase.shape)
+            if isinstance(phase, self.xp.Tensor):
+                phase.copy_(shifted.to(phase.device))
             else:
-                phase[...] = output.reshape(phase.shape).cpu().numpy()
+                phase[...] = shifted.cpu().numpy()

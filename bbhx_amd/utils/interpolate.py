@@ -19,30 +19,11 @@ class CubicSplineInterpolant(BBHxParallelModule):
         torch = self.xp
         x = to_dev(x, self.device)
         y_all = to_dev(y_all, self.device)
-        if x.ndim == 1 or y_all.ndim == 1:
-            if x.ndim != 1 or y_all.ndim != 1:
-                raise ValueError("If providing flattened x and y_all, need to both be flattened.")
-            if length is None or num_modes is None or num_bin_all is None or num_interp_params is None:
-                raise ValueError("If providing flattened arrays, need to provide dimensional information: "
-                                 "length, num_modes, num_bin_all, num_interp_params.")
-            if x.numel() != length * num_modes * num_bin_all:
-                raise ValueError(f"Length of the x array is not correct. It is supposed to be "
-                                 f"{length * num_modes * num_bin_all}. It is currently {x.numel()}.")
-            if y_all.numel() != length * num_modes * num_bin_all * num_interp_params:
-                raise ValueError(f"Length of the y_all array is not correct. It is supposed to be "
-                                 f"{length * num_modes * num_bin_all * num_interp_params}. It is currently {y_all.numel()}.")
-        else:
-            if x.ndim != 3:
-                raise ValueError("If providing shaped array, needs to be 3D with shape (num_bin_all, num_modes, length).")
-            if y_all.ndim != 4:
-                raise ValueError("If providing shaped array, needs to be 4D with shape "
-                                 "(num_interp_params, num_bin_all, num_modes, length).")
-            num_interp_params, num_bin_all, num_modes, length = y_all.shape
-            assert tuple(y_all.shape[1:]) == tuple(x.shape)
-            x = x.flatten()
-            y_all = y_all.flatten()
-        if length < 3:
-            raise ValueError("splines need at least 3 knots.")
+        num_interp_params, num_bin_all, num_modes, length = self.spline_dims(
+            x.ndim, x.numel(), tuple(x.shape), y_all.ndim, y_all.numel(), tuple(y_all.shape), num_interp_params,
+            num_bin_all, num_modes, length)
+        x = x.flatten()
+        y_all = y_all.flatten()
         ninterps = num_modes * num_interp_params * num_bin_all
         self.degree = 3
         self.length = length
@@ -57,6 +38,37 @@ class CubicSplineInterpolant(BBHxParallelModule):
         self.interpolate_arrays(x, y_all, B, upper_diag, diag, lower_diag, length, num_interp_params, num_modes,
                                 num_bin_all)
         self.x = x.clone()
+
+    @staticmethod
+    def spline_dims(x_ndim, x_numel, x_shape, y_ndim, y_numel, y_shape, num_interp_params=None, num_bin_all=None,
+                    num_modes=None, length=None):
+        """(num_interp_params, num_bin_all, num_modes, length) of the arrays, with
+        the reference's shape rules (interpolate.py:93-126): both flat with the
+        four sizes given, or x (B, M, L) and y_all (P, B, M, L); at least 3 knots."""
+        if x_ndim == 1 or y_ndim == 1:
+            if x_ndim != 1 or y_ndim != 1:
+                raise ValueError("If providing flattened x and y_all, need to both be flattened.")
+            if length is None or num_modes is None or num_bin_all is None or num_interp_params is None:
+                raise ValueError("If providing flattened arrays, need to provide dimensional information: "
+                                 "length, num_modes, num_bin_all, num_interp_params.")
+            if x_numel != length * num_modes * num_bin_all:
+                raise ValueError(f"Length of the x array is not correct. It is supposed to be "
+                                 f"{length * num_modes * num_bin_all}. It is currently {x_numel}.")
+            if y_numel != length * num_modes * num_bin_all * num_interp_params:
+                raise ValueError(f"Length of the y_all array is not correct. It is supposed to be "
+                                 f"{length * num_modes * num_bin_all * num_interp_params}. It is currently {y_numel}.")
+        else:
+            if x_ndim != 3:
+                raise ValueError("If providing shaped array, needs to be 3D with shape (num_bin_all, num_modes, length).")
+            if y_ndim != 4:
+                raise ValueError("If providing shaped array, needs to be 4D with shape "
+                                 "(num_interp_params, num_bin_all, num_modes, length).")
+            if tuple(y_shape[1:]) != tuple(x_shape):
+                raise ValueError(f"y_all shape {tuple(y_shape)} does not extend x shape {tuple(x_shape)}.")
+            num_interp_params, num_bin_all, num_modes, length = (int(v) for v in y_shape)
+        if length < 3:
+            raise ValueError("splines need at least 3 knots.")
+        return int(num_interp_params), int(num_bin_all), int(num_modes), int(length)
 
     @property
     def interpolate_arrays(self):

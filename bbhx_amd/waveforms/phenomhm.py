@@ -231,6 +231,55 @@ class PhenomHMAmpPhase(BBHxParallelModule):
         self.waveform_carrier[2 * npp: 3 * npp] += tref_d[:, None, None].expand(
             num_bin_all, num_modes, self.length).flatten()
 
+    @staticmethod
+    def tobs_grid(Tobs, B, M, device):
+        """The observation length per (binary, mode) as a (B, M) float64 tensor:
+        None (no upper cut), a float, a (B,) or a (B, M) numpy array
+        (phenomhm.py:561-575)."""
+        import torch
+
+        if Tobs is None:
+            return torch.full((B, M), 1e300, dtype=torch.float64, device=device)
+        if isinstance(Tobs, float):
+            return torch.full((B, M), Tobs, dtype=torch.float64, device=device)
+        if isinstance(Tobs, np.ndarray) and Tobs.ndim == 1 and Tobs.shape[0] == B:
+            return to_dev(Tobs, device)[:, None].expand(B, M).contiguous()
+        if isinstance(Tobs, np.ndarray) and Tobs.shape == (B, M):
+            return to_dev(Tobs, device)
+        raise ValueError("Tobs entered incorrectly.")
+
+    @staticmethod
+    def window_bounds(tf, fs, Tobs_t):
+        """Frequency range of each (binary, mode) grid with 0 <= t(f) <= Tobs
+        (phenomhm.py:520-600): where t(f) crosses 0 (or Tobs) inside the grid,
+        the crossing frequency by linear interpolation between the two grid
+        points around it; otherwise the grid's own end. tf, fs: (B, M, L);
+        Tobs_t: (B, M). Raises if a grid crosses a bound more than once."""
+        import torch
+
+        def cross(level, lower):
+            f_end = (fs.min(dim=-1).values if lower else fs.max(dim=-1).values).clone()
+            fix = (tf.min(dim=-1).values < level) if lower else (tf.max(dim=-1).values > level)
+            nfix = int(fix.sum().item())
+            if nfix == 0:
+                return f_end
+            tfx, fsx, lv = tf[fix], fs[fix], level[fix][:, None]
+            hit = torch.nonzero((lv < tfx[:, 1:]) & (lv > tfx[:, :-1]))
+            if hit.shape[0] != nfix:
+                where = "x_min" if lower else "x_max"
+                hint = "zero" if lower else "the observation time"
+                raise ValueError(f"The search for a window at {where} did not run correctly. If t_ref is close to "
+                                 f"{hint}, the time-frequency correspondence may be crossing zero twice.")
+            seg = hit[:, 1]
+            rows = torch.arange(nfix, device=tf.device)
+            y1, y2 = fsx[rows, seg], fsx[rows, seg + 1]
+            x1, x2 = tfx[rows, seg], tfx[rows, seg + 1]
+            f_end[fix] = (y2 - y1) / (x2 - x1) * (lv[:, 0] - x1) + y1
+            return f_end
+
+        zero = torch.zeros_like(Tobs_t)
+        return cross(zero, True), cross(Tobs_t, False)
+
     def __call__(self, *args, Tobs=None, direct=False, **kwargs):
         """Run the waveform; unless ``direct``, refit each (binary, mode) grid to
         the part with 0 <= t(f) <= Tobs and run again (phenomhm.py:509-614)."""
@@ -238,55 +287,8 @@ class PhenomHMAmpPhase(BBHxParallelModule):
         self.run_wave(*args, **kwargs)
         if direct:
             return
-        B, M = self.num_bin_all, self.num_modes
-        tf = self.tf
-        fs = self.freqs_shaped
-
-        fix = tf.min(dim=-1).values < 0.0
-        f_min = fs.min(dim=-1).values.clone()
-        nfix = int(fix.sum().item())
-        if nfix > 0:
-            tfx, fsx = tf[fix], fs[fix]
-            win = torch.nonzero((0.0 < tfx[:, 1:]) & (0.0 > tfx[:, :-1]))
-            if win.shape[0] != nfix:
-                raise ValueError("The search for a window at x_min did not run correctly. If t_ref is close to zero, "
-                                 "the time-frequency correspondence may be crossing zero twice.")
-            seg = win[:, 1]
-            rows = torch.arange(nfix, device=self.device)
-            y1, y2 = fsx[rows, seg], fsx[rows, seg + 1]
-            x1, x2 = tfx[rows, seg], tfx[rows, seg + 1]
-            mslope = (y2 - y1) / (x2 - x1)
-            f_min[fix] = mslope * (0.0 - x1) + y1
-
-        if Tobs is None:
-            Tobs_t = torch.full((B, M), 1e300, dtype=torch.float64, device=self.device)
-        elif isinstance(Tobs, float):
-            Tobs_t = torch.full((B, M), Tobs, dtype=torch.float64, device=self.device)
-        elif isinstance(Tobs, np.ndarray) and Tobs.ndim == 1:
-            assert Tobs.shape[0] == B
-            Tobs_t = to_dev(Tobs, self.device)[:, None].expand(B, M).contiguous()
-        elif isinstance(Tobs, np.ndarray) and Tobs.ndim == 2:
-            assert Tobs.shape == (B, M)
-            Tobs_t = to_dev(Tobs, self.device)
-        else:
-            raise ValueError("Tobs entered incorrectly.")
-
-        fix = tf.max(dim=-1).values > Tobs_t
-        f_max = fs.max(dim=-1).values.clone()
-        nfix = int(fix.sum().item())
-        if nfix > 0:
-            tfx, fsx, Tx = tf[fix], fs[fix], Tobs_t[fix]
-            win = torch.nonzero((Tx[:, None] < tfx[:, 1:]) & (Tx[:, None] > tfx[:, :-1]))
-            if win.shape[0] != nfix:
-                raise ValueError("The search for a window at x_max did not run correctly. If t_ref is close to the "
-                                 "observation time, the time-frequency correspondence may be crossing zero twice.")
-            seg = win[:, 1]
-            rows = torch.arange(nfix, device=self.device)
-            y1, y2 = fsx[rows, seg], fsx[rows, seg + 1]
-            x1, x2 = tfx[rows, seg], tfx[rows, seg + 1]
-            mslope = (y2 - y1) / (x2 - x1)
-            f_max[fix] = mslope * (Tx - x1) + y1
-
+        Tobs_t = self.tobs_grid(Tobs, self.num_bin_all, self.num_modes, self.device)
+        f_min, f_max = self.window_bounds(self.tf, self.freqs_shaped, Tobs_t)
         new_freqs = np_logspace_rows(torch.log10(f_min), torch.log10(f_max), self.length)
         tmp_kwargs = kwargs.copy()
         tmp_kwargs["freqs"] = new_freqs

@@ -18,9 +18,12 @@
  *   ringdown frequencies         b*(M+1) + m, PhenomD value in column M
  *   complex arrays               interleaved (re, im) doubles
  *
- * The library keeps per-device grow-only scratch buffers (coefficient records,
- * band tables, partial sums). Calls that run concurrently on different streams
- * of one device share them and must not overlap.
+ * The library keeps grow-only scratch buffers (coefficient records, band
+ * tables, partial sums), one set per (device, stream): calls queued on
+ * different streams never share scratch memory. A buffer that is replaced by a
+ * larger one is kept until bbhx_hip_release_workspace, so a launch sequence
+ * captured in a graph stays valid after later, larger calls on the same
+ * stream; replays of one graph must not run concurrently with each other.
  *
  * Reference file:line citations are relative to src/bbhx/ of mikekatz04/BBHx.
  */
@@ -279,7 +282,9 @@ int bbhx_hip_hdyn_prep(double* dataConstants, double* totals, const double* data
 int bbhx_hip_timing_enable(int on);
 int bbhx_hip_timing_read(double* ms_setup, double* ms_eval, int* ncalls);
 
-/* Releases the library's scratch buffers on the current device (synchronises it). */
+/* Releases the library's scratch buffers on the current device, retired ones
+ * included (synchronises it). Graphs captured before the call must not be
+ * replayed after it. */
 int bbhx_hip_release_workspace(void);
 
 /* Library version string. */
