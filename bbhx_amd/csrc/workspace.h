@@ -7,8 +7,11 @@
 // a larger one is retired, not freed, until workspace_release_all(), so kernels
 // still queued on the old buffer, and launch sequences captured in a graph
 // before the growth, keep valid memory (no device synchronisation and no free
-// on the growth path, so growing inside a graph capture is also safe). In
-// steady state (same or smaller problem sizes) no allocation happens on any call.
+// on the growth path). A buffer never grows while its stream is being captured
+// into a graph: an allocation there would invalidate the capture, so the
+// request fails instead, and the caller must first run the same work once,
+// eagerly, on the capture stream. In steady state (same or smaller problem
+// sizes) no allocation happens on any call.
 // Calls that overlap on one stream cannot overlap in time (a stream runs its
 // work in order); replays of one graph that run concurrently with each other
 // would share its buffers and must not be issued.
@@ -35,7 +38,8 @@ enum WsSlot : int {
 };
 
 // Returns a device pointer with at least 'bytes' bytes on the current device,
-// private to work queued on 'stream', or nullptr on allocation failure.
+// private to work queued on 'stream', or nullptr on allocation failure (also
+// when the buffer would have to grow while 'stream' is capturing).
 void* workspace_get(int slot, size_t bytes, hipStream_t stream);
 
 // Frees every buffer of the current device, retired ones included (after a

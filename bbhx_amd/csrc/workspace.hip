@@ -32,6 +32,15 @@ void* workspace_get(int slot, size_t bytes, hipStream_t stream)
     DeviceWs& d = g_ws[dev];
     Buf& b = d.sets[stream].slot[slot];
     if (b.bytes >= bytes && b.ptr) return b.ptr;
+    // never allocate while the stream is being captured into a graph (an
+    // allocation there invalidates the capture): the caller must run the same
+    // entry point once on the capture stream, eagerly, before capturing
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(stream, &cs) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    if (cs != hipStreamCaptureStatusNone) return nullptr;
     // grow by at least 1.5x to avoid repeated reallocation on slowly growing sizes
     size_t want = bytes < 4096 ? 4096 : bytes;
     if (b.bytes && want < b.bytes + b.bytes / 2) want = b.bytes + b.bytes / 2;

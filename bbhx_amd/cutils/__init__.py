@@ -28,7 +28,8 @@ _ip = C.POINTER(C.c_int)
 _i64p = C.POINTER(C.c_int64)
 _vp = C.c_void_p
 
-STATUS = {0: "ok", 1: "invalid arguments", 2: "kernel launch/runtime error", 3: "workspace allocation failed",
+STATUS = {0: "ok", 1: "invalid arguments", 2: "kernel launch/runtime error",
+          3: "workspace allocation failed (under graph capture: run the call once on the capture stream first)",
           4: "unsupported (l, m) mode"}
 
 

@@ -38,8 +38,9 @@ def log(*a):
 
 def graph_time(fn, steps, warmup):
     """Capture fn() (device work only, no host synchronisation) in a hipGraph
-    after warm-up calls on a side stream; returns (ms per replay over `steps`
-    replays, the captured outputs)."""
+    after warm-up calls on the capture stream (the library sizes its scratch
+    buffers there; it never allocates during a capture); returns (ms per
+    replay over `steps` replays, the captured outputs)."""
     import torch
 
     st = torch.cuda.Stream()
@@ -50,7 +51,7 @@ def graph_time(fn, steps, warmup):
     torch.cuda.current_stream().wait_stream(st)
     torch.cuda.synchronize()
     g = torch.cuda.CUDAGraph()
-    with torch.cuda.graph(g):
+    with torch.cuda.graph(g, stream=st):
         out = fn()
     torch.cuda.synchronize()
     g.replay()
@@ -226,7 +227,7 @@ def bench_direct_ragged(args):
     torch.cuda.current_stream().wait_stream(st)
     torch.cuda.synchronize()
     g = torch.cuda.CUDAGraph()
-    with torch.cuda.graph(g):
+    with torch.cuda.graph(g, stream=st):
         flags.zero_()
         out = like.get_ll_fused_device(plan, rows, ts, te, flags, MODES)
     torch.cuda.synchronize()
@@ -265,7 +266,7 @@ def bench_direct_ragged(args):
         like.get_ll_fused_device(plan, rows, ts, te, flags, MODES, precision=1)  # allocates the records
         torch.cuda.synchronize()
         g32 = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(g32):
+        with torch.cuda.graph(g32, stream=st):
             flags.zero_()
             out32 = like.get_ll_fused_device(plan, rows, ts, te, flags, MODES)
         torch.cuda.synchronize()

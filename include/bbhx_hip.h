@@ -24,6 +24,10 @@
  * larger one is kept until bbhx_hip_release_workspace, so a launch sequence
  * captured in a graph stays valid after later, larger calls on the same
  * stream; replays of one graph must not run concurrently with each other.
+ * Scratch is never allocated while the stream is being captured (that would
+ * invalidate the capture; the call returns 3 instead): before capturing a
+ * call, run it once eagerly on the capture stream with a batch at least as
+ * large.
  *
  * Reference file:line citations are relative to src/bbhx/ of mikekatz04/BBHx.
  */

@@ -146,7 +146,7 @@ def test_grad_graph_replays(grad_problem):
     torch.cuda.current_stream().wait_stream(st)
     torch.cuda.synchronize()
     g = torch.cuda.CUDAGraph()
-    with torch.cuda.graph(g):
+    with torch.cuda.graph(g, stream=st):  # the warm-up stream: the library's scratch exists there
         ll_s, grad_s = het.get_ll_grad_device(s_rows, s_ts, s_te)
     torch.cuda.synchronize()
     rng = np.random.default_rng(9)
