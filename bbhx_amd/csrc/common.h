@@ -34,6 +34,7 @@ enum Status : int {
     ERR_LAUNCH = 2,    // kernel launch or runtime error
     ERR_ALLOC = 3,     // workspace allocation failed
     ERR_MODE = 4,      // unsupported (l, m)
+    ERR_CAPACITY = 5,  // batch larger than a fixed-capacity (allocation-free) entry point holds
 };
 
 struct cd {

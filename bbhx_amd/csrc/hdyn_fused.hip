@@ -17,6 +17,8 @@
 //
 // FP64 throughout ("plain" form). No atomics; the reduction order is fixed.
 
+#include <map>
+#include <mutex>
 #include <vector>
 
 #include "bbhx_hip.h"
@@ -205,6 +207,35 @@ int hdyn_fused_check(double* d_h, double* h_h, const double* params, const doubl
     return check_modes(ells_host, mms_host, numModes, ml);
 }
 
+// ---- the allocation-free form: records in a fixed device array of the library
+
+constexpr long NOALLOC_MAX_RECORDS = BBHX_HIP_NOALLOC_MAX_RECORDS;
+
+// One copy per device, held until the process ends. In host code the name is
+// only a stand-in: its device address comes from hipGetSymbolAddress.
+static __device__ __align__(256) unsigned char scratch[sizeof(FusedRec) * NOALLOC_MAX_RECORDS];
+
+std::mutex g_scratch_mutex;
+std::map<int, void*> g_scratch_addr;  // device -> address of 'scratch' on that device
+
+// Address of 'scratch' on the current device, looked up once per device (with
+// that device current) and kept; nullptr if the lookup fails.
+void* scratch_address()
+{
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    auto it = g_scratch_addr.find(dev);
+    if (it != g_scratch_addr.end()) return it->second;
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    g_scratch_addr[dev] = p;
+    return p;
+}
+
 }  // namespace
 
 extern "C" int bbhx_hip_hdyn_fused(double* d_h, double* h_h, const double* params, const double* t_start,
@@ -227,6 +258,27 @@ extern "C" int bbhx_hip_hdyn_fused(double* d_h, double* h_h, const double* param
     return hdyn_fused_launch(recs, d_h, h_h, params, t_start, t_end, numBinAll, ml, numModes, freqs, h0,
                              dataConstants, K, qnm, table_len, dspin, initial_t_val, orb, TDItag, rescaled, tdi2,
                              include_last_segment, logL, d_d, st, true);
+}
+
+extern "C" int bbhx_hip_hdyn_fused_noalloc(double* d_h, double* h_h, const double* params, const double* t_start,
+                                           const double* t_end, int numBinAll, const int* ells_host,
+                                           const int* mms_host, int numModes, const double* freqs, const double* h0,
+                                           const double* dataConstants, int K, const double* qnm, int table_len,
+                                           double dspin, double initial_t_val, const bbhx_response* orbits,
+                                           int TDItag, int rescaled, int tdi2, int include_last_segment,
+                                           int precision, double* logL, double d_d, void* stream)
+{
+    resp::OrbitTab orb;
+    ModeList ml;
+    const int chk = hdyn_fused_check(d_h, h_h, params, t_start, t_end, numBinAll, ells_host, mms_host, numModes,
+                                     freqs, h0, dataConstants, K, qnm, table_len, orbits, precision, orb, ml);
+    if (chk != OK) return chk < 0 ? OK : chk;
+    if ((long)numBinAll * numModes > NOALLOC_MAX_RECORDS) return ERR_CAPACITY;
+    FusedRec* recs = static_cast<FusedRec*>(scratch_address());
+    if (!recs) return ERR_LAUNCH;
+    return hdyn_fused_launch(recs, d_h, h_h, params, t_start, t_end, numBinAll, ml, numModes, freqs, h0,
+                             dataConstants, K, qnm, table_len, dspin, initial_t_val, orb, TDItag, rescaled, tdi2,
+                             include_last_segment, logL, d_d, reinterpret_cast<hipStream_t>(stream), false);
 }
 
 extern "C" int bbhx_hip_release_workspace(void)

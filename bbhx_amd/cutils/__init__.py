@@ -30,7 +30,11 @@ _vp = C.c_void_p
 
 STATUS = {0: "ok", 1: "invalid arguments", 2: "kernel launch/runtime error",
           3: "workspace allocation failed (under graph capture: run the call once on the capture stream first)",
-          4: "unsupported (l, m) mode"}
+          4: "unsupported (l, m) mode",
+          5: "batch above the capacity of the allocation-free entry point (BBHX_HIP_NOALLOC_MAX_RECORDS records)"}
+
+# (template, mode) records held by bbhx_hip_hdyn_fused_noalloc (include/bbhx_hip.h)
+NOALLOC_MAX_RECORDS = 24576
 
 
 class BackendUnavailableException(RuntimeError):
@@ -73,6 +77,9 @@ def load_library():
         "bbhx_hip_hdyn_fused": [_vp, _vp, _vp, _vp, _vp, C.c_int, _ip, _ip, C.c_int, _vp, _vp, _vp, C.c_int, _vp,
                                 C.c_int, C.c_double, C.c_double, _vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                 _vp, C.c_double, _vp],
+        "bbhx_hip_hdyn_fused_noalloc": [_vp, _vp, _vp, _vp, _vp, C.c_int, _ip, _ip, C.c_int, _vp, _vp, _vp, C.c_int,
+                                        _vp, C.c_int, C.c_double, C.c_double, _vp, C.c_int, C.c_int, C.c_int, C.c_int,
+                                        C.c_int, _vp, C.c_double, _vp],
         "bbhx_hip_hdyn_fused_grad": [_vp, _vp, _vp, _vp, _vp, _vp, C.c_int, _ip, _ip, C.c_int, _vp, _vp, _vp,
                                      C.c_int, _vp, C.c_int, C.c_double, C.c_double, _vp, C.c_int, C.c_int, C.c_int,
                                      C.c_int, _vp, C.c_double, _vp],
@@ -109,6 +116,7 @@ def exported_symbols():
         "bbhx_hip_get_phenomd_ringdown_frequencies", "bbhx_hip_response_create", "bbhx_hip_response_destroy",
         "bbhx_hip_response_add_orbits", "bbhx_hip_response_compute", "bbhx_hip_interpolate", "bbhx_hip_InterpTDI",
         "bbhx_hip_direct_sum", "bbhx_hip_hdyn", "bbhx_hip_direct_like", "bbhx_hip_hdyn_fused",
+        "bbhx_hip_hdyn_fused_noalloc",
         "bbhx_hip_hdyn_fused_grad", "bbhx_hip_direct_plan_create", "bbhx_hip_direct_plan_destroy",
         "bbhx_hip_direct_plan_bytes", "bbhx_hip_direct_plan_band", "bbhx_hip_direct_fused", "bbhx_hip_hdyn_prep",
         "bbhx_hip_direct_fused_grad", "bbhx_hip_direct_plan_enable_grad", "bbhx_hip_direct_plan_set_precision",
@@ -339,22 +347,27 @@ def _check_hdyn_sizes(params, d_h, h_h, t_start, t_end, freqs, h0, data_constant
 
 
 def hdyn_fused(d_h, h_h, params, t_start, t_end, ells, mms, freqs, h0, data_constants, qnm, dspin, initial_t_val,
-               response, TDItag, rescaled, tdi2, include_last_segment=True, precision=0, logL=None, d_d=0.0):
+               response, TDItag, rescaled, tdi2, include_last_segment=True, precision=0, logL=None, d_d=0.0,
+               noalloc=False):
     """Fused heterodyned likelihood (no reference counterpart; see include/bbhx_hip.h).
-    ``logL`` (B,) float64 device, optional: also written, -1/2 (d_d + Re h_h - 2 Re d_h)."""
+    ``logL`` (B,) float64 device, optional: also written, -1/2 (d_d + Re h_h - 2 Re d_h).
+    ``noalloc``: run bbhx_hip_hdyn_fused_noalloc (no device allocation; at most
+    NOALLOC_MAX_RECORDS (template, mode) records; calls on one GPU must not overlap)."""
     torch = _torch()
     f64, c128 = torch.float64, torch.complex128
     B, K = _check_hdyn_sizes(params, d_h, h_h, t_start, t_end, freqs, h0, data_constants, qnm, logL)
     el, elp = _host_i32(ells)
     mm, mmp = _host_i32(mms)
     table_len = int(qnm.numel()) // 8
-    st = load_library().bbhx_hip_hdyn_fused(
+    lib = load_library()
+    entry = lib.bbhx_hip_hdyn_fused_noalloc if noalloc else lib.bbhx_hip_hdyn_fused
+    st = entry(
         _dev(d_h, c128), _dev(h_h, c128), _dev(params, f64, "params"), _dev(t_start, f64), _dev(t_end, f64), B,
         elp, mmp, int(el.size), _dev(freqs, f64), _dev(h0, c128, "h0"), _dev(data_constants, c128), K,
         _dev(qnm, f64), table_len, float(dspin), float(initial_t_val), response.handle, int(TDItag),
         int(bool(rescaled)), int(bool(tdi2)), int(bool(include_last_segment)), int(precision),
         None if logL is None else _dev(logL, f64, "logL"), float(d_d), _stream())
-    _check(st, "hdyn_fused")
+    _check(st, "hdyn_fused_noalloc" if noalloc else "hdyn_fused")
 
 
 def hdyn_fused_grad(d_h, h_h, grad, params, t_start, t_end, ells, mms, freqs, h0, data_constants, qnm, dspin,

@@ -7,7 +7,8 @@
  *     default stream) and returns as soon as the work is queued, except where a
  *     host-side table must be uploaded first (noted below);
  *   - returns 0 on success or a nonzero status: 1 invalid arguments,
- *     2 launch/runtime error, 3 workspace allocation failure, 4 unsupported mode.
+ *     2 launch/runtime error, 3 workspace allocation failure, 4 unsupported mode,
+ *     5 batch above the capacity of an allocation-free entry point.
  *     It never exits the process (the reference calls exit() on errors,
  *     cutils/global.h:14-22).
  *
@@ -175,6 +176,26 @@ int bbhx_hip_hdyn_fused(double* d_h, double* h_h, const double* params, const do
                         int K, const double* qnm, int table_len, double dspin, double initial_t_val,
                         const bbhx_response* orbits, int TDItag, int rescaled, int tdi2,
                         int include_last_segment, int precision, double* logL, double d_d, void* stream);
+
+/* No counterpart in the reference. The same computation as
+ * bbhx_hip_hdyn_fused, with the same arguments and results (bit for bit), for
+ * callers that cannot allocate device memory: code under HIP graph capture
+ * that must not call hipMalloc, tight serving loops, harnesses that forbid
+ * allocation. It calls no hipMalloc / hipFree and no allocating library
+ * routine: the per-(template, mode) records live in a fixed array of the
+ * library, BBHX_HIP_NOALLOC_MAX_RECORDS records (about 22 MB) per device,
+ * reserved when the library loads on a device and held until the process
+ * ends. numBinAll * numModes above that returns status 5. Every call on one
+ * device shares the array, which keeps the last call's records: calls that
+ * overlap on one GPU (on different streams) race and must not be issued.
+ * No event timing (bbhx_hip_timing_* does not see these calls). */
+#define BBHX_HIP_NOALLOC_MAX_RECORDS 24576
+int bbhx_hip_hdyn_fused_noalloc(double* d_h, double* h_h, const double* params, const double* t_start,
+                                const double* t_end, int numBinAll, const int* ells_host, const int* mms_host,
+                                int numModes, const double* freqs, const double* h0, const double* dataConstants,
+                                int K, const double* qnm, int table_len, double dspin, double initial_t_val,
+                                const bbhx_response* orbits, int TDItag, int rescaled, int tdi2,
+                                int include_last_segment, int precision, double* logL, double d_d, void* stream);
 
 /* Gradient of the heterodyned log-likelihood (no counterpart in the reference,
  * which has no derivatives). Same inputs as bbhx_hip_hdyn_fused (precision is
