@@ -85,6 +85,20 @@ __device__ inline double cbrt_pos(double x)
     return y * (y3 + 2.0 * x) * rcp_pos(2.0 * y3 + x);
 }
 
+// x^(1/6) for finite x > 0 in the single-precision range: a single-precision
+// estimate 2^(log2(x)/6) (relative error ~2e-7) and one Halley step for
+// y^6 = x, y (5 y^6 + 7 x) / (7 y^6 + 5 x), whose cubic convergence leaves
+// only FP64 rounding (within a few ulp of pow(x, 1/6)); one root in place of
+// a square root followed by a cube root.
+__device__ inline double root6_pos(double x)
+{
+    const float yf = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf((float)x) * (1.0f / 6.0f));
+    const double y = (double)yf;
+    const double y2 = y * y;
+    const double y6 = y2 * y2 * y2;
+    return y * fma(5.0, y6, 7.0 * x) * rcp_pos(fma(7.0, y6, 5.0 * x));
+}
+
 // A polynomial coefficient as a scalar-register operand at its point of use
 // (rebuilt there by two scalar moves): the FMA chains below then take it as
 // their addend directly, instead of copying a vector-register copy of the

@@ -253,6 +253,13 @@ __device__ inline Dual<N> cbrt_pos(const Dual<N>& x)
     return chain(c, c * rcp_pos(3.0 * x.v), x);
 }
 template <int N>
+__device__ inline Dual<N> root6_pos(const Dual<N>& x)
+{
+    const double c = root6_pos(x.v);
+    return chain(c, c * rcp_pos(6.0 * x.v), x);
+}
+
+template <int N>
 __device__ inline Dual<N> rcp_pos(const Dual<N>& x)
 {
     const double r = rcp_pos(x.v);

@@ -136,9 +136,14 @@ __device__ inline void residuals(cd* rs, const FusedRec* __restrict__ rb, int M,
             cd G[6];
             double dphi;
             resp::resp_links(r.s, orb, nr, f, tf, G, dphi, tab);
-            phase = phase + dphi;
+            // The waveform phase multiplies the link terms after they are
+            // formed, not inside their exponentials: the TDI combination takes
+            // differences of nearly equal link terms at low frequency, so the
+            // link terms' relative phases must not carry the rounding of this
+            // large phase (1e4-1e7 rad). Modes outside the observation window
+            // contribute nothing (WaveformBuild.cu:50).
             if (in_window(win, tf)) {
-                const cd ap = amp * cexpi_tab(phase, tab);
+                const cd ap = amp * cexpi_tab(phase + dphi, tab);
                 for (int k = 0; k < 6; k++) S[k] += G[k] * ap;
             }
         }

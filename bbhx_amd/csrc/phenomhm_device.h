@@ -13,6 +13,7 @@
 #pragma once
 
 #include "common.h"
+#include "dual.h"
 
 namespace bbhx {
 namespace phm {
@@ -118,7 +119,7 @@ template <class S>
 __device__ inline P6T<S> pow6(S x)
 {
     P6T<S> p;
-    p.sixth = cbrt_pos(sqrt(x));
+    p.sixth = root6_pos(x);
     p.third = p.sixth * p.sixth;
     p.two_thirds = p.third * p.third;
     p.four_thirds = x * p.third;
@@ -227,6 +228,9 @@ __device__ inline S pn_habs(const PhmRecT<S>& r, nd<S> v)
 // v4 = (m/2)^(1/3) v3, the v^-3.5 prefactors of the four PN amplitudes cancel
 // exactly, leaving |H(v1)| |H(v3)| / |H(v2)| (1 for the (2,2) mode; 0 where the
 // reference's beta_term1 vanishes).
+//
+// Below amp_fi the amplitude map is Mf_amp = (2/m) Mf (PhenomHMWaveform.cu:2200-2212),
+// so v3 = (2/m)^(1/3) v1 = v2 and the factor is |H(v1)| exactly.
 template <class S>
 __device__ inline S hm_amp_factor(const PhmRecT<S>& r, nd<S> Mf_wf, nd<S> Mf_amp)
 {
@@ -234,6 +238,7 @@ __device__ inline S hm_amp_factor(const PhmRecT<S>& r, nd<S> Mf_wf, nd<S> Mf_amp
     const S v1 = cbrt_pos(r.pn_vscale * Mf_wf);
     const S h1 = pn_habs(r, v1);
     if (h1 == 0.0) return 0.0;
+    if (!(Mf_wf > r.amp_fi)) return h1;
     const S v2 = v1 * r.pn_v2ratio;
     const S v3 = cbrt_pos(r.pn_vscale * Mf_amp);
     // |H(v2)| > 0 wherever |H(v1)| > 0 (the same factors of eta and delta vanish)
@@ -318,13 +323,18 @@ __device__ inline S dphi_mrd(const PhmRecT<S>& r, nd<S> f, nd<S> iMf)
            r.p_etaInv;
 }
 
-// IMRPhenDPhase and its derivative at the same Mf, sharing the powers.
+// IMRPhenDPhase and its derivative at the same Mf, sharing the powers. With
+// 'same' set (the lane's f equals the one 'pre' was built from), the inspiral
+// branch reuses the powers 'pre' instead of computing them again.
 template <class S>
-__device__ inline void phase_imr(const PhmRecT<S>& r, nd<S> f, S& ph, S* dph)
+__device__ inline void phase_imr(const PhmRecT<S>& r, nd<S> f, S& ph, S* dph, const P6T<S>* pre = nullptr,
+                                 bool same = false)
 {
     const S iMf = rcp_pos(f);
     if (!(f >= PHI_JOIN_INS)) {
-        P6T<S> p = pow6(f);
+        P6T<S> p;
+        if (pre && same) p = *pre;
+        else p = pow6(f);
         const double pi13 = 1.4645918875615231;
         S v = p.third * pi13;
         S logv = log(v);
@@ -869,7 +879,10 @@ __device__ inline void phm_eval(const PhmRecT<S>& r, double f, S& amp, S& phase,
         sa = r.ar; sb = r.br; c1 = r.PhDCconst; c2 = r.phaseC;
     }
     S ph, dph;
-    phase_imr(r, sa * Mf_wf + sb, ph, &dph);
+    // below both joins the amplitude and phase maps are the same (2/m) Mf
+    // (PhenomHMWaveform.cu:2200-2212, :2107-2150): the powers of Mf_amp serve both
+    const S Mf_ph = sa * Mf_wf + sb;
+    phase_imr(r, Mf_ph, ph, &dph, &p, value(Mf_ph) == value(Mf_amp));
     const S phi = r.cshift + ((ph * rcp_pos(sa) - c1) + c2);
     phase = -r.t0 * (Mf_wf - r.Mf_ref) + (phi - (r.mm * r.phi0));
     const double INV_TWO_PI = 0.15915494309189535;

@@ -16,8 +16,12 @@ module provides its own equal-arm model in the same layout:
 * link ``ij`` is received by spacecraft i and emitted by j; its unit vector
   points from j to i, ``n_ij = (x_i - x_j) / |x_i - x_j|``.
 
-Any object exposing the same attributes (``pycppdetector_args``, ``t_base``,
-``LINKS``) can be passed to :class:`LISATDIResponse` instead.
+Orbits from elsewhere (a lisaanalysistools orbit file exported to arrays, a
+numerical orbit) enter through :class:`TabulatedOrbits`, which takes the
+arrays of ``add_orbit_information`` (lisaresponse.pyx:41-71) directly or from
+an ``.npz`` file. Any object exposing the same attributes
+(``pycppdetector_args_on`` or ``pycppdetector_args``, ``t_base``, ``LINKS``)
+can be passed to :class:`LISATDIResponse` as well.
 """
 
 from __future__ import annotations
@@ -96,10 +100,78 @@ class EqualArmlengthOrbits:
     def pycppdetector_args_on(self, device):
         """Arguments of ``add_orbit_information`` (lisaresponse.pyx:41-71) with device tables."""
         n, L, x = self.to_device(device)
-        return (self.dt, self.N, n, L, x, np.asarray(LINKS, np.int32), np.asarray(SC_R, np.int32),
-                np.asarray(SC_E, np.int32), self.armlength)
+        return (self.dt, self.N, n, L, x, np.asarray(self.LINKS, np.int32), np.asarray(self.SC_R, np.int32),
+                np.asarray(self.SC_E, np.int32), self.armlength)
 
     def as_host_dict(self):
         """Tables for the CPU oracle."""
-        return dict(dt=self.dt, N=self.N, n_arr=self.n_base, x_arr=self.x_base, links=np.asarray(LINKS, np.int32),
+        return dict(dt=self.dt, N=self.N, n_arr=self.n_base, x_arr=self.x_base, links=np.asarray(self.LINKS, np.int32),
                     armlength=self.armlength)
+
+    def to_arrays(self):
+        """The tables as the keyword arguments of :class:`TabulatedOrbits`."""
+        return dict(dt=self.dt, n_arr=self.n_base, L_arr=self.ltt_base, x_arr=self.x_base,
+                    links=np.asarray(self.LINKS, np.int32), sc_r=np.asarray(self.SC_R, np.int32),
+                    sc_e=np.asarray(self.SC_E, np.int32), armlength=self.armlength)
+
+
+class TabulatedOrbits(EqualArmlengthOrbits):
+    """Orbit tables given as arrays, in the layout of the reference's
+    ``add_orbit_information(dt, N, n_arr, L_arr, x_arr, links, sc_r, sc_e,
+    armlength)`` (lisaresponse.pyx:41-71): link unit vectors n (N, 6, 3), light
+    travel times L (N, 6) [s], spacecraft positions x (N, 3, 3) [m] on the
+    uniform grid t = k dt, k = 0..N-1; ``links`` the six link ids (ij: received
+    by spacecraft i, emitted by j) with their receiver ``sc_r`` and emitter
+    ``sc_e`` spacecraft. The arrays are checked (shapes, the six ids 12, 23,
+    31, 13, 32, 21 once each, sc_r / sc_e consistent with them, unit vectors,
+    dt > 0) and then used as given.
+
+    ``TabulatedOrbits.from_file(path)`` reads an ``.npz`` archive with the keys
+    dt, n, L, x, links, sc_r, sc_e and armlength (lisaanalysistools' HDF5
+    orbit files need h5py, which is not part of this build: export their
+    datasets to such an archive)."""
+
+    def __init__(self, dt, n_arr, L_arr, x_arr, links=LINKS, sc_r=SC_R, sc_e=SC_E, armlength=2.5e9,
+                 force_backend=None):
+        n = np.ascontiguousarray(np.asarray(n_arr, dtype=np.float64))
+        L = np.ascontiguousarray(np.asarray(L_arr, dtype=np.float64))
+        x = np.ascontiguousarray(np.asarray(x_arr, dtype=np.float64))
+        links = [int(v) for v in np.asarray(links).ravel()]
+        sc_r = [int(v) for v in np.asarray(sc_r).ravel()]
+        sc_e = [int(v) for v in np.asarray(sc_e).ravel()]
+        if not (float(dt) > 0.0):
+            raise ValueError("dt must be positive")
+        N = x.shape[0] if x.ndim == 3 else -1
+        if N < 2 or x.shape != (N, 3, 3) or n.shape != (N, 6, 3) or L.shape != (N, 6):
+            raise ValueError(f"orbit arrays must be x (N, 3, 3), n (N, 6, 3), L (N, 6) with N >= 2; got "
+                             f"{x.shape}, {n.shape}, {L.shape}")
+        if sorted(links) != sorted(LINKS) or len(sc_r) != 6 or len(sc_e) != 6:
+            raise ValueError(f"links must hold each of {LINKS} once, with six sc_r and sc_e entries")
+        for l, r, e in zip(links, sc_r, sc_e):
+            if (r, e) != (l // 10, l % 10):
+                raise ValueError(f"link {l} must be received by spacecraft {l // 10} and emitted by {l % 10}")
+        if not np.all(np.isfinite(x)) or not np.all(np.isfinite(n)) or not np.all(np.isfinite(L)):
+            raise ValueError("orbit arrays must be finite")
+        if np.max(np.abs(np.sqrt((n * n).sum(-1)) - 1.0)) > 1e-6:
+            raise ValueError("link vectors n must be unit vectors")
+        if not (float(armlength) > 0.0):
+            raise ValueError("armlength must be positive")
+        self.LINKS, self.SC_R, self.SC_E = links, sc_r, sc_e
+        self.dt = float(dt)
+        self.N = N
+        self.armlength = float(armlength)
+        self.t_base = np.arange(N) * self.dt
+        self.x_base, self.n_base, self.ltt_base = x, n, L
+        self._force_backend = force_backend
+        self._dev = None
+
+    @classmethod
+    def from_file(cls, path, **kwargs):
+        with np.load(path, allow_pickle=False) as z:
+            return cls(float(z["dt"]), z["n"], z["L"], z["x"], links=z["links"], sc_r=z["sc_r"], sc_e=z["sc_e"],
+                       armlength=float(z["armlength"]), **kwargs)
+
+    def save(self, path):
+        """Write the tables to an ``.npz`` archive that ``from_file`` reads."""
+        np.savez(path, dt=self.dt, n=self.n_base, L=self.ltt_base, x=self.x_base, links=np.asarray(self.LINKS),
+                 sc_r=np.asarray(self.SC_R), sc_e=np.asarray(self.SC_E), armlength=self.armlength)

@@ -168,7 +168,14 @@ class OracleWaveform:
             out = direct_sum(buf, B, length, M, t_start, t_end)
             return out.squeeze() if squeeze else out
         if direct:
-            raise NotImplementedError
+            # per-mode direct outputs (waveformbuild.py:603-636): the direct sum
+            # of each mode's slice of the buffer on its own
+            bf = buf.reshape(9, B, M, length)
+            out = np.zeros((B, 3, M, length), dtype=np.complex128)
+            for mi in range(M):
+                out[:, :, mi, :] = direct_sum(np.ascontiguousarray(bf[:, :, mi, :]).ravel(), B, length, 1, t_start,
+                                              t_end)
+            return out.squeeze() if squeeze else out
         # spline + dense interpolation (waveformbuild.py:638-700)
         _, c1, c2, c3 = interpolate(self.freqs, buf, length, 9, M, B)
         data_freqs = np.asarray(freqs, dtype=np.float64)

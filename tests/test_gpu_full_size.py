@@ -70,8 +70,8 @@ def test_heterodyned_all_templates_vs_oracle_full_size(het_full):
     and the PhenomHM phases reach 1e4-1e6 rad, so FP64 rounding alone moves
     logL by about 1e-6: the oracle built with FMA contraction (the same sources,
     a different rounding) differs from itself by up to ~1.5e-6. The test also
-    holds the HIP paths to that floor (at most 3x the oracle's own FMA spread)
-    and asks for 99 % of templates inside the plain 1e-6."""
+    holds the HIP paths to that floor: their largest and 99th-percentile
+    differences are at most 3x those of the oracle's own FMA spread."""
     import oracle
     from bbhx_amd.utils.sensitivity import AET1SensitivityMatrix
     from oracle.pipeline import OracleHeterodyne, OracleWaveform
@@ -90,14 +90,16 @@ def test_heterodyned_all_templates_vs_oracle_full_size(het_full):
                                     modes=MODES)
             lls.append(ohet.get_ll(p)[0])
     ll_o, ll_fma = lls
-    floor = float(np.max(np.abs(ll_fma - ll_o)))
+    spread = np.abs(ll_fma - ll_o)
     for name, ll in (("fused", ll_fused), ("kernel-by-kernel", ll_plain)):
         d = np.abs(ll - ll_o)
-        print(f"{name}: max |dlogL| {d.max():.3e}, {np.sum(d > 1e-6)} of {d.size} above 1e-6; oracle FMA spread "
-              f"{floor:.3e}; |logL| in [{np.abs(ll_o).min():.3e}, {np.abs(ll_o).max():.3e}]", flush=True)
+        print(f"{name}: max |dlogL| {d.max():.3e}, 99th percentile {np.quantile(d, 0.99):.3e}, {np.sum(d > 1e-6)} "
+              f"of {d.size} above 1e-6; oracle FMA spread: max {spread.max():.3e}, 99th percentile "
+              f"{np.quantile(spread, 0.99):.3e}, {np.sum(spread > 1e-6)} above 1e-6; |logL| in "
+              f"[{np.abs(ll_o).min():.3e}, {np.abs(ll_o).max():.3e}]", flush=True)
         assert np.all(d <= 1e-6 + 1e-12 * np.abs(ll_o)), name
-        assert d.max() <= 3.0 * max(floor, 1e-7), name
-        assert np.mean(d <= 1e-6) >= 0.99, name
+        assert d.max() <= 3.0 * max(spread.max(), 1e-7), name
+        assert np.quantile(d, 0.99) <= 3.0 * max(np.quantile(spread, 0.99), 1e-7), name
 
 
 def test_heterodyned_reference_identity_full_size(het_full):

@@ -157,6 +157,30 @@ def test_response(backend, orbits, TDItag, rescaled, tdi2):
     assert rel_err(g[1], o[1]) < 1e-11
 
 
+def test_response_with_orbit_tables_from_a_file(backend, orbits, tmp_path):
+    """Orbits given as arrays in the add_orbit_information layout
+    (lisaresponse.pyx:41-71), written to and read back from a file: the
+    response equals the one from the analytic tables bit for bit, and the
+    oracle with the same tables. A different link order in the file (the
+    response finds links 12, 23, 31 by id) gives the same result too."""
+    from bbhx_amd.utils.orbits import TabulatedOrbits
+
+    kw = orbits.to_arrays()
+    TabulatedOrbits(**kw).save(tmp_path / "orbits.npz")
+    tab = TabulatedOrbits.from_file(tmp_path / "orbits.npz")
+    order = [5, 3, 0, 4, 1, 2]
+    kw2 = dict(kw, n_arr=kw["n_arr"][:, order], L_arr=kw["L_arr"][:, order], links=kw["links"][order],
+               sc_r=kw["sc_r"][order], sc_e=kw["sc_e"][order])
+    shuffled = TabulatedOrbits(**kw2)
+    g0, o0 = _response_case(backend, orbits, "AET", False, True)
+    g1, o1 = _response_case(backend, tab, "AET", False, True)
+    g2, _ = _response_case(backend, shuffled, "AET", False, True)
+    np.testing.assert_array_equal(g1, g0)
+    np.testing.assert_array_equal(g2, g0)
+    for k in range(3, 9):
+        assert rel_err(g1[k], o1[k]) < 1e-10, k
+
+
 def test_spline(backend):
     import torch
     from scipy.interpolate import CubicSpline
@@ -399,6 +423,41 @@ def test_full_waveform(backend, orbits, small_problem):
     o = sp["owave"](*p, freqs=sp["f"], length=1024, modes=MODES, direct=False, fill=True)
     assert np.all(np.isfinite(g))
     assert rel_err(g, o, axis=None) < 1e-8
+
+
+def test_output_format_combine(backend, orbits, small_problem):
+    """fill + combine (waveformbuild.py:663-677): all templates added into one
+    data stream, against the oracle's combined stream and against the sum of
+    the per-template (fill) streams."""
+    from bbhx_amd.waveformbuild import BBHWaveformFD
+
+    sp = small_problem
+    p = perturb(sp["ref"], 4, 23, scale=1e-4)
+    wave = BBHWaveformFD(response_kwargs=dict(orbits=orbits), force_backend=backend)
+    g = host(wave(*p, freqs=sp["f"], length=1024, modes=MODES, direct=False, fill=True, combine=True))
+    o = sp["owave"](*p, freqs=sp["f"], length=1024, modes=MODES, direct=False, fill=True, combine=True)
+    sep = host(wave(*p, freqs=sp["f"], length=1024, modes=MODES, direct=False, fill=True))
+    assert g.shape == (3, sp["f"].size)
+    assert rel_err(g, o, axis=None) < 1e-8
+    np.testing.assert_allclose(g, sep.sum(axis=0), rtol=0, atol=1e-13 * np.abs(g).max())
+
+
+def test_output_format_direct_per_mode(backend, orbits, small_problem):
+    """direct, not compressed (waveformbuild.py:603-636): (B, 3, M, L) per-mode
+    outputs on the given grid, against the oracle; their sum over modes is the
+    compressed (direct_sum) output."""
+    from bbhx_amd.waveformbuild import BBHWaveformFD
+
+    sp = small_problem
+    p = perturb(sp["ref"], 3, 24, scale=1e-4)
+    fr = np.logspace(-4.3, -2.0, 400)
+    wave = BBHWaveformFD(response_kwargs=dict(orbits=orbits), force_backend=backend)
+    g = host(wave(*p, freqs=fr, modes=MODES, direct=True, compress=False))
+    o = sp["owave"](*p, freqs=fr, modes=MODES, direct=True, compress=False)
+    c = host(wave(*p, freqs=fr, modes=MODES, direct=True, compress=True))
+    assert g.shape == (3, 3, len(MODES), fr.size)
+    assert rel_err(g, o, axis=None) < 1e-9
+    np.testing.assert_allclose(g.sum(axis=2), c, rtol=0, atol=1e-13 * np.abs(c).max())
 
 
 def test_direct_likelihood(backend, orbits, small_problem):
