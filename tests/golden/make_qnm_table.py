@@ -7,7 +7,7 @@ the same numbers as the reference's ``waveforms/ringdownphenomd.py:3-3020``;
 this script reads that file as text and writes the three columns. It is run
 once, by hand, where the reference is available:
 
-    python bbhx_amd/waveforms/make_qnm_table.py /root/reference/src/bbhx/waveforms/ringdownphenomd.py
+    python tests/golden/make_qnm_table.py /root/reference/src/bbhx/waveforms/ringdownphenomd.py
 """
 
 import re
@@ -30,7 +30,7 @@ def parse_table(path):
 def main(src):
     spin_step, a, fring, fdamp = parse_table(src)
     assert a.shape == fring.shape == fdamp.shape == (1001,)
-    dst = Path(__file__).with_name("qnm_phenomd.txt")
+    dst = Path(__file__).resolve().parents[2] / "bbhx_amd" / "waveforms" / "qnm_phenomd.txt"
     header = f"PhenomD QNM table, dspin={spin_step!r}; columns: final spin, f_ring, f_damp (units of 1/M_final)"
     np.savetxt(dst, np.column_stack([a, fring, fdamp]), fmt="%.17g", header=header)
     print(f"wrote {dst} ({a.size} rows)")
