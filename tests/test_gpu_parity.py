@@ -226,6 +226,30 @@ def test_spline_chunked_launches(backend):
             assert rel_err(gc[p, b, 0, :-1], oc[:-1]) < 1e-9, (p, b)
 
 
+def test_spline_one_lane_per_system_path(backend):
+    """Batches of at least 2048 systems take the one-lane-per-system solver
+    (smaller ones the partitioned one-wavefront-per-system solver, which the
+    tests above exercise): 4096 binaries x 9 quantities of 40 knots against the
+    oracle on systems at the start, middle and end, and scipy on one."""
+    import torch
+    from scipy.interpolate import CubicSpline
+
+    from bbhx_amd.utils.interpolate import CubicSplineInterpolant
+
+    rng = np.random.default_rng(6)
+    B, M, P, L = 4096, 1, 9, 40
+    x = np.sort(np.exp(rng.uniform(np.log(1e-4), np.log(1e-1), (B, M, L))), axis=-1)
+    y = np.sin(40 * x[None] * (1 + np.arange(P)[:, None, None, None])) + 0.1 * rng.standard_normal((P, B, M, L))
+    spl = CubicSplineInterpolant(dev(x), dev(y), force_backend=backend)
+    g = [host(c).reshape(P, B, M, L) for c in (spl.c1, spl.c2, spl.c3)]
+    for p, b in ((0, 0), (4, 2047), (8, 4095)):
+        _, c1o, c2o, c3o = oracle.interpolate(x[b].ravel(), y[p, b].ravel(), L, 1, 1, 1)
+        for gc, oc in zip(g, (c1o, c2o, c3o)):
+            assert rel_err(gc[p, b, 0, :-1], oc[:-1]) < 1e-9, (p, b)
+    cs = CubicSpline(x[100, 0], y[3, 100, 0], bc_type="not-a-knot")
+    np.testing.assert_allclose(g[0][3, 100, 0, :-1], cs.c[2], rtol=1e-8, atol=1e-8 * np.abs(cs.c[2]).max())
+
+
 def test_direct_sum_and_hdyn(backend):
     import torch
 
