@@ -75,6 +75,8 @@ struct bbhx_direct_plan {
     // FP32-bulk form (bbhx_hip_direct_plan_set_precision)
     int precision = 0;                   // 0: FP64, 1: FP32 bulk, FP64 phase and t(f)
     void* CfF = nullptr;                 // (S, M) 192-B segment records of that form
+    int* loud = nullptr;                 // (max_items,) work items of the binaries recomputed in FP64
+    int* loud_cnt = nullptr;             // (2,) their count, and the FP64 pass's next-item counter
     int max_chunks = 0;           // absolute chunks of min_chunk bins
 };
 
@@ -1015,6 +1017,34 @@ __global__ __launch_bounds__(64) void k_dir_reduce(double* __restrict__ d_h, dou
     }
 }
 
+// ---------------------------------------------------------------- FP32-bulk fallback
+//
+// The FP32-bulk form's error is at most ~1.3e-7 h_h per template (measured on
+// the uneven and the large batches), so templates with h_h above
+// FP32_FALLBACK_HH (error bound ~4e-4 in logL there, 2.5x inside the 1e-3 target)
+// are evaluated again with the FP64 records and kernel, their work items only.
+
+constexpr double FP32_FALLBACK_HH = 3000.0;
+
+__global__ void k_dir_loud_zero(int* __restrict__ loud_cnt)
+{
+    if (threadIdx.x < 2) loud_cnt[threadIdx.x] = 0;
+}
+
+// one lane per entry of the chunk-major order: the items of loud binaries,
+// appended in any order (each item writes its own partial, so the order does
+// not change the result)
+__global__ void k_dir_loud_list(int* __restrict__ loud, int* __restrict__ loud_cnt, const int* __restrict__ perm,
+                                const int* __restrict__ total, const int* __restrict__ band, int B,
+                                const double* __restrict__ h_h)
+{
+    const int pi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pi >= total[0]) return;
+    const int it = perm[pi];
+    const int b = owner(band, B, it);
+    if (h_h[2 * b] > FP32_FALLBACK_HH) loud[atomicAdd(loud_cnt, 1)] = it;
+}
+
 // ---------------------------------------------------------------- gradient kernels
 
 __global__ void k_dirg_zero(double* __restrict__ y, long n)
@@ -1199,7 +1229,8 @@ T* dalloc(size_t n, bool& ok)
 void plan_free(bbhx_direct_plan* p)
 {
     void* bufs[] = {p->off, p->len, p->recs, p->X, p->Cf, p->Yc, p->grid_lo, p->grid_istep, p->band,
-                    p->total, p->partial, p->perm, p->cnt, p->next, p->Yadj, p->grecs, p->gpart, p->CfF};
+                    p->total, p->partial, p->perm, p->cnt, p->next, p->Yadj, p->grecs, p->gpart, p->CfF,
+                    p->loud, p->loud_cnt};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
 }
@@ -1284,7 +1315,7 @@ extern "C" long bbhx_hip_direct_plan_bytes(const bbhx_direct_plan* p)
     long bytes = SM * 8 * (1 + RS + NQ) + (long)p->B * p->M * (sizeof(FusedRec) + 16) + p->max_items * 36 + p->B * 24 +
                  4L * (p->max_chunks + 1);
     if (p->Yadj) bytes += SM * 8 * NA + (long)p->B * p->M * (sizeof(bbhx::gradk::GradRec) + 8 * bbhx::gradk::NGRAD);
-    if (p->CfF) bytes += SM * (long)sizeof(SegRecF);
+    if (p->CfF) bytes += SM * (long)sizeof(SegRecF) + p->max_items * 4 + 8;
     return bytes;
 }
 
@@ -1379,6 +1410,18 @@ int direct_launch(bbhx_direct_plan* p, double* d_h, double* h_h, double* grad, i
                            reinterpret_cast<const cd*>(dataChannels), noiseFactors, p->N, B, M, chunk);
     hipLaunchKernelGGL(k_dir_reduce, dim3(B), dim3(64), 0, st, d_h, h_h, p->partial, p->band,
                        p->total, B);
+    if (f32) {
+        // loud templates again in FP64 (their items' partials overwritten), then every
+        // binary reduced again (the others' partials are unchanged)
+        hipLaunchKernelGGL(k_dir_loud_zero, dim3(1), dim3(64), 0, st, p->loud_cnt);
+        hipLaunchKernelGGL(k_dir_loud_list, dim3(item_blocks), dim3(256), 0, st, p->loud, p->loud_cnt, p->perm,
+                           p->total, p->band, B, h_h);
+        hipLaunchKernelGGL((k_dir_like_t<false, SegRec>), dim3(grid), dim3(block), 0, st, p->partial, nullptr,
+                           p->band, p->loud_cnt, p->loud, p->loud_cnt + 1, reinterpret_cast<const SegRec*>(p->Cf),
+                           p->grid_lo, p->grid_istep, p->off, p->len, t_start, t_end, dataFreqs,
+                           reinterpret_cast<const cd*>(dataChannels), noiseFactors, p->N, B, M, chunk);
+        hipLaunchKernelGGL(k_dir_reduce, dim3(B), dim3(64), 0, st, d_h, h_h, p->partial, p->band, p->total, B);
+    }
     if (grad) {
         hipLaunchKernelGGL(k_dirg_spline_adj, dim3(B), dim3(64), 0, st, p->Yadj, p->X, p->off, p->len, M);
         hipLaunchKernelGGL(k_dirg_knots, dim3(B * M), dim3(64), 0, st, p->gpart, p->grecs, p->Yadj, p->X, p->off,
@@ -1411,7 +1454,16 @@ extern "C" int bbhx_hip_direct_plan_set_precision(bbhx_direct_plan* p, int preci
     if (precision == 1 && !p->CfF) {
         bool ok = true;
         p->CfF = dalloc<SegRecF>((size_t)p->S * p->M, ok);
-        if (!ok) return ERR_ALLOC;
+        p->loud = dalloc<int>((size_t)p->max_items, ok);
+        p->loud_cnt = dalloc<int>(2, ok);
+        if (!ok) {
+            for (void* b : {p->CfF, (void*)p->loud, (void*)p->loud_cnt})
+                if (b) (void)hipFree(b);
+            p->CfF = nullptr;
+            p->loud = nullptr;
+            p->loud_cnt = nullptr;
+            return ERR_ALLOC;
+        }
     }
     p->precision = precision;
     return OK;

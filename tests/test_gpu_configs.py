@@ -8,8 +8,9 @@ oracle.
 * Uneven batching: the 8192-template ragged batch (total mass 1e4-1e8,
   sparse lengths 256-4096 from their observable bands) on 1 yr at 10 s
   (1,577,907 bins), fused: the oracle on 32 templates, among them the
-  shortest and the longest grids; on all 8192, finite values and the
-  Cauchy-Schwarz bound |d_h| <= sqrt(d_d h_h) on each template's band.
+  shortest and the longest grids; on all 8192, finite values, the
+  Cauchy-Schwarz bound |d_h| <= sqrt(d_d h_h) on each template's band, and the
+  FP32-bulk form within |dlogL| < 1e-3 of the FP64 form.
 * Gradient, 1024 templates around the reference, K = 256, 2^20 bins: the
   adjoint gradient against fourth-order central differences of the oracle's
   heterodyned logL on 8 templates (1e-5 relative, plus the differences' own
@@ -95,6 +96,16 @@ def test_uneven_batch_8192_templates():
     assert np.all(h_h.real >= 0.0) and np.all(np.abs(h_h.imag) <= 1e-9 * np.maximum(h_h.real, 1.0))
     # Cauchy-Schwarz on each band: the band's <d|d> is at most the full d_d
     assert np.all(np.abs(d_h) <= np.sqrt(like.d_d * h_h.real) * (1 + 1e-12) + 1e-9)
+    # the FP32-bulk form of the same batch: within 1e-3 of the FP64 form on every template
+    plan = like.direct_plan(lengths, MODES)
+    ll64 = like.get_ll_fused(p, lengths, modes=MODES, plan=plan)
+    ll32 = like.get_ll_fused(p, lengths, modes=MODES, plan=plan, precision=1)
+    plan.set_precision(0)
+    d32 = np.abs(ll32 - ll64)
+    print(f"FP32 bulk vs FP64: max |dlogL| {d32.max():.3e}, median {np.median(d32):.3e}, "
+          f"{int(np.sum(h_h.real > 3000.0))} templates recomputed in FP64", flush=True)
+    assert np.all(d32 < 1e-3)
+    del plan
     # oracle on 32 templates: the shortest and longest grids and 30 seeded others
     order = np.argsort(lengths, kind="stable")
     pick = np.unique(np.concatenate([order[:2], order[-2:], np.random.default_rng(7).choice(8192, 28, False)]))
