@@ -75,8 +75,11 @@ struct bbhx_direct_plan {
     // FP32-bulk form (bbhx_hip_direct_plan_set_precision)
     int precision = 0;                   // 0: FP64, 1: FP32 bulk, FP64 phase and t(f)
     void* CfF = nullptr;                 // (S, M) 192-B segment records of that form
-    int* loud = nullptr;                 // (max_items,) work items of the binaries recomputed in FP64
-    int* loud_cnt = nullptr;             // (2,) their count, and the FP64 pass's next-item counter
+    int* loud = nullptr;                 // (max_items,) work items of the binaries evaluated in FP64
+    int* quiet = nullptr;                // (max_items,) work items of the binaries evaluated in FP32
+    int* loud_cnt = nullptr;             // (4,) loud count, its pass's next-item counter; the same for quiet
+    int* route = nullptr;                // (B,) 1: the h_h estimate sends the binary to the FP64 pass
+    int* split_cnt = nullptr;            // (max_items / 256 + 1,) loud entries per block of the order, scanned
     int max_chunks = 0;           // absolute chunks of min_chunk bins
 };
 
@@ -1017,32 +1020,142 @@ __global__ __launch_bounds__(64) void k_dir_reduce(double* __restrict__ d_h, dou
     }
 }
 
-// ---------------------------------------------------------------- FP32-bulk fallback
+// ---------------------------------------------------------------- FP32-bulk routing
 //
 // The FP32-bulk form's error is at most ~1.3e-7 h_h per template (measured on
 // the uneven and the large batches), so templates with h_h above
-// FP32_FALLBACK_HH (error bound ~4e-4 in logL there, 2.5x inside the 1e-3 target)
-// are evaluated again with the FP64 records and kernel, their work items only.
+// FP32_FALLBACK_HH (error bound ~4e-4 in logL there, 2.5x inside the 1e-3
+// target) are evaluated with the FP64 records and kernel. Which ones is first
+// decided from an estimate of h_h made before the dense pass (k_dir_route);
+// after it, any binary the estimate sent to FP32 whose h_h exceeds the bound
+// is evaluated again in FP64 (its work items only), so the estimate decides
+// the cost and never the result.
 
 constexpr double FP32_FALLBACK_HH = 3000.0;
 
 __global__ void k_dir_loud_zero(int* __restrict__ loud_cnt)
 {
-    if (threadIdx.x < 2) loud_cnt[threadIdx.x] = 0;
+    if (threadIdx.x < 4) loud_cnt[threadIdx.x] = 0;
 }
 
-// one lane per entry of the chunk-major order: the items of loud binaries,
-// appended in any order (each item writes its own partial, so the order does
-// not change the result)
+// h_h of each binary estimated from its sparse knots (one wavefront per
+// binary), each mode on its own grid and the modes added without their
+// interference:
+//   4 sum_m sum_c sum_i A_mi^2 |T_mci|^2 w_c(x_i)^2 (x_{i+1} - x_i) / df
+// over knots inside the data band and the observation window, w taken at the
+// nearest bin of a uniform dense grid (for another grid the estimate, and so
+// only the split between the passes, is coarser).
+__global__ __launch_bounds__(64) void k_dir_route(int* __restrict__ route, const double* __restrict__ X,
+                                                  const double* __restrict__ Yc, const long* __restrict__ off,
+                                                  const int* __restrict__ len, const double* __restrict__ t_start,
+                                                  const double* __restrict__ t_end, const double* __restrict__ freqs,
+                                                  const double* __restrict__ wts, int N, int M)
+{
+    const int b = blockIdx.x, lane = threadIdx.x;
+    const int L = len[b];
+    const long base = off[b];
+    const double ts = t_start[b], te = t_end[b];
+    const double f0 = freqs[0], f1 = freqs[N - 1];
+    const double idf = f1 > f0 ? (N - 1) / (f1 - f0) : 0.0;
+    double s = 0.0;
+    for (int m = 0; m < M; m++) {
+        for (int i = lane; i + 1 < L; i += 64) {
+            const long p = (base + i) * M + m;
+            const double x = X[p], xn = X[p + M];
+            if (!(x >= f0 && x <= f1 && xn > x)) continue;
+            const double* y = Yc + p * NQ;
+            const double tf = y[2];
+            if (!((tf >= ts) && ((tf <= te) || (te <= 0.0)))) continue;
+            const int k = min(N - 1, max(0, (int)((x - f0) * idf + 0.5)));
+            double t = 0.0;
+            for (int ch = 0; ch < 3; ch++) {
+                const double w = wts[(long)ch * N + k];
+                t += (y[3 + 2 * ch] * y[3 + 2 * ch] + y[4 + 2 * ch] * y[4 + 2 * ch]) * (w * w);
+            }
+            s += (y[0] * y[0]) * t * ((fmin(xn, f1) - x) * idf);
+        }
+    }
+    s = wsum(s);
+    if (lane == 0) route[b] = 4.0 * s > FP32_FALLBACK_HH ? 1 : 0;
+}
+
+// The chunk-major order split into the items of binaries routed to FP64
+// (`loud`) and the others (`quiet`), each list keeping the order (the items in
+// flight then share their dense chunks, whose data come from L2): a count of
+// loud entries per block of SPLIT_BLOCK entries, its exclusive scan
+// (k_dir_cnt_scan), and the placement of every entry.
+constexpr int SPLIT_BLOCK = 256;
+
+__device__ inline bool routed_loud(int pi, int n, const int* __restrict__ perm, const int* __restrict__ band, int B,
+                                   const int* __restrict__ route, int& it)
+{
+    it = pi < n ? perm[pi] : 0;
+    return pi < n && route[owner(band, B, it)] != 0;
+}
+
+__global__ __launch_bounds__(SPLIT_BLOCK) void k_dir_split_count(int* __restrict__ split_cnt,
+                                                                 const int* __restrict__ perm,
+                                                                 const int* __restrict__ total,
+                                                                 const int* __restrict__ band, int B,
+                                                                 const int* __restrict__ route)
+{
+    __shared__ int wc[SPLIT_BLOCK / 64];
+    int it;
+    const bool l = routed_loud(blockIdx.x * SPLIT_BLOCK + threadIdx.x, total[0], perm, band, B, route, it);
+    const unsigned long long m = __ballot(l);
+    if ((threadIdx.x & 63) == 0) wc[threadIdx.x >> 6] = __popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int c = 0;
+        for (int w = 0; w < SPLIT_BLOCK / 64; w++) c += wc[w];
+        split_cnt[blockIdx.x] = c;
+    }
+}
+
+// after the scan: split_cnt[blk] = loud entries before block blk. Entry pi goes
+// to loud[#loud before pi] or quiet[pi - #loud before pi]; the last entry
+// writes both counts (cnt[0] loud, cnt[2] quiet; zeroed before, for an empty order)
+__global__ __launch_bounds__(SPLIT_BLOCK) void k_dir_split_write(int* __restrict__ loud, int* __restrict__ quiet,
+                                                                 int* __restrict__ cnt,
+                                                                 const int* __restrict__ split_cnt,
+                                                                 const int* __restrict__ perm,
+                                                                 const int* __restrict__ total,
+                                                                 const int* __restrict__ band, int B,
+                                                                 const int* __restrict__ route)
+{
+    __shared__ int wc[SPLIT_BLOCK / 64];
+    const int n = total[0];
+    const int pi = blockIdx.x * SPLIT_BLOCK + threadIdx.x;
+    int it;
+    const bool l = routed_loud(pi, n, perm, band, B, route, it);
+    const unsigned long long m = __ballot(l);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) wc[w] = __popcll(m);
+    __syncthreads();
+    int before = split_cnt[blockIdx.x];
+    for (int k = 0; k < w; k++) before += wc[k];
+    const int lpos = before + __popcll(m & ((1ull << lane) - 1ull));
+    if (pi < n) {
+        if (l) loud[lpos] = it;
+        else quiet[pi - lpos] = it;
+        if (pi == n - 1) {
+            cnt[0] = lpos + (l ? 1 : 0);
+            cnt[2] = n - cnt[0];
+        }
+    }
+}
+
+// after the passes: the items of binaries evaluated in FP32 whose h_h exceeds
+// the bound
 __global__ void k_dir_loud_list(int* __restrict__ loud, int* __restrict__ loud_cnt, const int* __restrict__ perm,
                                 const int* __restrict__ total, const int* __restrict__ band, int B,
-                                const double* __restrict__ h_h)
+                                const double* __restrict__ h_h, const int* __restrict__ route)
 {
     const int pi = blockIdx.x * blockDim.x + threadIdx.x;
     if (pi >= total[0]) return;
     const int it = perm[pi];
     const int b = owner(band, B, it);
-    if (h_h[2 * b] > FP32_FALLBACK_HH) loud[atomicAdd(loud_cnt, 1)] = it;
+    if (!route[b] && h_h[2 * b] > FP32_FALLBACK_HH) loud[atomicAdd(loud_cnt, 1)] = it;
 }
 
 // ---------------------------------------------------------------- gradient kernels
@@ -1230,7 +1343,7 @@ void plan_free(bbhx_direct_plan* p)
 {
     void* bufs[] = {p->off, p->len, p->recs, p->X, p->Cf, p->Yc, p->grid_lo, p->grid_istep, p->band,
                     p->total, p->partial, p->perm, p->cnt, p->next, p->Yadj, p->grecs, p->gpart, p->CfF,
-                    p->loud, p->loud_cnt};
+                    p->loud, p->quiet, p->loud_cnt, p->route, p->split_cnt};
     for (void* b : bufs)
         if (b) (void)hipFree(b);
 }
@@ -1315,7 +1428,7 @@ extern "C" long bbhx_hip_direct_plan_bytes(const bbhx_direct_plan* p)
     long bytes = SM * 8 * (1 + RS + NQ) + (long)p->B * p->M * (sizeof(FusedRec) + 16) + p->max_items * 36 + p->B * 24 +
                  4L * (p->max_chunks + 1);
     if (p->Yadj) bytes += SM * 8 * NA + (long)p->B * p->M * (sizeof(bbhx::gradk::GradRec) + 8 * bbhx::gradk::NGRAD);
-    if (p->CfF) bytes += SM * (long)sizeof(SegRecF) + p->max_items * 4 + 8;
+    if (p->CfF) bytes += SM * (long)sizeof(SegRecF) + p->max_items * 8 + 16 + p->B * 4L + 4L * (p->max_items / 256 + 1);
     return bytes;
 }
 
@@ -1394,16 +1507,31 @@ int direct_launch(bbhx_direct_plan* p, double* d_h, double* h_h, double* grad, i
                            p->perm, p->next, reinterpret_cast<const SegRec*>(p->Cf), p->grid_lo, p->grid_istep,
                            p->off, p->len, t_start, t_end, dataFreqs, reinterpret_cast<const cd*>(dataChannels),
                            noiseFactors, p->N, B, M, chunk);
-    else if (f32)
-        // the FP32-bulk kernel holds 6 wavefronts per SIMD against the FP64 form's 4, so
-        // a grid sized for the FP64 form is scaled by 3/2 to fill the chip as fully
-        // (measured at 8192 uneven templates: grid 1024 as given 253 ms, 2048 203 ms)
+    else if (f32) {
+        // binaries whose h_h estimate exceeds the bound go to the FP64 pass, the
+        // others to the FP32 pass; the FP32-bulk kernel holds 6 wavefronts per SIMD
+        // against the FP64 form's 4, so a grid sized for the FP64 form is scaled by
+        // 3/2 to fill the chip as fully (measured at 8192 uneven templates, all in
+        // FP32: grid 1024 as given 253 ms, 2048 203 ms)
+        hipLaunchKernelGGL(k_dir_route, dim3(B), dim3(64), 0, st, p->route, p->X, p->Yc, p->off, p->len, t_start,
+                           t_end, dataFreqs, noiseFactors, p->N, M);
+        hipLaunchKernelGGL(k_dir_loud_zero, dim3(1), dim3(64), 0, st, p->loud_cnt);
+        const int split_blocks = ceil_div(p->max_items, SPLIT_BLOCK);  // <= max_items / 256 + 1
+        hipLaunchKernelGGL(k_dir_split_count, dim3(split_blocks), dim3(SPLIT_BLOCK), 0, st, p->split_cnt, p->perm,
+                           p->total, p->band, B, p->route);
+        hipLaunchKernelGGL(k_dir_cnt_scan, dim3(1), dim3(1024), 0, st, p->split_cnt, split_blocks);
+        hipLaunchKernelGGL(k_dir_split_write, dim3(split_blocks), dim3(SPLIT_BLOCK), 0, st, p->loud, p->quiet,
+                           p->loud_cnt, p->split_cnt, p->perm, p->total, p->band, B, p->route);
         hipLaunchKernelGGL((k_dir_like_t<false, SegRecF>), dim3((unsigned)std::min<long>(3L * grid / 2, 2147483647L)),
                            dim3(block), 0, st, p->partial, nullptr,
-                           p->band, p->total, p->perm, p->next, static_cast<const SegRecF*>(p->CfF), p->grid_lo,
-                           p->grid_istep, p->off, p->len, t_start, t_end, dataFreqs,
+                           p->band, p->loud_cnt + 2, p->quiet, p->loud_cnt + 3, static_cast<const SegRecF*>(p->CfF),
+                           p->grid_lo, p->grid_istep, p->off, p->len, t_start, t_end, dataFreqs,
                            reinterpret_cast<const cd*>(dataChannels), noiseFactors, p->N, B, M, chunk);
-    else
+        hipLaunchKernelGGL((k_dir_like_t<false, SegRec>), dim3(grid), dim3(block), 0, st, p->partial, nullptr,
+                           p->band, p->loud_cnt, p->loud, p->loud_cnt + 1, reinterpret_cast<const SegRec*>(p->Cf),
+                           p->grid_lo, p->grid_istep, p->off, p->len, t_start, t_end, dataFreqs,
+                           reinterpret_cast<const cd*>(dataChannels), noiseFactors, p->N, B, M, chunk);
+    } else
         hipLaunchKernelGGL((k_dir_like_t<false, SegRec>), dim3(grid), dim3(block), 0, st, p->partial, nullptr,
                            p->band, p->total, p->perm, p->next, reinterpret_cast<const SegRec*>(p->Cf), p->grid_lo,
                            p->grid_istep, p->off, p->len, t_start, t_end, dataFreqs,
@@ -1411,11 +1539,12 @@ int direct_launch(bbhx_direct_plan* p, double* d_h, double* h_h, double* grad, i
     hipLaunchKernelGGL(k_dir_reduce, dim3(B), dim3(64), 0, st, d_h, h_h, p->partial, p->band,
                        p->total, B);
     if (f32) {
-        // loud templates again in FP64 (their items' partials overwritten), then every
-        // binary reduced again (the others' partials are unchanged)
+        // binaries of the FP32 pass whose h_h exceeds the bound again in FP64 (their
+        // items' partials overwritten), then every binary reduced again (the others'
+        // partials are unchanged)
         hipLaunchKernelGGL(k_dir_loud_zero, dim3(1), dim3(64), 0, st, p->loud_cnt);
         hipLaunchKernelGGL(k_dir_loud_list, dim3(item_blocks), dim3(256), 0, st, p->loud, p->loud_cnt, p->perm,
-                           p->total, p->band, B, h_h);
+                           p->total, p->band, B, h_h, p->route);
         hipLaunchKernelGGL((k_dir_like_t<false, SegRec>), dim3(grid), dim3(block), 0, st, p->partial, nullptr,
                            p->band, p->loud_cnt, p->loud, p->loud_cnt + 1, reinterpret_cast<const SegRec*>(p->Cf),
                            p->grid_lo, p->grid_istep, p->off, p->len, t_start, t_end, dataFreqs,
@@ -1455,13 +1584,20 @@ extern "C" int bbhx_hip_direct_plan_set_precision(bbhx_direct_plan* p, int preci
         bool ok = true;
         p->CfF = dalloc<SegRecF>((size_t)p->S * p->M, ok);
         p->loud = dalloc<int>((size_t)p->max_items, ok);
-        p->loud_cnt = dalloc<int>(2, ok);
+        p->quiet = dalloc<int>((size_t)p->max_items, ok);
+        p->loud_cnt = dalloc<int>(4, ok);
+        p->route = dalloc<int>((size_t)p->B, ok);
+        p->split_cnt = dalloc<int>((size_t)(p->max_items / 256 + 1), ok);
         if (!ok) {
-            for (void* b : {p->CfF, (void*)p->loud, (void*)p->loud_cnt})
+            for (void* b : {p->CfF, (void*)p->loud, (void*)p->quiet, (void*)p->loud_cnt, (void*)p->route,
+                            (void*)p->split_cnt})
                 if (b) (void)hipFree(b);
             p->CfF = nullptr;
             p->loud = nullptr;
+            p->quiet = nullptr;
             p->loud_cnt = nullptr;
+            p->route = nullptr;
+            p->split_cnt = nullptr;
             return ERR_ALLOC;
         }
     }
