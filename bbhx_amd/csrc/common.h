@@ -99,6 +99,42 @@ __device__ inline double root6_pos(double x)
     return y * fma(5.0, y6, 7.0 * x) * rcp_pos(fma(7.0, y6, 5.0 * x));
 }
 
+// Natural logarithm of a finite normal x > 0. x = 2^e m with m in
+// [sqrt(1/2), sqrt(2)); log m = 2 atanh(s), s = (m - 1) / (m + 1), |s| <= 0.1716,
+// from the atanh Taylor series 2 (s + s^3/3 + ... + s^21/21) (truncation below
+// 3e-18 relative: s^22 / 23 < 2e-19); log x = e ln2 + log m with ln2 in two
+// parts. m - 1 is exact (Sterbenz), s is corrected once after the reciprocal,
+// and the result is within ~1 ulp of the correctly rounded log.
+__device__ inline double log_pos(double x)
+{
+    double m = __builtin_amdgcn_frexp_mant(x);  // [0.5, 1)
+    int e = __builtin_amdgcn_frexp_exp(x);
+    if (m < 0.70710678118654752) {
+        m = m + m;
+        e -= 1;
+    }
+    const double num = m - 1.0, den = m + 1.0;
+    const double r = rcp_pos(den);
+    double s = num * r;
+    s = fma(fma(-s, den, num), r, s);
+    const double t = s * s;
+    double q = 1.0 / 21.0;
+    q = fma(q, t, 1.0 / 19.0);
+    q = fma(q, t, 1.0 / 17.0);
+    q = fma(q, t, 1.0 / 15.0);
+    q = fma(q, t, 1.0 / 13.0);
+    q = fma(q, t, 1.0 / 11.0);
+    q = fma(q, t, 1.0 / 9.0);
+    q = fma(q, t, 1.0 / 7.0);
+    q = fma(q, t, 1.0 / 5.0);
+    q = fma(q, t, 1.0 / 3.0);
+    const double s2 = s + s;
+    const double lm = fma(s2 * t, q, s2);
+    const double ed = (double)e;
+    const double LN2_HI = 0.69314718055994529, LN2_LO = 2.3190468138462996e-17;
+    return fma(ed, LN2_HI, fma(ed, LN2_LO, lm));
+}
+
 // A polynomial coefficient as a scalar-register operand at its point of use
 // (rebuilt there by two scalar moves): the FMA chains below then take it as
 // their addend directly, instead of copying a vector-register copy of the

@@ -271,6 +271,11 @@ __device__ inline Dual<N> log(const Dual<N>& x)
     return chain(::log(x.v), rcp_pos(x.v), x);
 }
 template <int N>
+__device__ inline Dual<N> log_pos(const Dual<N>& x)
+{
+    return chain(log_pos(x.v), rcp_pos(x.v), x);
+}
+template <int N>
 __device__ inline Dual<N> exp(const Dual<N>& x)
 {
     const double e = ::exp(x.v);

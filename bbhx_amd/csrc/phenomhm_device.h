@@ -137,8 +137,9 @@ template <class S>
 struct PhmRecT {
     // binary level
     S M_sec, amp_scale, t0, phi0, Mf_ref;
-    // leading-order PN amplitude model (IMRPhenomHMOnePointFiveSpinPN inputs)
-    S pn_M, pn_eta, pn_delta, pn_Xs, pn_Xa, pn_pref;
+    // leading-order PN amplitude of this mode (IMRPhenomHMOnePointFiveSpinPN):
+    // |H(v)| = |(c1 v + c2 v^2 + c3 v^3 + c4 v^4) + i c4i v^4|
+    S pn_c1, pn_c2, pn_c3, pn_c4, pn_c4i;
     S pn_vscale, pn_v2ratio;  // 2 pi M / m and (2/m)^(1/3) of this mode
     // PhenomD amplitude (mode independent)
     S a_fRD, a_fDM, a_g1, a_g2, a_g3, a_fmax;
@@ -199,28 +200,43 @@ __device__ inline S amp_imr(const PhmRecT<S>& r, nd<S> f, const P6T<S>& p)
     return pre * (r.a_d0 + r.a_d1 * f + r.a_d2 * f2 + r.a_d3 * f3 + r.a_d4 * f4);
 }
 
-// |H_lm(v)| of the leading-order PN amplitude (IMRPhenomHMOnePointFiveSpinPN,
-// PhenomHMWaveform.cu:2405-2449) without the common v^-3.5 prefactor.
+// The coefficients of |H_lm(v)|, the leading-order PN amplitude
+// (IMRPhenomHMOnePointFiveSpinPN, PhenomHMWaveform.cu:2405-2449) without the
+// common v^-3.5 prefactor, as a polynomial in v per (binary, mode): c[0..3] of
+// v..v^4 in the real part, c[4] of v^4 in the imaginary part. Modes other
+// than (2,1), (3,3), (3,2), (4,4), (4,3) have none (their factor is 1).
+template <class S>
+__device__ inline void pn_habs_coeffs(int l, int m, S eta, S delta, S Xs, S Xa, S c[5])
+{
+    for (int k = 0; k < 5; k++) c[k] = 0.0;
+    if (l == 2 && m == 1) {
+        const double pre = 0.47140452079103168;  // sqrt(2) / 3
+        c[0] = pre * delta;
+        c[1] = pre * (-1.5 * (Xa + delta * Xs));
+        c[2] = pre * (delta * ((335.0 / 672.0) + (eta * 117.0 / 56.0)));
+        c[3] = pre * (Xa * (3427.0 / 1344 - eta * 2101.0 / 336) + delta * Xs * (3427.0 / 1344 - eta * 965 / 336) +
+                      delta * (-PI));
+        c[4] = pre * (delta * (-0.5 - 2.0 * 0.69314718056));
+    } else if (l == 3 && m == 3) {
+        c[0] = 0.75 * sqrt(5.0 / 7.0) * delta;
+    } else if (l == 3 && m == 2) {
+        c[1] = (1.0 / 3.0) * sqrt(5.0 / 7.0) * (1.0 - 3.0 * eta);
+    } else if (l == 4 && m == 4) {
+        c[1] = (4.0 / 9.0) * sqrt(10.0 / 7.0) * (1.0 - 3.0 * eta);
+    } else if (l == 4 && m == 3) {
+        c[2] = 0.75 * sqrt(3.0 / 35.0) * delta * (1.0 - 2.0 * eta);
+    }
+}
+
+// |H_lm(v)| from the record's coefficients: one polynomial for every mode, so
+// the evaluation has no branch on the mode
 template <class S>
 __device__ inline S pn_habs(const PhmRecT<S>& r, nd<S> v)
 {
-    const int l = r.ell, m = r.mm;
-    const S eta = r.pn_eta, delta = r.pn_delta;
-    if (l == 2 && m == 1) {
-        const S v2 = v * v, v3 = v * v2, v4 = v * v3;
-        const S Xs = r.pn_Xs, Xa = r.pn_Xa;
-        S re = v * delta - v2 * 1.5 * (Xa + delta * Xs) + v3 * delta * ((335.0 / 672.0) + (eta * 117.0 / 56.0)) +
-                    v4 * (Xa * (3427.0 / 1344 - eta * 2101.0 / 336) + delta * Xs * (3427.0 / 1344 - eta * 965 / 336) +
-                          delta * (-PI));
-        S im = v4 * (delta * (-0.5 - 2.0 * 0.69314718056));
-        const S pre = (sqrt(2.0) / 3.0);
-        return hypot(pre * re, pre * im);
-    }
-    if (l == 3 && m == 3) return fabs(0.75 * sqrt(5.0 / 7.0) * (v * delta));
-    if (l == 3 && m == 2) return fabs((1.0 / 3.0) * sqrt(5.0 / 7.0) * (v * v * (1.0 - 3.0 * eta)));
-    if (l == 4 && m == 4) return fabs((4.0 / 9.0) * sqrt(10.0 / 7.0) * v * v * (1.0 - 3.0 * eta));
-    if (l == 4 && m == 3) return fabs(0.75 * sqrt(3.0 / 35.0) * v * v * v * delta * (1.0 - 2.0 * eta));
-    return 1.0;
+    const S re = v * (r.pn_c1 + v * (r.pn_c2 + v * (r.pn_c3 + v * r.pn_c4)));
+    const S v2 = v * v;
+    const S im = (v2 * v2) * r.pn_c4i;
+    return sqrt(re * re + im * im);
 }
 
 // The HM amplitude factor beta * HM1 / HM2 of get_amp (PhenomHMWaveform.cu:2473-2511).
@@ -291,9 +307,9 @@ __device__ inline S dphi_ins(const PhmRecT<S>& r, nd<S> v, nd<S> logv)
 }
 
 template <class S>
-__device__ inline S phi_int(const PhmRecT<S>& r, nd<S> Mf, nd<S> iMf)
+__device__ inline S phi_int(const PhmRecT<S>& r, nd<S> Mf, nd<S> iMf, nd<S> logMf)
 {
-    return r.beta1 * Mf - r.beta3 * (iMf * iMf * iMf) * (1.0 / 3.0) + r.beta2 * log(Mf);
+    return r.beta1 * Mf - r.beta3 * (iMf * iMf * iMf) * (1.0 / 3.0) + r.beta2 * logMf;
 }
 
 template <class S>
@@ -325,29 +341,33 @@ __device__ inline S dphi_mrd(const PhmRecT<S>& r, nd<S> f, nd<S> iMf)
 
 // IMRPhenDPhase and its derivative at the same Mf, sharing the powers. With
 // 'same' set (the lane's f equals the one 'pre' was built from), the inspiral
-// branch reuses the powers 'pre' instead of computing them again.
+// branch reuses the powers 'pre' instead of computing them again. One log(Mf)
+// serves both the inspiral (log v = (log Mf + log pi) / 3) and the intermediate
+// piece, so a wavefront whose lanes straddle their join computes it once.
 template <class S>
 __device__ inline void phase_imr(const PhmRecT<S>& r, nd<S> f, S& ph, S* dph, const P6T<S>* pre = nullptr,
                                  bool same = false)
 {
     const S iMf = rcp_pos(f);
-    if (!(f >= PHI_JOIN_INS)) {
-        P6T<S> p;
-        if (pre && same) p = *pre;
-        else p = pow6(f);
-        const double pi13 = 1.4645918875615231;
-        S v = p.third * pi13;
-        S logv = log(v);
-        ph = phi_ins(r, f, p, logv, iMf);
-        if (dph) *dph = dphi_ins(r, v, logv);
-        return;
-    }
-    if (f >= r.fMRDJoin) {
+    const bool ins = !(f >= PHI_JOIN_INS);
+    if (!ins && f >= r.fMRDJoin) {
         ph = r.p_etaInv * phi_mrd(r, f, iMf) + r.C1MRD + r.C2MRD * f;
         if (dph) *dph = dphi_mrd(r, f, iMf) + r.C2MRD;
         return;
     }
-    ph = r.p_etaInv * phi_int(r, f, iMf) + r.C1Int + r.C2Int * f;
+    const S logMf = log_pos(f);
+    if (ins) {
+        P6T<S> p;
+        if (pre && same) p = *pre;
+        else p = pow6(f);
+        const double pi13 = 1.4645918875615231, LOG_PI = 1.1447298858494002;
+        S v = p.third * pi13;
+        S logv = (logMf + LOG_PI) * (1.0 / 3.0);
+        ph = phi_ins(r, f, p, logv, iMf);
+        if (dph) *dph = dphi_ins(r, v, logv);
+        return;
+    }
+    ph = r.p_etaInv * phi_int(r, f, iMf, logMf) + r.C1Int + r.C2Int * f;
     if (dph) *dph = dphi_int(r, iMf) + r.C2Int;
 }
 
@@ -576,12 +596,13 @@ __device__ inline void setup_phenomd(PhmRecT<S>& r, nd<S> m1, nd<S> m2, nd<S> ch
         S dins = dphi_ins(r, vj, log(vj));
         r.C2Int = dins - dphi_int(r, ij);
         P6T<S> pj = pow6(S(PHI_JOIN_INS));
-        S logvj = log(pj.third * 1.4645918875615231);
-        r.C1Int = phi_ins(r, PHI_JOIN_INS, pj, logvj, ij) - r.p_etaInv * phi_int(r, PHI_JOIN_INS, ij) -
+        const double logj = log_pos(PHI_JOIN_INS);
+        S logvj = (logj + 1.1447298858494002) * (1.0 / 3.0);
+        r.C1Int = phi_ins(r, PHI_JOIN_INS, pj, logvj, ij) - r.p_etaInv * phi_int(r, PHI_JOIN_INS, ij, logj) -
                   r.C2Int * PHI_JOIN_INS;
         const S fj = r.fMRDJoin;
         const S ifj = 1.0 / fj;
-        S phi_int_j = r.p_etaInv * phi_int(r, fj, ifj) + r.C1Int + r.C2Int * fj;
+        S phi_int_j = r.p_etaInv * phi_int(r, fj, ifj, log_pos(fj)) + r.C1Int + r.C2Int * fj;
         S fj2 = fj * fj;
         S dphi_int_j = r.C2Int + (r.beta1 + r.beta3 / (fj2 * fj2) + r.beta2 / fj) / r.p_eta;
         r.C2MRD = dphi_int_j - dphi_mrd(r, fj, ifj);
@@ -793,12 +814,14 @@ __device__ inline bool phm_setup(PhmRecT<S>& r, nd<S> m1_SI, nd<S> m2_SI, nd<S> 
         S Min = m1 + m2;
         S M1 = m1 / Min, M2 = m2 / Min;
         S M = M1 + M2;
-        r.pn_M = M;
-        r.pn_eta = M1 * M2 / (M * M);
-        r.pn_delta = sqrt(1.0 - 4 * r.pn_eta);
-        r.pn_Xs = 0.5 * (chi1 + chi2);
-        r.pn_Xa = 0.5 * (chi1 - chi2);
-        r.pn_pref = M * M * PI * sqrt(r.pn_eta * 2.0 / 3);
+        const S eta = M1 * M2 / (M * M);
+        S c[5];
+        pn_habs_coeffs(r.ell, r.mm, eta, S(sqrt(1.0 - 4 * eta)), S(0.5 * (chi1 + chi2)), S(0.5 * (chi1 - chi2)), c);
+        r.pn_c1 = c[0];
+        r.pn_c2 = c[1];
+        r.pn_c3 = c[2];
+        r.pn_c4 = c[3];
+        r.pn_c4i = c[4];
         r.pn_vscale = M * 2.0 * PI / (S)r.mm;
         r.pn_v2ratio = cbrt(2.0 / (S)r.mm);
     }
