@@ -1297,6 +1297,9 @@ __global__ __launch_bounds__(64) void k_dirg_knots(double* __restrict__ gpart, c
     const gradk::GradRec& G = grecs[bm];
     const int n = len[b];
     const long base = off[b];
+    __shared__ double2 tab[129];
+    etab_to_lds(tab);
+    __syncthreads();
     double acc[gradk::NGRAD];
     for (int p = 0; p < gradk::NGRAD; p++) acc[p] = 0.0;
     for (int i = threadIdx.x; i < n; i += 64) {
@@ -1310,7 +1313,7 @@ __global__ __launch_bounds__(64) void k_dirg_knots(double* __restrict__ gpart, c
             a_A = aA;
             a_Ph = aPh;
         };
-        gradk::grad_point_core(G, orb, X[rec], win, false, weights, TDItag, rescaled, tdi2, dist, acc);
+        gradk::grad_point_core(G, orb, X[rec], win, false, weights, TDItag, rescaled, tdi2, dist, acc, tab);
     }
     for (int p = 0; p < gradk::NGRAD; p++) acc[p] = wsum(acc[p]);
     if (threadIdx.x == 0)

@@ -111,6 +111,39 @@ __device__ inline void grad_setup_one(GradRec& g, const double* __restrict__ par
     g.s = s;
 }
 
+// The same record in three independent parts (the parts of phm_setup, and the
+// response basis), so that a set-up kernel can run each long dependency chain
+// in a wavefront of its own: PART 0 the mode-dependent PhenomHM coefficients,
+// PART 1 the (2,2) reference, PART 2 the response basis. They write disjoint
+// fields.
+template <int PART>
+__device__ inline void grad_setup_part(GradRec& g, const double* __restrict__ params, int B, int b, int l, int m,
+                                       const double* __restrict__ qnm, int table_len, double dspin)
+{
+    if constexpr (PART == 2) {
+        GradResp s;
+        grad_resp_setup(s, l, m, params[fused::P_PHI * B + b], params[fused::P_INC * B + b],
+                        params[fused::P_LAM * B + b], params[fused::P_BETA * B + b], params[fused::P_PSI * B + b]);
+        g.s = s;
+    } else {
+        const double m1 = params[fused::P_M1 * B + b], m2 = params[fused::P_M2 * B + b];
+        const double Mt = m1 + m2, q = m1 / m2;
+        const double dq = Mt / ((1.0 + q) * (1.0 + q));
+        D4 m1d(m1), m2d(m2);
+        m1d.d[0] = m1;
+        m1d.d[1] = dq;
+        m2d.d[0] = m2;
+        m2d.d[1] = -dq;
+        const D4 c1d = D4::seed(params[fused::P_CHI1 * B + b], 2), c2d = D4::seed(params[fused::P_CHI2 * B + b], 3);
+        D4 finmass, finspin, rdD, dmD, rd, dm;
+        phm::final_state(m1d, m2d, c1d, c2d, finmass, finspin);
+        fused::phenomd_ringdown(finmass, finspin, qnm, table_len, dspin, rdD, dmD);
+        if constexpr (PART == 0) phm::hm_ringdown(l, m, finmass, finspin, rd, dm);
+        phm::phm_setup<D4, PART>(g.w, m1d * MSUN_SI, m2d * MSUN_SI, c1d, c2d, D4(params[fused::P_DIST * B + b]), l,
+                                 m, rd, dm, rdD, dmD, false);
+    }
+}
+
 __device__ inline double dsinc(double a, double cos_a, double sinc_a) { return a == 0.0 ? 0.0 : (cos_a - sinc_a) / a; }
 
 // Gradient contributions of the waveform and response of one (frequency, mode)
@@ -121,11 +154,16 @@ __device__ inline double dsinc(double a, double cos_a, double sinc_a) { return a
 // t = t(f) + t_ref. weights(T, E, A, W, a_A, a_phi) supplies W, a_A, a_phi once
 // T, E = e^{i phi} and A are known. use_window: contribute only if t lies in
 // the observation window (the heterodyned form's rule for its nodes).
-template <class Weights>
+template <class Weights, class E>
 __device__ inline void grad_point_core(const GradRec& G, const resp::OrbitTab& o, double f, const fused::Window& win,
                                        bool use_window, Weights weights, int TDItag, int rescaled, int tdi2,
-                                       double dist, double acc[NGRAD])
+                                       double dist, double acc[NGRAD], const E* __restrict__ tab)
 {
+    auto sincos_t = [&](double th, double* s_, double* c_) {
+        const cd e = cexpi_tab(th, tab);
+        *s_ = e.im;
+        *c_ = e.re;
+    };
     D4 A, Ph, Tf;
     phm::phm_eval(G.w, f, A, Ph, Tf);
     const double t = Tf.v + win.t_ref;
@@ -172,7 +210,7 @@ __device__ inline void grad_point_core(const GradRec& G, const resp::OrbitTab& o
 
     // orbital Doppler phase and its derivatives (Response.cu:278-296)
     double sa, ca;
-    sincos_red(OMEGA0 * t, &sa, &ca);
+    sincos_t(OMEGA0 * t, &sa, &ca);
     const double c2f = 2. * PI / C_SI * f;
     const double pRd = c2f * (R.k[0] * (AU_SI * ca) + R.k[1] * (AU_SI * sa));
     const double pRd_t = c2f * AU_SI * OMEGA0 * (-R.k[0] * sa + R.k[1] * ca);
@@ -183,13 +221,13 @@ __device__ inline void grad_point_core(const GradRec& G, const resp::OrbitTab& o
     // single-link terms as resp_eval forms them
     const double x = PI * f * o.L / C_SI;
     double sx, cx;
-    sincos_red(x, &sx, &cx);
+    sincos_t(x, &sx, &cx);
     cd nHn[3];
     double sm[3], sp[3], dsm[3], dsp[3];
     for (int j = 0; j < 3; j++) {
         nHn[j] = R.Yp * (un[j] * un[j] - vn[j] * vn[j]) + R.Yc * (2.0 * un[j] * vn[j]);
         double s, c;
-        sincos_red(x * kn[j], &s, &c);
+        sincos_t(x * kn[j], &s, &c);
         const double am = x * (1. - kn[j]), ap = x * (1. + kn[j]);
         const double sin_m = sx * c - cx * s, sin_p = sx * c + cx * s;
         const double cos_m = cx * c + sx * s, cos_p = cx * c - sx * s;
@@ -202,7 +240,7 @@ __device__ inline void grad_point_core(const GradRec& G, const resp::OrbitTab& o
     const int ka[3] = {1, 2, 0}, kb[3] = {2, 0, 1};
     cd eL[3], gL[3];
     for (int L = 0; L < 3; L++) {
-        eL[L] = cexpi_red(x * (1. + (kp[ka[L]] + kp[kb[L]]) * o.inv_L));
+        eL[L] = cexpi_tab(x * (1. + (kp[ka[L]] + kp[kb[L]]) * o.inv_L), tab);
         gL[L] = mul_i(x * nHn[L]);
     }
     // Gm = g sm e (G23, G31, G12), Gp = g sp e (G32, G13, G21)
@@ -219,7 +257,7 @@ __device__ inline void grad_point_core(const GradRec& G, const resp::OrbitTab& o
     const cd one = mk(1.0, 0.0);
     cd T[3];
     cd bm[3], bp[3];  // beta of Gm[L], Gp[L]
-    const cd E1 = cexpi_red(phase);
+    const cd E1 = cexpi_tab(phase, tab);
     const double Av = A.v;
     cd W[3];
     double aA = 0.0, aPh = 0.0;

@@ -8,11 +8,13 @@
 // k_grad_setup_fwd one lane per (template, mode): the double record of the
 //                forward pass;
 // k_grad_setup   (on a side stream, concurrently with the two kernels that
-//                follow) one lane per (template, mode): a PhenomHM record whose
+//                follow) three lanes per (template, mode), one per independent
+//                part, each in a wavefront of its own: a PhenomHM record whose
 //                every coefficient carries its derivatives with respect to
 //                (ln M, q, chi1, chi2) (Dual<4> instantiation of the set-up
-//                code), plus the polarisation basis, wave vector and Y factors
-//                with their derivatives in lam, beta (analytic) and cos inc.
+//                code; its mode-dependent part and its (2,2) reference part),
+//                and the polarisation basis, wave vector and Y factors with
+//                their derivatives in lam, beta (analytic) and cos inc.
 // k_grad_weights one workgroup per template (as the forward kernel's):
 //   1. forward: residuals r = h / h0 on the K nodes (LDS) and d_h, h_h,
 //      exactly as the forward kernel;
@@ -57,14 +59,21 @@ __global__ __launch_bounds__(64) void k_grad_setup_fwd(FusedRec* __restrict__ re
 }
 
 // the records with tangents, one lane per (template, mode)
+// records with tangents over SETUP_PARTS * nb blocks of 64 lanes, nb =
+// ceil(B * M / 64): block k does part k / nb of records 64 (k % nb) + tid
 __global__ __launch_bounds__(64) void k_grad_setup(GradRec* __restrict__ grecs, const double* __restrict__ params,
                                                    int B, ModeList modes, int M, const double* __restrict__ qnm,
                                                    int table_len, double dspin)
 {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nb = (B * M + 63) / 64;
+    const int part = blockIdx.x / nb;
+    const int idx = (blockIdx.x - part * nb) * 64 + threadIdx.x;
     if (idx >= B * M) return;
     const int b = idx / M, mi = idx % M;
-    grad_setup_one(grecs[idx], params, B, b, modes.ell[mi], modes.mm[mi], qnm, table_len, dspin);
+    const int l = modes.ell[mi], m = modes.mm[mi];
+    if (part == 0) grad_setup_part<0>(grecs[idx], params, B, b, l, m, qnm, table_len, dspin);
+    else if (part == 1) grad_setup_part<1>(grecs[idx], params, B, b, l, m, qnm, table_len, dspin);
+    else grad_setup_part<2>(grecs[idx], params, B, b, l, m, qnm, table_len, dspin);
 }
 
 // Adjoint weights of the heterodyned sums: for node j and channel c,
@@ -111,7 +120,8 @@ __device__ inline void adjoint_weights(cd* gs, const cd* rs, const double* __res
 // X = T A e^{i phi}), so that W_c = g_c A e^{i phi}, a_A = Re S, a_phi = -A Im S
 // with S = sum_c g_c T_c e^{i phi}.
 __device__ inline void grad_point(const GradRec& G, const resp::OrbitTab& o, double f, const Window& win,
-                                  const cd g[3], int TDItag, int rescaled, int tdi2, double dist, double acc[NGRAD])
+                                  const cd g[3], int TDItag, int rescaled, int tdi2, double dist, double acc[NGRAD],
+                                  const double2* tab)
 {
     auto weights = [&](const cd T[3], cd E1, double Av, cd W[3], double& aA, double& aPh) {
         cd S1 = mk(0.0, 0.0);
@@ -122,7 +132,7 @@ __device__ inline void grad_point(const GradRec& G, const resp::OrbitTab& o, dou
         aA = S1.re;
         aPh = -Av * S1.im;
     };
-    grad_point_core(G, o, f, win, true, weights, TDItag, rescaled, tdi2, dist, acc);
+    grad_point_core(G, o, f, win, true, weights, TDItag, rescaled, tdi2, dist, acc, tab);
 }
 
 // Forward pass and adjoint weights of one template (one wavefront): d_h, h_h
@@ -180,11 +190,14 @@ __global__ __launch_bounds__(64) void k_grad_points(double* __restrict__ partial
     const double dist = params[P_DIST * B + b];
     const GradRec& G = grecs[bm];
     const cd* g_b = gw + (long)b * 3 * K;
+    __shared__ double2 tab[129];
+    etab_to_lds(tab);
+    __syncthreads();
     double acc[NGRAD];
     for (int p = 0; p < NGRAD; p++) acc[p] = 0.0;
     for (int j = threadIdx.x; j < K; j += 64) {
         const cd g[3] = {g_b[j], g_b[K + j], g_b[2 * K + j]};
-        grad_point(G, orb, freqs[j], win, g, TDItag, rescaled, tdi2, dist, acc);
+        grad_point(G, orb, freqs[j], win, g, TDItag, rescaled, tdi2, dist, acc, tab);
     }
     for (int p = 0; p < NGRAD; p++) acc[p] = wsum(acc[p]);
     if (threadIdx.x == 0)
@@ -265,7 +278,8 @@ extern "C" int bbhx_hip_hdyn_fused_grad(double* d_h, double* h_h, double* grad, 
     if (timed) (void)hipEventRecord(ev[0], st);
     if (hipEventRecord(side->fork, st) != hipSuccess || hipStreamWaitEvent(side->stream, side->fork, 0) != hipSuccess)
         return ERR_LAUNCH;
-    hipLaunchKernelGGL(k_grad_setup, dim3(ceil_div((long)nrec, 64)), dim3(64), 0, side->stream, grecs, params,
+    hipLaunchKernelGGL(k_grad_setup, dim3(SETUP_PARTS * ceil_div((long)nrec, 64)), dim3(64), 0, side->stream, grecs,
+                       params,
                        numBinAll, ml, numModes, qnm, table_len, dspin);
     if (hipEventRecord(side->join, side->stream) != hipSuccess) return ERR_LAUNCH;
     hipLaunchKernelGGL(k_grad_setup_fwd, dim3(SETUP_PARTS * ceil_div((long)nrec, 64)), dim3(64), 0, st, recs, params,
