@@ -111,6 +111,258 @@ __device__ inline void grad_setup_one(GradRec& g, const double* __restrict__ par
     g.s = s;
 }
 
+// ------------------------------------------------------------ PhenomHM tangents
+//
+// Amplitude, phase and t(f) of one (binary, mode) with their derivatives in the
+// four intrinsic directions: the evaluation of phm::phm_eval<D4> (same
+// branches, same expressions), arranged so that the functions of the mapped
+// frequency x alone (its powers, logarithm, reciprocal) are evaluated once with
+// their derivative in x (Dual<1>), and every sum of record coefficients times
+// such functions is accumulated as value, coefficient tangents and x
+// derivative, x's own tangents applied once at the end (chain rule):
+//   d/dk sum_i c_i b_i(x) = sum_i c_i,k b_i(x) + x_k sum_i c_i b_i'(x).
+// Only the pieces that are not of that form (the ringdown Lorentzian and
+// arctangent, the mode's PN amplitude factor, the final combination) carry four
+// tangents through every operation.
+
+using X1 = Dual<1>;
+
+struct Lin {
+    double v, c[NI], x;
+};
+
+__device__ inline Lin lin0(double v = 0.0)
+{
+    Lin L;
+    L.v = v;
+    for (int k = 0; k < NI; k++) L.c[k] = 0.0;
+    L.x = 0.0;
+    return L;
+}
+
+// L += c b(x)
+__device__ inline void lin(Lin& L, const D4& c, const X1& b)
+{
+    L.v = fma(c.v, b.v, L.v);
+    for (int k = 0; k < NI; k++) L.c[k] = fma(c.d[k], b.v, L.c[k]);
+    L.x = fma(c.v, b.d[0], L.x);
+}
+
+// L += c
+__device__ inline void lin(Lin& L, const D4& c)
+{
+    L.v += c.v;
+    for (int k = 0; k < NI; k++) L.c[k] += c.d[k];
+}
+
+// the accumulated sum as a D4, at the point x (value and tangents of x)
+__device__ inline D4 lin_at(const Lin& L, const D4& x)
+{
+    D4 r;
+    r.v = L.v;
+    for (int k = 0; k < NI; k++) r.d[k] = fma(L.x, x.d[k], L.c[k]);
+    return r;
+}
+
+// phm::amp_imr at x = Mf_amp (xa: its value and tangents; x: the same value as
+// the Dual<1> variable; p: the powers of x)
+__device__ inline D4 amp_imr_tan(const phm::PhmRecT<D4>& r, const D4& xa, const X1& x, const phm::P6T<X1>& p)
+{
+    Lin Lp = lin0();
+    lin(Lp, r.a_amp0, rcp_pos(x * p.sixth));
+    const D4 pre = lin_at(Lp, xa);
+    if (!(x.v >= phm::AMP_JOIN_INS)) {
+        Lin L = lin0(1.0);
+        lin(L, r.a_pf23, p.two_thirds);
+        lin(L, r.a_pf1, x);
+        lin(L, r.a_pf43, p.four_thirds);
+        lin(L, r.a_pf53, p.five_thirds);
+        lin(L, r.a_pf2, p.two);
+        lin(L, r.a_rho1, p.seven_thirds);
+        lin(L, r.a_rho2, p.eight_thirds);
+        lin(L, r.a_rho3, x * p.two);
+        return pre * lin_at(L, xa);
+    }
+    if (x.v >= r.a_fmax.v) return pre * phm::amp_mrd(r, xa);
+    const X1 x3 = x * p.two, x4 = x * x3;
+    Lin L = lin0();
+    lin(L, r.a_d0);
+    lin(L, r.a_d1, x);
+    lin(L, r.a_d2, p.two);
+    lin(L, r.a_d3, x3);
+    lin(L, r.a_d4, x4);
+    return pre * lin_at(L, xa);
+}
+
+// phm::phase_imr at x = Mf_ph: the PhenomD phase and its derivative in Mf
+__device__ inline void phase_imr_tan(const phm::PhmRecT<D4>& r, const D4& xp, const X1& x,
+                                     const phm::P6T<X1>* pre, bool same, D4& ph, D4& dph)
+{
+    const X1 iMf = rcp_pos(x);
+    const bool ins = !(x.v >= phm::PHI_JOIN_INS);
+    if (!ins && x.v >= r.fMRDJoin.v) {
+        const X1 f075 = sqrt(x * sqrt(x));
+        const D4 q = (xp - r.alpha5 * r.p_fRD) * rcp_pos(r.p_Rholm * r.p_fDM * r.p_Taulm);
+        Lin L = lin0();
+        lin(L, r.alpha2, -iMf);
+        lin(L, r.alpha3, (4.0 / 3.0) * f075);
+        lin(L, r.alpha1, x);
+        const D4 phi_mrd = lin_at(L, xp) + r.alpha4 * r.p_Rholm * atan(q);
+        Lin C = lin0();
+        lin(C, r.C1MRD);
+        lin(C, r.C2MRD, x);
+        ph = r.p_etaInv * phi_mrd + lin_at(C, xp);
+        Lin Ld = lin0();
+        lin(Ld, r.alpha1);
+        lin(Ld, r.alpha2, iMf * iMf);
+        lin(Ld, r.alpha3, f075 * iMf);
+        const D4 dm = lin_at(Ld, xp) + r.alpha4 * rcp_pos(r.p_fDM * r.p_Taulm * (1.0 + q * q));
+        dph = dm * r.p_etaInv + r.C2MRD;
+        return;
+    }
+    const X1 logMf = log_pos(x);
+    if (!ins) {
+        Lin L = lin0();
+        lin(L, r.beta1, x);
+        lin(L, r.beta3, (-1.0 / 3.0) * (iMf * iMf * iMf));
+        lin(L, r.beta2, logMf);
+        Lin C = lin0();
+        lin(C, r.C1Int);
+        lin(C, r.C2Int, x);
+        ph = r.p_etaInv * lin_at(L, xp) + lin_at(C, xp);
+        const X1 im2 = iMf * iMf;
+        Lin Ld = lin0();
+        lin(Ld, r.beta1);
+        lin(Ld, r.beta3, im2 * im2);
+        lin(Ld, r.beta2, iMf);
+        dph = lin_at(Ld, xp) * r.p_etaInv + r.C2Int;
+        return;
+    }
+    phm::P6T<X1> p;
+    if (pre && same) p = *pre;
+    else p = phm::pow6(x);
+    const double pi13 = 1.4645918875615231, LOG_PI = 1.1447298858494002;
+    const X1 v = p.third * pi13;
+    const X1 logv = (logMf + LOG_PI) * (1.0 / 3.0);
+    const X1 im13 = p.two_thirds * iMf;
+    const X1 im23 = im13 * im13;
+    Lin L = lin0();
+    lin(L, r.q_init);
+    lin(L, r.q_23, p.two_thirds);
+    lin(L, r.q_13, p.third);
+    lin(L, r.q_13log, logv * p.third);
+    lin(L, r.q_logv, logv);
+    lin(L, r.q_m13, im13);
+    lin(L, r.q_m23, im23);
+    lin(L, r.q_m1, iMf);
+    lin(L, r.q_m53, im23 * iMf);
+    Lin L2 = lin0();
+    lin(L2, r.q_1, x);
+    lin(L2, r.q_43, p.four_thirds);
+    lin(L2, r.q_53, p.five_thirds);
+    lin(L2, r.q_2, p.two);
+    ph = lin_at(L, xp) + lin_at(L2, xp) * r.p_etaInv;
+    // d phi_ins / dMf with v = (pi Mf)^(1/3) (phm::dphi_ins)
+    const X1 v2 = v * v, v3 = v * v2, v4 = v * v3, v5 = v * v4, v6 = v * v5, v7 = v * v6;
+    const X1 iv = rcp_pos(v), iv2 = iv * iv, iv4 = iv2 * iv2;
+    const X1 w = (iv4 * iv4) * (PI / 3.0);
+    Lin Ld = lin0();
+    lin(Ld, r.pn_v7, 2.0 * (v7 * w));
+    lin(Ld, r.pn_v6, v6 * w);
+    lin(Ld, r.pn_vl6, (1.0 + logv) * (v6 * w));
+    lin(Ld, r.pn_vl5, v5 * w);
+    lin(Ld, r.pn_v4, -1.0 * (v4 * w));
+    lin(Ld, r.pn_v3, -2.0 * (v3 * w));
+    lin(Ld, r.pn_v2, -3.0 * (v2 * w));
+    lin(Ld, r.pn_v0, -5.0 * w);
+    const double ipi13 = 0.68278406325529568, ipi23 = 0.46619407703541166, ipi = 0.31830988618379067;
+    Lin Ls = lin0();
+    lin(Ls, r.sigma1);
+    lin(Ls, r.sigma2, v * ipi13);
+    lin(Ls, r.sigma3, v2 * ipi23);
+    lin(Ls, r.sigma4, v3 * ipi);
+    dph = lin_at(Ld, xp) + lin_at(Ls, xp) * r.p_etaInv;
+}
+
+// phm::pn_habs: |H(v)| with its polynomial parts as sums in v
+__device__ inline D4 pn_habs_tan(const phm::PhmRecT<D4>& r, const D4& v)
+{
+    const X1 x = X1::seed(v.v, 0);
+    const X1 x2 = x * x, x3 = x * x2, x4 = x2 * x2;
+    Lin Lr = lin0();
+    lin(Lr, r.pn_c1, x);
+    lin(Lr, r.pn_c2, x2);
+    lin(Lr, r.pn_c3, x3);
+    lin(Lr, r.pn_c4, x4);
+    Lin Li = lin0();
+    lin(Li, r.pn_c4i, x4);
+    const D4 re = lin_at(Lr, v), im = lin_at(Li, v);
+    return sqrt(re * re + im * im);
+}
+
+// phm::hm_amp_factor
+__device__ inline D4 hm_amp_factor_tan(const phm::PhmRecT<D4>& r, const D4& Mf_wf, const D4& Mf_amp)
+{
+    if (r.ell == 2 && r.mm == 2) return D4(1.0);
+    const D4 v1 = cbrt_pos(r.pn_vscale * Mf_wf);
+    const D4 h1 = pn_habs_tan(r, v1);
+    if (h1.v == 0.0) return D4(0.0);
+    if (!(Mf_wf.v > r.amp_fi.v)) return h1;
+    const D4 v2 = v1 * r.pn_v2ratio;
+    const D4 v3 = cbrt_pos(r.pn_vscale * Mf_amp);
+    return h1 * pn_habs_tan(r, v3) * rcp_pos(pn_habs_tan(r, v2));
+}
+
+// phm::phm_eval<D4>: amplitude, phase and t(f) with their intrinsic tangents
+__device__ inline void phm_eval_tan(const phm::PhmRecT<D4>& r, double f, D4& amp, D4& phase, D4& tf)
+{
+    const D4 Mf_wf = f * r.M_sec;
+    D4 a, b;
+    if (Mf_wf.v > r.amp_fi.v) {
+        if (Mf_wf.v > r.amp_fr.v) {
+            a = D4(1.0);
+            b = r.amp_Br;
+        } else {
+            a = r.amp_Am;
+            b = r.amp_Bm;
+        }
+    } else {
+        a = r.amp_Ai;
+        b = D4(0.0);
+    }
+    const D4 Mf_amp = a * Mf_wf + b;
+    const X1 xa = X1::seed(Mf_amp.v, 0);
+    const phm::P6T<X1> p = phm::pow6(xa);
+    D4 A = amp_imr_tan(r, Mf_amp, xa, p);
+    A *= hm_amp_factor_tan(r, Mf_wf, Mf_amp);
+    amp = A * r.amp_scale;
+
+    D4 sa, sb, c1, c2;
+    if (!(Mf_wf.v > r.fi.v)) {
+        sa = r.ai;
+        sb = r.bi;
+        c1 = D4(0.0);
+        c2 = D4(0.0);
+    } else if (!(Mf_wf.v > r.fr.v)) {
+        sa = r.am;
+        sb = r.bm;
+        c1 = r.PhDBconst;
+        c2 = r.PhDBAterm;
+    } else {
+        sa = r.ar;
+        sb = r.br;
+        c1 = r.PhDCconst;
+        c2 = r.phaseC;
+    }
+    const D4 Mf_ph = sa * Mf_wf + sb;
+    D4 ph, dph;
+    phase_imr_tan(r, Mf_ph, X1::seed(Mf_ph.v, 0), &p, Mf_ph.v == Mf_amp.v, ph, dph);
+    const D4 phi = r.cshift + ((ph * rcp_pos(sa) - c1) + c2);
+    phase = -r.t0 * (Mf_wf - r.Mf_ref) + (phi - (r.mm * r.phi0));
+    const double INV_TWO_PI = 0.15915494309189535;
+    tf = (r.M_sec * (dph * INV_TWO_PI)) - ((r.t0 * INV_TWO_PI) * r.M_sec);
+}
+
 // The same record in three independent parts (the parts of phm_setup, and the
 // response basis), so that a set-up kernel can run each long dependency chain
 // in a wavefront of its own: PART 0 the mode-dependent PhenomHM coefficients,
@@ -165,7 +417,7 @@ __device__ inline void grad_point_core(const GradRec& G, const resp::OrbitTab& o
         *c_ = e.re;
     };
     D4 A, Ph, Tf;
-    phm::phm_eval(G.w, f, A, Ph, Tf);
+    phm_eval_tan(G.w, f, A, Ph, Tf);
     const double t = Tf.v + win.t_ref;
     if (use_window && !fused::in_window(win, t)) return;
     const GradResp& R = G.s;
@@ -231,10 +483,20 @@ __device__ inline void grad_point_core(const GradRec& G, const resp::OrbitTab& o
         const double am = x * (1. - kn[j]), ap = x * (1. + kn[j]);
         const double sin_m = sx * c - cx * s, sin_p = sx * c + cx * s;
         const double cos_m = cx * c + sx * s, cos_p = cx * c - sx * s;
-        sm[j] = am == 0.0 ? 1.0 : sin_m / am;
-        sp[j] = ap == 0.0 ? 1.0 : sin_p / ap;
-        dsm[j] = dsinc(am, cos_m, sm[j]);
-        dsp[j] = dsinc(ap, cos_p, sp[j]);
+        const double prod = am * ap;
+        if (prod != 0.0) {  // 1/am and 1/ap from one reciprocal of their product
+            const double ip = rcp_pos(prod);
+            const double iam = ap * ip, iap = am * ip;
+            sm[j] = sin_m * iam;
+            sp[j] = sin_p * iap;
+            dsm[j] = (cos_m - sm[j]) * iam;
+            dsp[j] = (cos_p - sp[j]) * iap;
+        } else {  // f = 0 or a link along k: sinc(0) = 1, sinc'(0) = 0
+            sm[j] = am == 0.0 ? 1.0 : sin_m / am;
+            sp[j] = ap == 0.0 ? 1.0 : sin_p / ap;
+            dsm[j] = dsinc(am, cos_m, sm[j]);
+            dsp[j] = dsinc(ap, cos_p, sp[j]);
+        }
     }
     // link L = 0: pair (23, 32) with e23; L = 1: (31, 13) with e31; L = 2: (12, 21) with e12
     const int ka[3] = {1, 2, 0}, kb[3] = {2, 0, 1};
