@@ -423,41 +423,39 @@ __device__ inline void grad_point_core(const GradRec& G, const resp::OrbitTab& o
     const GradResp& R = G.s;
 
     // orbit projections (values), their time derivatives, and the spacecraft
-    // projections' derivatives in lam, beta; the links' derivatives in lam,
-    // beta are formed later, link by link, to keep fewer values live
+    // projections' derivatives in lam, beta, from the packed row of the table
+    // (values P and differences D to the next row: P(t) = P + a D, dP/dt =
+    // D / dt); the links' derivatives in lam, beta are formed later, link by
+    // link, to keep fewer values live
     int w = int(floor(t * o.inv_dt));
     w = w < 0 ? 0 : (w > o.N - 2 ? o.N - 2 : w);
     const double a = (t - o.dt * w) * o.inv_dt;
-    const double* x0 = o.x + (long)w * 9;
-    const double* n0 = o.n + (long)w * 18;
-    const int li[3] = {o.li12, o.li23, o.li31};
+    const double* row = o.p + (long)w * resp::ORB_ROW;
+    // the interpolated vector q (0..2 spacecraft, 3..5 links 12, 23, 31) and its difference
+    auto vec = [&](int q, double v[3], double d[3]) {
+        for (int c = 0; c < 3; c++) {
+            d[c] = row[resp::ORB_VALS + 3 * q + c];
+            v[c] = fma(d[c], a, row[3 * q + c]);
+        }
+    };
     double kp[3], dkp[3][3];
     for (int i = 0; i < 3; i++) {
-        const double* lo = x0 + 3 * i;
-        const double* hi = lo + 9;
-        const double l0 = resp::dot3(R.k, lo), h0v = resp::dot3(R.k, hi);
-        kp[i] = fma(h0v - l0, a, l0);
-        dkp[0][i] = (h0v - l0) * o.inv_dt;
-        for (int q = 0; q < 2; q++) {
-            const double lq = resp::dot3(R.dk[q], lo), hq = resp::dot3(R.dk[q], hi);
-            dkp[1 + q][i] = fma(hq - lq, a, lq);
-        }
+        double v[3], d[3];
+        vec(i, v, d);
+        kp[i] = resp::dot3(R.k, v);
+        dkp[0][i] = resp::dot3(R.k, d) * o.inv_dt;
+        for (int q = 0; q < 2; q++) dkp[1 + q][i] = resp::dot3(R.dk[q], v);
     }
     double kn[3], un[3], vn[3], dkn_t[3], dun_t[3], dvn_t[3];
     for (int j = 0; j < 3; j++) {
-        const double* lo = n0 + 3 * li[j];
-        const double* hi = lo + 18;
-        double l0 = resp::dot3(R.k, lo), h0v = resp::dot3(R.k, hi);
-        kn[j] = fma(h0v - l0, a, l0);
-        dkn_t[j] = (h0v - l0) * o.inv_dt;
-        l0 = resp::dot3(R.u, lo);
-        h0v = resp::dot3(R.u, hi);
-        un[j] = fma(h0v - l0, a, l0);
-        dun_t[j] = (h0v - l0) * o.inv_dt;
-        l0 = resp::dot3(R.v, lo);
-        h0v = resp::dot3(R.v, hi);
-        vn[j] = fma(h0v - l0, a, l0);
-        dvn_t[j] = (h0v - l0) * o.inv_dt;
+        double v[3], d[3];
+        vec(3 + j, v, d);
+        kn[j] = resp::dot3(R.k, v);
+        dkn_t[j] = resp::dot3(R.k, d) * o.inv_dt;
+        un[j] = resp::dot3(R.u, v);
+        dun_t[j] = resp::dot3(R.u, d) * o.inv_dt;
+        vn[j] = resp::dot3(R.v, v);
+        dvn_t[j] = resp::dot3(R.v, d) * o.inv_dt;
     }
 
     // orbital Doppler phase and its derivatives (Response.cu:278-296)
@@ -576,17 +574,12 @@ __device__ inline void grad_point_core(const GradRec& G, const resp::OrbitTab& o
         dunL[0] = dun_t[L];
         dvnL[0] = dvn_t[L];
         {
-            const double* lo = n0 + 3 * li[L];
-            const double* hi = lo + 18;
+            double v[3], d[3];
+            vec(3 + L, v, d);
             for (int q = 0; q < 2; q++) {
-                double lq = resp::dot3(R.dk[q], lo), hq = resp::dot3(R.dk[q], hi);
-                dknL[1 + q] = fma(hq - lq, a, lq);
-                lq = resp::dot3(R.du[q], lo);
-                hq = resp::dot3(R.du[q], hi);
-                dunL[1 + q] = fma(hq - lq, a, lq);
-                lq = resp::dot3(R.dv[q], lo);
-                hq = resp::dot3(R.dv[q], hi);
-                dvnL[1 + q] = fma(hq - lq, a, lq);
+                dknL[1 + q] = resp::dot3(R.dk[q], v);
+                dunL[1 + q] = resp::dot3(R.du[q], v);
+                dvnL[1 + q] = resp::dot3(R.dv[q], v);
             }
         }
         const cd BL = bm[L] * sm[L] + bp[L] * sp[L];
