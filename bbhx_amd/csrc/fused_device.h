@@ -24,9 +24,14 @@ struct FusedRec {
     resp::RespRec s;
 };
 
+// The modes, and the order in which the residual pass evaluates them: modes
+// of equal m next to each other, pair[k] = 1 where order[k + 1] has the m of
+// order[k] and is evaluated with it (see residuals).
 struct ModeList {
     int ell[MAX_MODES];
     int mm[MAX_MODES];
+    int order[MAX_MODES];
+    int pair[MAX_MODES];
 };
 
 // PhenomD (2,2) ringdown frequencies in units of 1/M from the QNM table spline
@@ -115,10 +120,21 @@ __device__ inline bool in_window(const Window& w, double tf)
 // combination T_c is linear in the link terms with coefficients that depend
 // on the frequency alone (WaveformBuild.cu:48-60 sums conj(T A e^{i phi}) mode
 // by mode; the same sum, reassociated). 'tab' is the e^{i theta} table.
+//
+// Two modes of equal m (the (2,2) and (3,2), the (3,3) and (4,3)) have the same
+// PhenomD amplitude, phase and t(f) wherever both are on the first piece of
+// their frequency maps (Mf = (2/m) f M for both) and the phase on its inspiral
+// piece: those use only coefficients that do not depend on the mode's ringdown
+// (the (2,2) amplitude, the TaylorF2 and inspiral fits, t0, phi0), so the
+// second mode's amplitude, phase and t(f) equal the first's except for its PN
+// amplitude factor, bit for bit as its own evaluation would give them. When a
+// whole wavefront is there, the pair is evaluated once: one PhenomHM
+// evaluation, the second mode's factor, and one response evaluation with the
+// two modes' Y factors weighted by their amplitudes.
 template <class E>
-__device__ inline void residuals(cd* rs, const FusedRec* __restrict__ rb, int M, const Window& win,
-                                 const double* __restrict__ freqs, const cd* __restrict__ h0, int K,
-                                 const resp::OrbitTab& orb, int TDItag, int rescaled, int tdi2,
+__device__ inline void residuals(cd* rs, const FusedRec* __restrict__ rb, const ModeList& ml, int M,
+                                 const Window& win, const double* __restrict__ freqs, const cd* __restrict__ h0,
+                                 int K, const resp::OrbitTab& orb, int TDItag, int rescaled, int tdi2,
                                  const E* __restrict__ tab)
 {
     for (int j = threadIdx.x; j < K; j += blockDim.x) {
@@ -126,15 +142,32 @@ __device__ inline void residuals(cd* rs, const FusedRec* __restrict__ rb, int M,
         const resp::NodeResp nr = resp::resp_node(orb, f);
         cd S[6];
         for (int k = 0; k < 6; k++) S[k] = mk(0.0, 0.0);
-        for (int mi = 0; mi < M; mi++) {
-            const FusedRec& r = rb[mi];
-            double amp, phase, tf;
-            phm::phm_eval(r.w, f, amp, phase, tf);
+        for (int k = 0; k < M; k++) {
+            const FusedRec& r = rb[ml.order[k]];
+            double amp, phase, tf, ad;
+            phm::phm_eval(r.w, f, amp, phase, tf, &ad);
             // phenomhm.py:487-507: phase += 2 pi f (t_ref - t0); tf += t_ref
             phase = phase + ((f * win.dt_shift) * 2) * PI;
             tf = tf + win.t_ref;
             cd G[6];
             double dphi;
+            if (ml.pair[k]) {
+                const FusedRec& r2 = rb[ml.order[k + 1]];
+                const double Mf = f * r.w.M_sec;
+                const bool same = Mf <= fmin(fmin(r.w.fi, r2.w.fi), fmin(r.w.amp_fi, r2.w.amp_fi)) &&
+                                  r.w.ai * Mf < phm::PHI_JOIN_INS;
+                if (__all(same)) {
+                    const double amp2 = (ad * phm::hm_amp_factor(r2.w, Mf, r2.w.amp_Ai * Mf)) * r2.w.amp_scale;
+                    const cd Yp = r.s.Yp * amp + r2.s.Yp * amp2, Yc = r.s.Yc * amp + r2.s.Yc * amp2;
+                    resp::resp_links_y(r.s, Yp, Yc, orb, nr, f, tf, G, dphi, tab);
+                    if (in_window(win, tf)) {
+                        const cd e = cexpi_tab(phase + dphi, tab);
+                        for (int q = 0; q < 6; q++) S[q] += G[q] * e;
+                    }
+                    k++;
+                    continue;
+                }
+            }
             resp::resp_links(r.s, orb, nr, f, tf, G, dphi, tab);
             // The waveform phase multiplies the link terms after they are
             // formed, not inside their exponentials: the TDI combination takes

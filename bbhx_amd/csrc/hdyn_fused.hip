@@ -44,16 +44,17 @@ __global__ __launch_bounds__(64) void k_fused_setup(FusedRec* __restrict__ recs,
 // One workgroup per template, of residual_waves(B, K) wavefronts (one at large
 // batches; several at small ones, so that the chip still holds two per SIMD).
 // Dynamic LDS holds the residuals r[3][K] of this template. The register budget
-// is capped so that two wavefronts share each SIMD (256 VGPRs each; a few
-// values spill to scratch):
-// measured on MI355X at 4096 templates x 128 nodes, 0.23 ms against 0.30 ms for
-// one wavefront per SIMD without spills, and 0.38 ms for three per SIMD.
+// is capped so that two wavefronts share each SIMD (256 VGPRs each, no spills):
+// measured on MI355X at 4096 templates x 128 nodes, 0.151 ms; capped at three
+// per SIMD (168 VGPRs) 79 values spill to scratch and it takes 0.245 ms. Reading
+// the records from LDS instead of through the scalar cache was 2 % faster but
+// spilled 27 VGPRs to scratch, and was not kept.
 __global__ __launch_bounds__(64 * MAX_RESIDUAL_WAVES) __attribute__((amdgpu_waves_per_eu(2))) void k_fused_eval(
     double* __restrict__ d_h, double* __restrict__ h_h, double* __restrict__ ll, double d_d,
     const FusedRec* __restrict__ recs, const double* __restrict__ params, const double* __restrict__ t_start,
-    const double* __restrict__ t_end, int B, int M, const double* __restrict__ freqs, const cd* __restrict__ h0,
-    const cd* __restrict__ dc, int K, resp::OrbitTab orb, int TDItag, int rescaled, int tdi2, int jend,
-    double initial_t_val)
+    const double* __restrict__ t_end, int B, ModeList ml, int M, const double* __restrict__ freqs,
+    const cd* __restrict__ h0, const cd* __restrict__ dc, int K, resp::OrbitTab orb, int TDItag, int rescaled,
+    int tdi2, int jend, double initial_t_val)
 {
     extern __shared__ cd rs[];  // [3][K]
     __shared__ double2 tab[129];
@@ -65,7 +66,7 @@ __global__ __launch_bounds__(64 * MAX_RESIDUAL_WAVES) __attribute__((amdgpu_wave
     win.te = t_end[b];
     win.dt_shift = win.t_ref - initial_t_val;
     __syncthreads();
-    residuals(rs, recs + (long)b * M, M, win, freqs, h0, K, orb, TDItag, rescaled, tdi2, tab);
+    residuals(rs, recs + (long)b * M, ml, M, win, freqs, h0, K, orb, TDItag, rescaled, tdi2, tab);
     __syncthreads();
     double sums[4];
     segment_sums(rs, freqs, dc, K, jend, sums);
@@ -125,6 +126,23 @@ int check_modes(const int* ells_host, const int* mms_host, int numModes, ModeLis
         ml.ell[m] = l;
         ml.mm[m] = mm;
     }
+    // evaluation order: each mode followed by the next unplaced mode of equal m
+    bool placed[MAX_MODES] = {};
+    int k = 0;
+    for (int m = 0; m < numModes; m++) {
+        if (placed[m]) continue;
+        placed[m] = true;
+        ml.order[k] = m;
+        ml.pair[k++] = 0;
+        for (int n = m + 1; n < numModes; n++)
+            if (!placed[n] && ml.mm[n] == ml.mm[m]) {
+                placed[n] = true;
+                ml.pair[k - 1] = 1;
+                ml.order[k] = n;
+                ml.pair[k++] = 0;
+                break;
+            }
+    }
     return OK;
 }
 
@@ -183,7 +201,7 @@ int hdyn_fused_launch(FusedRec* recs, double* d_h, double* h_h, const double* pa
     const size_t lds = sizeof(cd) * 3 * (size_t)K;
     hipLaunchKernelGGL(k_fused_eval, dim3(numBinAll), dim3(64 * residual_waves(numBinAll, K)), lds, st, d_h, h_h,
                        logL, d_d, recs, params, t_start, t_end,
-                       numBinAll, numModes, freqs, reinterpret_cast<const cd*>(h0),
+                       numBinAll, ml, numModes, freqs, reinterpret_cast<const cd*>(h0),
                        reinterpret_cast<const cd*>(dataConstants), K, orb, TDItag, rescaled, tdi2, jend,
                        initial_t_val);
     if (timed) (void)hipEventRecord(ev[2], st);

@@ -141,9 +141,9 @@ __device__ inline void grad_point(const GradRec& G, const resp::OrbitTab& o, dou
 __global__ __launch_bounds__(64 * MAX_RESIDUAL_WAVES) __attribute__((amdgpu_waves_per_eu(2))) void k_grad_weights(
     double* __restrict__ d_h, double* __restrict__ h_h, double* __restrict__ ll, double d_d, cd* __restrict__ gw,
     const FusedRec* __restrict__ recs, const double* __restrict__ params, const double* __restrict__ t_start,
-    const double* __restrict__ t_end, int B, int M, const double* __restrict__ freqs, const cd* __restrict__ h0,
-    const cd* __restrict__ dc, int K, resp::OrbitTab orb, int TDItag, int rescaled, int tdi2, int jend,
-    double initial_t_val)
+    const double* __restrict__ t_end, int B, ModeList ml, int M, const double* __restrict__ freqs,
+    const cd* __restrict__ h0, const cd* __restrict__ dc, int K, resp::OrbitTab orb, int TDItag, int rescaled,
+    int tdi2, int jend, double initial_t_val)
 {
     extern __shared__ cd rs[];  // residuals [3][K]
     __shared__ double2 tab[129];
@@ -155,7 +155,7 @@ __global__ __launch_bounds__(64 * MAX_RESIDUAL_WAVES) __attribute__((amdgpu_wave
     win.te = t_end[b];
     win.dt_shift = win.t_ref - initial_t_val;
     __syncthreads();
-    residuals(rs, recs + (long)b * M, M, win, freqs, h0, K, orb, TDItag, rescaled, tdi2, tab);
+    residuals(rs, recs + (long)b * M, ml, M, win, freqs, h0, K, orb, TDItag, rescaled, tdi2, tab);
     __syncthreads();
     double sums[4];
     segment_sums(rs, freqs, dc, K, jend, sums);
@@ -290,7 +290,7 @@ extern "C" int bbhx_hip_hdyn_fused_grad(double* d_h, double* h_h, double* grad, 
     const int jend = include_last_segment ? K : K - 1;
     hipLaunchKernelGGL(k_grad_weights, dim3(numBinAll), dim3(64 * residual_waves(numBinAll, K)),
                        sizeof(cd) * 3 * (size_t)K, st, d_h, h_h, logL, d_d, gw, recs,
-                       params, t_start, t_end, numBinAll, numModes, freqs, reinterpret_cast<const cd*>(h0),
+                       params, t_start, t_end, numBinAll, ml, numModes, freqs, reinterpret_cast<const cd*>(h0),
                        reinterpret_cast<const cd*>(dataConstants), K, orb, TDItag, rescaled, tdi2, jend,
                        initial_t_val);
     if (hipStreamWaitEvent(st, side->join, 0) != hipSuccess) return ERR_LAUNCH;

@@ -869,8 +869,10 @@ __device__ inline bool phm_setup(PhmRecT<S>& r, nd<S> m1_SI, nd<S> m2_SI, nd<S> 
 
 // Amplitude, phase and t(f) of one (binary, mode) at frequency f in Hz
 // (calculate_modes / calculate_modes_phenomd, PhenomHMWaveform.cu:2461-2662).
+// ad_out (optional): the PhenomD amplitude at the mapped frequency, before the
+// mode's PN factor and the distance scaling (amp = (A_D * factor) * amp_scale).
 template <class S>
-__device__ inline void phm_eval(const PhmRecT<S>& r, double f, S& amp, S& phase, S& tf)
+__device__ inline void phm_eval(const PhmRecT<S>& r, double f, S& amp, S& phase, S& tf, S* ad_out = nullptr)
 {
     // A PhenomD-only record (run_phenomd) carries identity maps (a = 1, b = 0,
     // breakpoints at 1e300) and l = m = 2, so it takes the same path: every map
@@ -888,6 +890,7 @@ __device__ inline void phm_eval(const PhmRecT<S>& r, double f, S& amp, S& phase,
     const S Mf_amp = a * Mf_wf + b;
     P6T<S> p = pow6(Mf_amp);
     S A = amp_imr(r, Mf_amp, p);
+    if (ad_out) *ad_out = A;
     A *= hm_amp_factor(r, Mf_wf, Mf_amp);
     amp = A * r.amp_scale;
 

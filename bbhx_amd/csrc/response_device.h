@@ -155,9 +155,12 @@ __device__ inline NodeResp resp_node(const OrbitTab& o, double f)
 // G[0..5] = G12, G21, G23, G32, G31, G13 (Response.cu:54-194), and the
 // orbital Doppler phase in 'phaseRdelay' (Response.cu:278-309); e^{i theta}
 // from the table 'tab' (cexpi_tab).
+// As resp_links, with the Y+ and Yx factors given (Yp, Yc in place of r.Yp,
+// r.Yc): the link terms are linear in them, so several modes that share f and t
+// give their weighted sum from one call.
 template <class E>
-__device__ inline void resp_links(const RespRec& r, const OrbitTab& o, const NodeResp& nr, double f, double t,
-                                  cd G[6], double& phaseRdelay, const E* __restrict__ tab)
+__device__ inline void resp_links_y(const RespRec& r, cd Yp, cd Yc, const OrbitTab& o, const NodeResp& nr, double f,
+                                    double t, cd G[6], double& phaseRdelay, const E* __restrict__ tab)
 {
     double kp[3], kn[3], un[3], vn[3];
     orbit_eval(o, r, t, kp, kn, un, vn);
@@ -170,7 +173,7 @@ __device__ inline void resp_links(const RespRec& r, const OrbitTab& o, const Nod
     cd nHn[3];
     double sp[3], sm[3];  // sm: sinc(x (1 - kn)), sp: sinc(x (1 + kn))
     for (int j = 0; j < 3; j++) {
-        nHn[j] = r.Yp * (un[j] * un[j] - vn[j] * vn[j]) + r.Yc * (2.0 * un[j] * vn[j]);
+        nHn[j] = Yp * (un[j] * un[j] - vn[j] * vn[j]) + Yc * (2.0 * un[j] * vn[j]);
         const cd e = cexpi_tab(x * kn[j], tab);
         const double s = e.im, c = e.re;
         // 1/am and 1/ap from one reciprocal of their product
@@ -196,6 +199,13 @@ __device__ inline void resp_links(const RespRec& r, const OrbitTab& o, const Nod
     G[3] = (g1 * sp[0]) * e23;
     G[4] = (g2 * sm[1]) * e31;
     G[5] = (g2 * sp[1]) * e31;
+}
+
+template <class E>
+__device__ inline void resp_links(const RespRec& r, const OrbitTab& o, const NodeResp& nr, double f, double t,
+                                  cd G[6], double& phaseRdelay, const E* __restrict__ tab)
+{
+    resp_links_y(r, r.Yp, r.Yc, o, nr, f, t, G, phaseRdelay, tab);
 }
 
 // TDI channels 1..3 from link terms G[0..5] (order of resp_links) at the
