@@ -422,42 +422,53 @@ template <class S>
 __device__ inline bool qnm_cw(S kappa, int l, int m_in, S& out_re, S& out_im)
 {
     int m = m_in < 0 ? -m_in : m_in;
-    // amplitude / phase pairs per power of kappa; 'base' is the kappa^0 real offset for
-    // zero-damped modes, whose series starts at kappa^1
-    double A[7], P[7];
+    // complex coefficient per power of kappa, A_i e^{i P_i} of the fits in
+    // Cartesian form (A_i cos P_i, A_i sin P_i; the published amplitude / phase
+    // pairs, evaluated once so that no sine or cosine is taken per record);
+    // 'base' is the kappa^0 real offset for zero-damped modes, whose series
+    // starts at kappa^1
+    const double* C;
+    const double* Sn;
     int n;
     double base;
     bool zd;
+    static constexpr double C22[5] = {-1.5137612398456926, 1.82438009537132, -1.9491472924851778, 1.3160523687197017,
+                                      -0.386253596520847};
+    static constexpr double S22[5] = {0.36798666300537497, -0.6913198959256075, 0.7807925359505982,
+                                      -0.5087007674840115, 0.13940135002826776};
+    static constexpr double C32[7] = {1.0224518751857428, 0.19456949695384276, -1.7046730536276729, 0.9334708870357518,
+                                      1.2817015416050352, -2.047141684957698, 0.8331491642591967};
+    static constexpr double S32[7] = {0.0049793799973692964, 0.15266302925156916, 0.00564189789146444,
+                                      0.15375458653347265, -0.8390203896453422, 1.0049939601312927,
+                                      -0.3937391358806763};
+    static constexpr double C44[5] = {-2.6333344971899724, 2.8983059270065543, -3.1068517011428427, 2.036655322665644,
+                                      -0.5743326515202851};
+    static constexpr double S44[5] = {0.3678874534055692, -0.6854411640438772, 0.8400728137834692, -0.6155714539424388,
+                                      0.18496398129024574};
+    static constexpr double C21[7] = {0.5885550727060468, -0.12446771988120166, 1.0237846421677674, -5.5694819533803495,
+                                      10.070990838680922, -8.324170006013349, 2.678477439380642};
+    static constexpr double S21[7] = {0.02563304821865536, 0.14217947242002135, -0.5240832169327109, 2.352281987769593,
+                                      -4.729608920661527, 4.251675580693241, -1.434409619054415};
+    static constexpr double C33[5] = {-2.06305539331511, 2.3690119585178913, -2.526835645169051, 1.6514625332798913,
+                                      -0.46707723604718027};
+    static constexpr double S33[5] = {0.36821285387997693, -0.6977975754308797, 0.8486034802921025, -0.6037524077483988,
+                                      0.17577435831730695};
+    static constexpr double C43[5] = {0.1697708308937892, -3.07896855783127, 4.110293599941198, -2.8795369847373125,
+                                      0.8476113318527659};
+    static constexpr double S43[5] = {0.11498576909179864, 0.3881163320136248, -1.0247991220739723, 0.9394221404554335,
+                                      -0.32682921703275014};
     if (l == 2 && m == 2) {
-        const double a[5] = {1.557847, 1.95097051, 2.09971716, 1.41094660, 0.41063923};
-        const double p[5] = {2.903124, 5.920970, 2.760585, 5.914340, 2.795235};
-        n = 5; base = 1.0; zd = true;
-        for (int i = 0; i < 5; i++) { A[i] = a[i]; P[i] = p[i]; }
+        C = C22; Sn = S22; n = 5; base = 1.0; zd = true;
     } else if (l == 3 && m == 2) {
-        const double a[7] = {1.022464, 0.24731213, 1.70468239, 0.94604882, 1.53189884, 2.28052668, 0.92150314};
-        const double p[7] = {0.004870, 0.665292, 3.138283, 0.163247, 5.703573, 2.685231, 5.841704};
-        n = 7; base = 0.0; zd = false;
-        for (int i = 0; i < 7; i++) { A[i] = a[i]; P[i] = p[i]; }
+        C = C32; Sn = S32; n = 7; base = 0.0; zd = false;
     } else if (l == 4 && m == 4) {
-        const double a[5] = {2.658908, 2.97825567, 3.21842350, 2.12764967, 0.60338186};
-        const double p[5] = {3.002787, 6.050955, 2.877514, 5.989669, 2.830031};
-        n = 5; base = 2.0; zd = true;
-        for (int i = 0; i < 5; i++) { A[i] = a[i]; P[i] = p[i]; }
+        C = C44; Sn = S44; n = 5; base = 2.0; zd = true;
     } else if (l == 2 && m == 1) {
-        const double a[7] = {0.589113, 0.18896353, 1.15012965, 6.04585476, 11.12627777, 9.34711461, 3.03838318};
-        const double p[7] = {0.043525, 2.289868, 5.810057, 2.741967, 5.844130, 2.669372, 5.791518};
-        n = 7; base = 0.0; zd = false;
-        for (int i = 0; i < 7; i++) { A[i] = a[i]; P[i] = p[i]; }
+        C = C21; Sn = S21; n = 7; base = 0.0; zd = false;
     } else if (l == 3 && m == 3) {
-        const double a[5] = {2.095657, 2.46964352, 2.66552551, 1.75836443, 0.49905688};
-        const double p[5] = {2.964973, 5.996734, 2.817591, 5.932693, 2.781658};
-        n = 5; base = 1.5; zd = true;
-        for (int i = 0; i < 5; i++) { A[i] = a[i]; P[i] = p[i]; }
+        C = C33; Sn = S33; n = 5; base = 1.5; zd = true;
     } else if (l == 4 && m == 3) {
-        const double a[5] = {0.205046, 3.10333396, 4.23612166, 3.02890198, 0.90843949};
-        const double p[5] = {0.595328, 3.016200, 6.038842, 2.826239, 5.915164};
-        n = 5; base = 1.5; zd = true;
-        for (int i = 0; i < 5; i++) { A[i] = a[i]; P[i] = p[i]; }
+        C = C43; Sn = S43; n = 5; base = 1.5; zd = true;
     } else {
         out_re = 0.0;
         out_im = 0.0;
@@ -466,10 +477,8 @@ __device__ inline bool qnm_cw(S kappa, int l, int m_in, S& out_re, S& out_im)
     S sum_re = 0.0, sum_im = 0.0;
     S kp = 1.0;
     for (int i = 0; i < n; i++) {
-        double s, c;
-        sincos(P[i], &s, &c);
-        sum_re += (A[i] * kp) * c;
-        sum_im += (A[i] * kp) * s;
+        sum_re += C[i] * kp;
+        sum_im += Sn[i] * kp;
         kp *= kappa;
     }
     S ans_re = zd ? base + kappa * sum_re : sum_re;
