@@ -43,12 +43,13 @@ __global__ __launch_bounds__(64) void k_fused_setup(FusedRec* __restrict__ recs,
 
 // One workgroup per template, of residual_waves(B, K) wavefronts (one at large
 // batches; several at small ones, so that the chip still holds two per SIMD).
-// Dynamic LDS holds the residuals r[3][K] of this template. The register budget
-// is capped so that two wavefronts share each SIMD (256 VGPRs each, no spills):
-// measured on MI355X at 4096 templates x 128 nodes, 0.151 ms; capped at three
-// per SIMD (168 VGPRs) 79 values spill to scratch and it takes 0.245 ms. Reading
-// the records from LDS instead of through the scalar cache was 2 % faster but
-// spilled 27 VGPRs to scratch, and was not kept.
+// Dynamic LDS holds the residuals r[3][K] of this template; static LDS its
+// records and the e^{i theta} table. The register budget is capped so that two
+// wavefronts share each SIMD (256 VGPRs each): 15 values live across the node
+// loop spill to scratch (12 MB written per launch at 4096 templates), which
+// costs less than the scalar-load waits of reading the records through the
+// scalar cache (no spills; 0.140 against 0.129 ms at 4096 x 128 nodes).
+// Capped at three per SIMD (168 VGPRs) 79 values spill and it takes 0.245 ms.
 __global__ __launch_bounds__(64 * MAX_RESIDUAL_WAVES) __attribute__((amdgpu_waves_per_eu(2))) void k_fused_eval(
     double* __restrict__ d_h, double* __restrict__ h_h, double* __restrict__ ll, double d_d,
     const FusedRec* __restrict__ recs, const double* __restrict__ params, const double* __restrict__ t_start,
@@ -65,8 +66,10 @@ __global__ __launch_bounds__(64 * MAX_RESIDUAL_WAVES) __attribute__((amdgpu_wave
     win.ts = t_start[b];
     win.te = t_end[b];
     win.dt_shift = win.t_ref - initial_t_val;
+    __shared__ FusedRec srec[MAX_MODES];
+    records_to_lds(srec, recs + (long)b * M, M);
     __syncthreads();
-    residuals(rs, recs + (long)b * M, ml, M, win, freqs, h0, K, orb, TDItag, rescaled, tdi2, tab);
+    residuals(rs, srec, ml, M, win, freqs, h0, K, orb, TDItag, rescaled, tdi2, tab);
     __syncthreads();
     double sums[4];
     segment_sums(rs, freqs, dc, K, jend, sums);

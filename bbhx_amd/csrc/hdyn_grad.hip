@@ -154,8 +154,10 @@ __global__ __launch_bounds__(64 * MAX_RESIDUAL_WAVES) __attribute__((amdgpu_wave
     win.ts = t_start[b];
     win.te = t_end[b];
     win.dt_shift = win.t_ref - initial_t_val;
+    __shared__ FusedRec srec[MAX_MODES];
+    records_to_lds(srec, recs + (long)b * M, M);
     __syncthreads();
-    residuals(rs, recs + (long)b * M, ml, M, win, freqs, h0, K, orb, TDItag, rescaled, tdi2, tab);
+    residuals(rs, srec, ml, M, win, freqs, h0, K, orb, TDItag, rescaled, tdi2, tab);
     __syncthreads();
     double sums[4];
     segment_sums(rs, freqs, dc, K, jend, sums);
