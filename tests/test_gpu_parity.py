@@ -499,6 +499,35 @@ def test_direct_likelihood(backend, orbits, small_problem):
     assert np.max(np.abs(ll - ll_o)) < 1e-6
 
 
+@pytest.mark.parametrize("modes", [[(2, 2), (3, 3)], [(4, 3), (2, 1), (3, 3), (4, 4)], [(3, 2), (4, 4), (2, 2)]])
+def test_heterodyned_mode_lists(backend, orbits, small_problem, modes):
+    # the residual pass evaluates modes of equal m together where that is exact;
+    # lists with no such pair, a pair apart in the list, and a pair listed
+    # second-first, against the oracle (which evaluates mode by mode in list order)
+    from bbhx_amd.likelihood import HeterodynedLikelihood
+    from bbhx_amd.utils.sensitivity import AET1SensitivityMatrix
+    from bbhx_amd.waveformbuild import BBHWaveformFD
+
+    sp = small_problem
+    wave = BBHWaveformFD(response_kwargs=dict(orbits=orbits), force_backend=backend)
+    sens = AET1SensitivityMatrix(sp["f"])
+    het = HeterodynedLikelihood(wave, sp["f"], sp["data"], sp["ref"], 128, sens_mat=sens,
+                                template_gen_kwargs=dict(modes=modes),
+                                reference_gen_kwargs=dict(length=1024, modes=modes), force_backend=backend)
+    psd_fn = lambda x: [AET1SensitivityMatrix(x)[i] for i in range(3)]  # noqa: E731
+    ohet = OracleHeterodyne(sp["owave"], sp["f"], sp["data"], sp["ref"], 128, psd_fn, reference_length=1024,
+                            modes=modes)
+    p = perturb(sp["ref"], 9, 29, scale=1e-4)
+    ll_o = ohet.get_ll(p)[0]
+    ll_f = het.get_ll(p, fused=True)
+    ll_p = het.get_ll(p, fused=False)
+    print("modes", modes, "max |dlogL| fused", np.max(np.abs(ll_f - ll_o)), "plain", np.max(np.abs(ll_p - ll_o)),
+          flush=True)
+    assert np.all(np.isfinite(ll_f))
+    assert np.max(np.abs(ll_f - ll_o)) < 1e-6
+    assert np.max(np.abs(ll_p - ll_o)) < 1e-6
+
+
 @pytest.mark.parametrize("K,last", [(128, "gpu"), (100, "cpu"), (257, "gpu")])
 def test_heterodyned_likelihood(backend, orbits, small_problem, K, last):
     from bbhx_amd.likelihood import HeterodynedLikelihood
