@@ -135,6 +135,65 @@ __device__ inline double log_pos(double x)
     return fma(ed, LN2_HI, fma(ed, LN2_LO, lm));
 }
 
+// e^x for finite x: x = k ln2 + r with ln2 in the two parts of fdlibm's
+// Cody-Waite reduction (|r| <= ln2 / 2), e^r by its Taylor series to r^13/13!
+// (truncation < 5e-18 relative), then 2^k by the exponent (0 below the
+// double range). Within ~1 ulp.
+__device__ inline double exp_fast(double x)
+{
+    const double k = rint(x * 1.4426950408889634);
+    double r = fma(-k, 6.93147180369123816490e-01, x);
+    r = fma(-k, 1.90821492927058770002e-10, r);
+    double p = 1.6059043836821613e-10;
+    p = fma(p, r, 2.08767569878681e-09);
+    p = fma(p, r, 2.505210838544172e-08);
+    p = fma(p, r, 2.755731922398589e-07);
+    p = fma(p, r, 2.7557319223985893e-06);
+    p = fma(p, r, 2.48015873015873e-05);
+    p = fma(p, r, 0.0001984126984126984);
+    p = fma(p, r, 0.001388888888888889);
+    p = fma(p, r, 0.008333333333333333);
+    p = fma(p, r, 0.041666666666666664);
+    p = fma(p, r, 0.16666666666666666);
+    p = fma(p, r, 0.5);
+    p = fma(p, r, 1.0);
+    p = fma(p, r, 1.0);
+    return __builtin_amdgcn_ldexp(p, (int)k);
+}
+
+// arctan of a finite x: |x| > 1 by atan|x| = pi/2 - atan(1/|x|); y in [0, 1]
+// then by the addition formula atan y = atan c_j + atan((y - c_j) / (1 + y c_j))
+// with c_j the double nearest tan(j pi/12), j = 0..3 the nearest node
+// (|argument| <= tan(pi/24) = 0.1317), atan c_j in two parts (j pi/12 plus the
+// first-order shift of the rounded c_j) and the last term by its Taylor series
+// to r^17 (truncation < 1e-18 relative); pi/2 in two parts. Within ~2 ulp.
+__device__ inline double atan_fast(double x)
+{
+    const double a = fabs(x);
+    const bool inv = a > 1.0;
+    const double y = inv ? rcp_pos(a) : a;
+    const int j = (y > 0.13165249758739583) + (y > 0.41421356237309503) + (y > 0.7673269879789604);
+    const double c = j == 0 ? 0.0 : (j == 1 ? 0.2679491924311227 : (j == 2 ? 0.5773502691896257 : 1.0));
+    const double bh = j == 0 ? 0.0 : (j == 1 ? 0.2617993877991494 : (j == 2 ? 0.5235987755982988 : 0.7853981633974483));
+    const double bl = j == 0 ? 0.0 : (j == 1 ? 1.8752499114174e-17 : (j == 2 ? 3.2330503585443845e-17 : 3.061616997868383e-17));
+    const double num = y - c, den = fma(y, c, 1.0);
+    const double id = rcp_pos(den);
+    double r = num * id;
+    r = fma(fma(-r, den, num), id, r);
+    const double r2 = r * r;
+    double q = 1.0 / 17.0;
+    q = fma(q, r2, -1.0 / 15.0);
+    q = fma(q, r2, 1.0 / 13.0);
+    q = fma(q, r2, -1.0 / 11.0);
+    q = fma(q, r2, 1.0 / 9.0);
+    q = fma(q, r2, -1.0 / 7.0);
+    q = fma(q, r2, 1.0 / 5.0);
+    q = fma(q, r2, -1.0 / 3.0);
+    const double at = bh + (bl + fma(r * r2, q, r));
+    const double res = inv ? 1.5707963267948966 - (at - 6.123233995736766e-17) : at;
+    return copysign(res, x);
+}
+
 // A polynomial coefficient as a scalar-register operand at its point of use
 // (rebuilt there by two scalar moves): the FMA chains below then take it as
 // their addend directly, instead of copying a vector-register copy of the

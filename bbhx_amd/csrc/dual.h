@@ -276,6 +276,17 @@ __device__ inline Dual<N> log_pos(const Dual<N>& x)
     return chain(log_pos(x.v), rcp_pos(x.v), x);
 }
 template <int N>
+__device__ inline Dual<N> exp_fast(const Dual<N>& x)
+{
+    const double e = exp_fast(x.v);
+    return chain(e, e, x);
+}
+template <int N>
+__device__ inline Dual<N> atan_fast(const Dual<N>& x)
+{
+    return chain(atan_fast(x.v), rcp_pos(1.0 + x.v * x.v), x);
+}
+template <int N>
 __device__ inline Dual<N> exp(const Dual<N>& x)
 {
     const double e = ::exp(x.v);

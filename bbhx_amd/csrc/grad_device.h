@@ -207,7 +207,7 @@ __device__ inline void phase_imr_tan(const phm::PhmRecT<D4>& r, const D4& xp, co
         lin(L, r.alpha2, -iMf);
         lin(L, r.alpha3, (4.0 / 3.0) * f075);
         lin(L, r.alpha1, x);
-        const D4 phi_mrd = lin_at(L, xp) + r.alpha4 * r.p_Rholm * atan(q);
+        const D4 phi_mrd = lin_at(L, xp) + r.alpha4 * r.p_Rholm * atan_fast(q);
         Lin C = lin0();
         lin(C, r.C1MRD);
         lin(C, r.C2MRD, x);

@@ -176,7 +176,7 @@ __device__ inline S amp_mrd(const PhmRecT<S>& r, nd<S> f)
     const S fDMg3 = r.a_fDM * r.a_g3;
     const S d = f - r.a_fRD;
     const S inv = rcp_pos(fDMg3);
-    return exp(-(d * r.a_g2) * inv) * (fDMg3 * r.a_g1) * rcp_pos(d * d + fDMg3 * fDMg3);
+    return exp_fast(-(d * r.a_g2) * inv) * (fDMg3 * r.a_g1) * rcp_pos(d * d + fDMg3 * fDMg3);
 }
 
 template <class S>
@@ -325,7 +325,7 @@ __device__ inline S phi_mrd(const PhmRecT<S>& r, nd<S> f, nd<S> iMf)
     const S sf = sqrt(f);
     const S f075 = sqrt(f * sf);
     return -(r.alpha2 * iMf) + (4.0 / 3.0) * (r.alpha3 * f075) + r.alpha1 * f +
-           r.alpha4 * r.p_Rholm * atan((f - r.alpha5 * r.p_fRD) * rcp_pos(r.p_Rholm * r.p_fDM * r.p_Taulm));
+           r.alpha4 * r.p_Rholm * atan_fast((f - r.alpha5 * r.p_fRD) * rcp_pos(r.p_Rholm * r.p_fDM * r.p_Taulm));
 }
 
 template <class S>
