@@ -18,18 +18,26 @@ using D4 = Dual<NI>;
 constexpr int NGRAD = 11;
 enum { G_LNM, G_Q, G_CHI1, G_CHI2, G_DIST, G_PHI, G_COSI, G_LAM, G_SINB, G_PSI, G_TREF };
 
+// The frame (u, v, k) is orthonormal, so its derivative in any sky angle p is a
+// rotation: dk/dp = a u + b v, du/dp = -a k + c v, dv/dp = -b k - c u with
+// a = u.dk/dp, b = v.dk/dp, c = v.du/dp (psi: a = b = 0, c = 1). rot[q] holds
+// (a, b, c) for p = lam (q = 0) and beta (q = 1); the derivative of any
+// projection n.k, n.u, n.v then follows from the three projections.
 struct GradResp {
     double u[3], v[3], k[3];
-    double dk[2][3], du[2][3], dv[2][3];  // d/d lam, d/d beta
-    cd Yp, Yc, dYp, dYc;                  // dY/d(cos inc)
-    double m;                             // azimuthal number
-    double inv_cosb;                      // d beta / d(sin beta)
+    double rot[2][3];     // (a, b, c) for lam, beta
+    cd Yp, Yc, dYp, dYc;  // dY/d(cos inc)
+    double m;             // azimuthal number
+    double inv_cosb;      // d beta / d(sin beta)
 };
 
 struct GradRec {
     phm::PhmRecT<D4> w;
     GradResp s;
 };
+
+// Re(a b)
+__device__ inline double re_mul(const cd& a, const cd& b) { return fma(a.re, b.re, -(a.im * b.im)); }
 
 __device__ inline void grad_resp_setup(GradResp& r, int l, int m, double phi, double inc, double lam, double beta,
                                        double psi)
@@ -44,26 +52,25 @@ __device__ inline void grad_resp_setup(GradResp& r, int l, int m, double phi, do
     r.v[0] = -cl * cp * sb - sl * sp;
     r.v[1] = -cp * sb * sl + cl * sp;
     r.v[2] = cb * cp;
-    // d/d lam
-    r.dk[0][0] = cb * sl;
-    r.dk[0][1] = -cb * cl;
-    r.dk[0][2] = 0.0;
-    r.du[0][0] = cp * cl + sl * sb * sp;
-    r.du[0][1] = sl * cp - sb * cl * sp;
-    r.du[0][2] = 0.0;
-    r.dv[0][0] = sl * cp * sb - cl * sp;
-    r.dv[0][1] = -cp * sb * cl - sl * sp;
-    r.dv[0][2] = 0.0;
-    // d/d beta
-    r.dk[1][0] = sb * cl;
-    r.dk[1][1] = sb * sl;
-    r.dk[1][2] = -cb;
-    r.du[1][0] = -cl * cb * sp;
-    r.du[1][1] = -cb * sl * sp;
-    r.du[1][2] = -sb * sp;
-    r.dv[1][0] = -cl * cp * cb;
-    r.dv[1][1] = -cp * cb * sl;
-    r.dv[1][2] = -sb * cp;
+    // dk and du in lam (q = 0) and beta (q = 1), projected on the frame
+    double dk[2][3], du[2][3];
+    dk[0][0] = cb * sl;
+    dk[0][1] = -cb * cl;
+    dk[0][2] = 0.0;
+    du[0][0] = cp * cl + sl * sb * sp;
+    du[0][1] = sl * cp - sb * cl * sp;
+    du[0][2] = 0.0;
+    dk[1][0] = sb * cl;
+    dk[1][1] = sb * sl;
+    dk[1][2] = -cb;
+    du[1][0] = -cl * cb * sp;
+    du[1][1] = -cb * sl * sp;
+    du[1][2] = -sb * sp;
+    for (int q = 0; q < 2; q++) {
+        r.rot[q][0] = resp::dot3(r.u, dk[q]);
+        r.rot[q][1] = resp::dot3(r.v, dk[q]);
+        r.rot[q][2] = resp::dot3(r.v, du[q]);
+    }
     r.inv_cosb = 1.0 / cb;
     r.m = m;
     // Y+ and Yx as resp_setup forms them, and their derivatives in inc
@@ -284,8 +291,8 @@ __device__ inline void phase_imr_tan(const phm::PhmRecT<D4>& r, const D4& xp, co
     dph = lin_at(Ld, xp) + lin_at(Ls, xp) * r.p_etaInv;
 }
 
-// phm::pn_habs: |H(v)| with its polynomial parts as sums in v
-__device__ inline D4 pn_habs_tan(const phm::PhmRecT<D4>& r, const D4& v)
+// phm::pn_habs2: |H(v)|^2 with its polynomial parts as sums in v
+__device__ inline D4 pn_habs2_tan(const phm::PhmRecT<D4>& r, const D4& v)
 {
     const X1 x = X1::seed(v.v, 0);
     const X1 x2 = x * x, x3 = x * x2, x4 = x2 * x2;
@@ -297,7 +304,7 @@ __device__ inline D4 pn_habs_tan(const phm::PhmRecT<D4>& r, const D4& v)
     Lin Li = lin0();
     lin(Li, r.pn_c4i, x4);
     const D4 re = lin_at(Lr, v), im = lin_at(Li, v);
-    return sqrt(re * re + im * im);
+    return re * re + im * im;
 }
 
 // phm::hm_amp_factor
@@ -305,12 +312,12 @@ __device__ inline D4 hm_amp_factor_tan(const phm::PhmRecT<D4>& r, const D4& Mf_w
 {
     if (r.ell == 2 && r.mm == 2) return D4(1.0);
     const D4 v1 = cbrt_pos(r.pn_vscale * Mf_wf);
-    const D4 h1 = pn_habs_tan(r, v1);
+    const D4 h1 = pn_habs2_tan(r, v1);
     if (h1.v == 0.0) return D4(0.0);
-    if (!(Mf_wf.v > r.amp_fi.v)) return h1;
+    if (!(Mf_wf.v > r.amp_fi.v)) return sqrt(h1);
     const D4 v2 = v1 * r.pn_v2ratio;
     const D4 v3 = cbrt_pos(r.pn_vscale * Mf_amp);
-    return h1 * pn_habs_tan(r, v3) * rcp_pos(pn_habs_tan(r, v2));
+    return sqrt(h1 * pn_habs2_tan(r, v3) * rcp_pos(pn_habs2_tan(r, v2)));
 }
 
 // phm::phm_eval<D4>: amplitude, phase and t(f) with their intrinsic tangents
@@ -398,35 +405,37 @@ __device__ inline void grad_setup_part(GradRec& g, const double* __restrict__ pa
 
 __device__ inline double dsinc(double a, double cos_a, double sinc_a) { return a == 0.0 ? 0.0 : (cos_a - sinc_a) / a; }
 
-// Gradient contributions of the waveform and response of one (frequency, mode)
-// to acc[11], for a log-likelihood whose first-order change is
+// Gradient contributions of the response of one (frequency, mode) to the
+// extrinsic entries of acc[11] (d, phi_ref, cos inc, lam, sin beta, psi,
+// t_ref), for a log-likelihood whose first-order change is
 //   d logL = a_A dA + a_phi d phi + Re sum_c W_c dT_c,
 // where A is the amplitude, phi the phase (PhenomHM phase + 2 pi f (t_ref - t0)
 // + the orbital Doppler phase) and T_c the TDI transfer functions at f and
 // t = t(f) + t_ref. weights(T, E, A, W, a_A, a_phi) supplies W, a_A, a_phi once
-// T, E = e^{i phi} and A are known. use_window: contribute only if t lies in
-// the observation window (the heterodyned form's rule for its nodes).
+// T, E = e^{i phi} and A are known. Av, Phv, Tfv: the PhenomHM amplitude, phase
+// and t(f) at f. On return (true) the intrinsic parameters' contribution is
+//   a_t dt(f) + a_A dA + a_phi dPhi   (a_t = dlogL/dt at fixed phase),
+// so the three scalars are all the intrinsic tangents need. use_window: return
+// false, contributing nothing, if t lies outside the observation window (the
+// heterodyned form's rule for its nodes).
 template <class Weights, class E>
-__device__ inline void grad_point_core(const GradRec& G, const resp::OrbitTab& o, double f, const fused::Window& win,
-                                       bool use_window, Weights weights, int TDItag, int rescaled, int tdi2,
-                                       double dist, double acc[NGRAD], const E* __restrict__ tab)
+__device__ inline bool grad_resp_part(const GradResp& R, const resp::OrbitTab& o, double f, double Av, double Phv,
+                                      double Tfv, const fused::Window& win, bool use_window, Weights weights,
+                                      int TDItag, int rescaled, int tdi2, double dist, double acc[NGRAD],
+                                      const E* __restrict__ tab, double& a_t, double& a_A, double& a_phi)
 {
     auto sincos_t = [&](double th, double* s_, double* c_) {
         const cd e = cexpi_tab(th, tab);
         *s_ = e.im;
         *c_ = e.re;
     };
-    D4 A, Ph, Tf;
-    phm_eval_tan(G.w, f, A, Ph, Tf);
-    const double t = Tf.v + win.t_ref;
-    if (use_window && !fused::in_window(win, t)) return;
-    const GradResp& R = G.s;
+    const double t = Tfv + win.t_ref;
+    if (use_window && !fused::in_window(win, t)) return false;
 
-    // orbit projections (values), their time derivatives, and the spacecraft
-    // projections' derivatives in lam, beta, from the packed row of the table
-    // (values P and differences D to the next row: P(t) = P + a D, dP/dt =
-    // D / dt); the links' derivatives in lam, beta are formed later, link by
-    // link, to keep fewer values live
+    // orbit projections on the frame and their time derivatives, from the
+    // packed row of the table (values P and differences D to the next row:
+    // P(t) = P + a D, dP/dt = D / dt); derivatives in the sky angles follow
+    // from the projections (the frame's rotation)
     int w = int(floor(t * o.inv_dt));
     w = w < 0 ? 0 : (w > o.N - 2 ? o.N - 2 : w);
     const double a = (t - o.dt * w) * o.inv_dt;
@@ -438,13 +447,14 @@ __device__ inline void grad_point_core(const GradRec& G, const resp::OrbitTab& o
             v[c] = fma(d[c], a, row[3 * q + c]);
         }
     };
-    double kp[3], dkp[3][3];
+    double kp[3], dkp_t[3], up[3], vp[3];
     for (int i = 0; i < 3; i++) {
         double v[3], d[3];
         vec(i, v, d);
         kp[i] = resp::dot3(R.k, v);
-        dkp[0][i] = resp::dot3(R.k, d) * o.inv_dt;
-        for (int q = 0; q < 2; q++) dkp[1 + q][i] = resp::dot3(R.dk[q], v);
+        dkp_t[i] = resp::dot3(R.k, d) * o.inv_dt;
+        up[i] = resp::dot3(R.u, v);
+        vp[i] = resp::dot3(R.v, v);
     }
     double kn[3], un[3], vn[3], dkn_t[3], dun_t[3], dvn_t[3];
     for (int j = 0; j < 3; j++) {
@@ -464,9 +474,10 @@ __device__ inline void grad_point_core(const GradRec& G, const resp::OrbitTab& o
     const double c2f = 2. * PI / C_SI * f;
     const double pRd = c2f * (R.k[0] * (AU_SI * ca) + R.k[1] * (AU_SI * sa));
     const double pRd_t = c2f * AU_SI * OMEGA0 * (-R.k[0] * sa + R.k[1] * ca);
-    const double pRd_l = c2f * AU_SI * (R.dk[0][0] * ca + R.dk[0][1] * sa);
-    const double pRd_b = c2f * AU_SI * (R.dk[1][0] * ca + R.dk[1][1] * sa);
-    const double phase = Ph.v + ((f * win.dt_shift) * 2) * PI + pRd;
+    // the Doppler phase's derivative in a sky angle is c2f (a u.R + b v.R)
+    const double pRd_u = c2f * AU_SI * (R.u[0] * ca + R.u[1] * sa);
+    const double pRd_v = c2f * AU_SI * (R.v[0] * ca + R.v[1] * sa);
+    const double phase = Phv + ((f * win.dt_shift) * 2) * PI + pRd;
 
     // single-link terms as resp_eval forms them
     const double x = PI * f * o.L / C_SI;
@@ -518,7 +529,6 @@ __device__ inline void grad_point_core(const GradRec& G, const resp::OrbitTab& o
     cd T[3];
     cd bm[3], bp[3];  // beta of Gm[L], Gp[L]
     const cd E1 = cexpi_tab(phase, tab);
-    const double Av = A.v;
     cd W[3];
     double aA = 0.0, aPh = 0.0;
     if (TDItag == 1) {
@@ -564,24 +574,22 @@ __device__ inline void grad_point_core(const GradRec& G, const resp::OrbitTab& o
         bm[2] = mk(0.0, 0.0) - wA * z + wE * (mk(2.0, 0.0) + z) - wT;  // G12
     }
 
-    // Q_d = sum_ij beta_ij dG_ij for d = t, lam, beta (0..2), psi (3), cos inc (4)
-    cd Q[5] = {mk(0.0, 0.0), mk(0.0, 0.0), mk(0.0, 0.0), mk(0.0, 0.0), mk(0.0, 0.0)};
+    // sum_ij beta_ij dG_ij in t (Qt) and, through the projections, in the
+    // sky angles and the Y factors. Per link L with G = g s e (g = i x nHn,
+    // nHn = Yp U + Yc V, U = un^2 - vn^2, V = 2 un vn):
+    //   dG = ixeB dnHn + egC dkn + egB (dkp_ka + dkp_kb)
+    // (ixeB = i x e B, egC = e g C x, egB = e g B i x / L with B, C the sinc
+    // terms' adjoint sums). With the frame's rotation (a, b, c) a sky angle
+    // gives dU = 2 kn (b vn - a un) + 2 c V, dV = -2 kn (a vn + b un) - 2 c U,
+    // dkn = a un + b vn, dkp = a up + b vp, so every sky derivative is
+    //   a Sa + b Sb + c Spsi
+    // with Spsi the psi derivative (a = b = 0, c = 1) and Sa, Sb sums over the
+    // links; cos inc enters only through (Yp, Yc).
+    cd Qt = mk(0.0, 0.0);
+    cd SU = mk(0.0, 0.0), SV = mk(0.0, 0.0), Sku = mk(0.0, 0.0), Skv = mk(0.0, 0.0);
+    cd Tu = mk(0.0, 0.0), Tv = mk(0.0, 0.0);
     const cd ix = mk(0.0, x);
     for (int L = 0; L < 3; L++) {
-        // this link's projections differentiated in t (0), lam (1), beta (2)
-        double dknL[3], dunL[3], dvnL[3];
-        dknL[0] = dkn_t[L];
-        dunL[0] = dun_t[L];
-        dvnL[0] = dvn_t[L];
-        {
-            double v[3], d[3];
-            vec(3 + L, v, d);
-            for (int q = 0; q < 2; q++) {
-                dknL[1 + q] = resp::dot3(R.dk[q], v);
-                dunL[1 + q] = resp::dot3(R.du[q], v);
-                dvnL[1 + q] = resp::dot3(R.dv[q], v);
-            }
-        }
         const cd BL = bm[L] * sm[L] + bp[L] * sp[L];
         const cd CL = bp[L] * dsp[L] - bm[L] * dsm[L];
         const cd eB = eL[L] * BL;
@@ -589,32 +597,53 @@ __device__ inline void grad_point_core(const GradRec& G, const resp::OrbitTab& o
         const cd egC = eg * CL * x;
         const cd egB = eg * BL * ix * o.inv_L;
         const cd ixeB = ix * eB;
-        for (int d = 0; d < 3; d++) {
-            const double dU = 2.0 * (un[L] * dunL[d] - vn[L] * dvnL[d]);
-            const double dV = 2.0 * (dunL[d] * vn[L] + un[L] * dvnL[d]);
-            const cd dnHn = R.Yp * dU + R.Yc * dV;
-            Q[d] += ixeB * dnHn + egC * dknL[d] + egB * (dkp[d][ka[L]] + dkp[d][kb[L]]);
-        }
-        // psi: du = v, dv = -u
-        {
-            const double dU = 4.0 * un[L] * vn[L];
-            const double dV = 2.0 * (vn[L] * vn[L] - un[L] * un[L]);
-            Q[3] += ixeB * (R.Yp * dU + R.Yc * dV);
-        }
-        // cos inc: only the Y factors
-        Q[4] += ixeB * (R.dYp * (un[L] * un[L] - vn[L] * vn[L]) + R.dYc * (2.0 * un[L] * vn[L]));
+        const double U = un[L] * un[L] - vn[L] * vn[L], V = 2.0 * un[L] * vn[L];
+        const double dU = 2.0 * (un[L] * dun_t[L] - vn[L] * dvn_t[L]);
+        const double dV = 2.0 * (dun_t[L] * vn[L] + un[L] * dvn_t[L]);
+        Qt += ixeB * (R.Yp * dU + R.Yc * dV) + egC * dkn_t[L] + egB * (dkp_t[ka[L]] + dkp_t[kb[L]]);
+        SU += ixeB * U;
+        SV += ixeB * V;
+        Sku += ixeB * (kn[L] * un[L]);
+        Skv += ixeB * (kn[L] * vn[L]);
+        Tu += egC * un[L] + egB * (up[ka[L]] + up[kb[L]]);
+        Tv += egC * vn[L] + egB * (vp[ka[L]] + vp[kb[L]]);
     }
+    // Re of: Spsi = 2 (Yp SV - Yc SU), Sa = Tu - 2 (Yp Sku + Yc Skv),
+    // Sb = Tv + 2 (Yp Skv - Yc Sku), Scos = dYp SU + dYc SV
+    const double Spsi = 2.0 * (re_mul(R.Yp, SV) - re_mul(R.Yc, SU));
+    const double Sa = Tu.re - 2.0 * (re_mul(R.Yp, Sku) + re_mul(R.Yc, Skv)) + aPh * pRd_u;
+    const double Sb = Tv.re + 2.0 * (re_mul(R.Yp, Skv) - re_mul(R.Yc, Sku)) + aPh * pRd_v;
+    const double Scos = re_mul(R.dYp, SU) + re_mul(R.dYc, SV);
 
     cd WT = mk(0.0, 0.0);
     for (int c = 0; c < 3; c++) WT += W[c] * T[c];
-    for (int k = 0; k < NI; k++) acc[k] += Q[0].re * Tf.d[k] + aA * A.d[k] + aPh * (Ph.d[k] + pRd_t * Tf.d[k]);
+    a_t = Qt.re + aPh * pRd_t;
+    a_A = aA;
+    a_phi = aPh;
     acc[G_DIST] += -aA * Av / dist;
     acc[G_PHI] += -R.m * WT.im;  // T ~ e^{i m phi_ref}
-    acc[G_COSI] += Q[4].re;
-    acc[G_LAM] += Q[1].re + aPh * pRd_l;
-    acc[G_SINB] += (Q[2].re + aPh * pRd_b) * R.inv_cosb;
-    acc[G_PSI] += Q[3].re;
-    acc[G_TREF] += Q[0].re + aPh * (2. * PI * f + pRd_t);
+    acc[G_COSI] += Scos;
+    acc[G_LAM] += fma(R.rot[0][0], Sa, fma(R.rot[0][1], Sb, R.rot[0][2] * Spsi));
+    acc[G_SINB] += fma(R.rot[1][0], Sa, fma(R.rot[1][1], Sb, R.rot[1][2] * Spsi)) * R.inv_cosb;
+    acc[G_PSI] += Spsi;
+    acc[G_TREF] += Qt.re + aPh * (2. * PI * f + pRd_t);
+    return true;
+}
+
+// Both parts at one (frequency, mode): the waveform's values from the record
+// with tangents.
+template <class Weights, class E>
+__device__ inline void grad_point_core(const GradRec& G, const resp::OrbitTab& o, double f, const fused::Window& win,
+                                       bool use_window, Weights weights, int TDItag, int rescaled, int tdi2,
+                                       double dist, double acc[NGRAD], const E* __restrict__ tab)
+{
+    D4 A, Ph, Tf;
+    phm_eval_tan(G.w, f, A, Ph, Tf);
+    double a_t, a_A, a_phi;
+    if (!grad_resp_part(G.s, o, f, A.v, Ph.v, Tf.v, win, use_window, weights, TDItag, rescaled, tdi2, dist, acc, tab,
+                        a_t, a_A, a_phi))
+        return;
+    for (int k = 0; k < NI; k++) acc[k] += a_t * Tf.d[k] + a_A * A.d[k] + a_phi * Ph.d[k];
 }
 
 

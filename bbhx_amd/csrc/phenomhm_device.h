@@ -231,12 +231,12 @@ __device__ inline void pn_habs_coeffs(int l, int m, S eta, S delta, S Xs, S Xa, 
 // |H_lm(v)| from the record's coefficients: one polynomial for every mode, so
 // the evaluation has no branch on the mode
 template <class S>
-__device__ inline S pn_habs(const PhmRecT<S>& r, nd<S> v)
+__device__ inline S pn_habs2(const PhmRecT<S>& r, nd<S> v)
 {
     const S re = v * (r.pn_c1 + v * (r.pn_c2 + v * (r.pn_c3 + v * r.pn_c4)));
     const S v2 = v * v;
     const S im = (v2 * v2) * r.pn_c4i;
-    return sqrt(re * re + im * im);
+    return re * re + im * im;
 }
 
 // The HM amplitude factor beta * HM1 / HM2 of get_amp (PhenomHMWaveform.cu:2473-2511).
@@ -252,13 +252,14 @@ __device__ inline S hm_amp_factor(const PhmRecT<S>& r, nd<S> Mf_wf, nd<S> Mf_amp
 {
     if (r.ell == 2 && r.mm == 2) return 1.0;
     const S v1 = cbrt_pos(r.pn_vscale * Mf_wf);
-    const S h1 = pn_habs(r, v1);
+    const S h1 = pn_habs2(r, v1);  // |H(v1)|^2
     if (h1 == 0.0) return 0.0;
-    if (!(Mf_wf > r.amp_fi)) return h1;
+    if (!(Mf_wf > r.amp_fi)) return sqrt(h1);
     const S v2 = v1 * r.pn_v2ratio;
     const S v3 = cbrt_pos(r.pn_vscale * Mf_amp);
-    // |H(v2)| > 0 wherever |H(v1)| > 0 (the same factors of eta and delta vanish)
-    return h1 * pn_habs(r, v3) * rcp_pos(pn_habs(r, v2));
+    // |H(v2)| > 0 wherever |H(v1)| > 0 (the same factors of eta and delta
+    // vanish); one square root of the squared magnitudes' ratio
+    return sqrt(h1 * pn_habs2(r, v3) * rcp_pos(pn_habs2(r, v2)));
 }
 
 // ---------------------------------------------------------------- phase
