@@ -112,6 +112,54 @@ __device__ inline bool in_window(const Window& w, double tf)
     return (tf >= w.ts) && ((tf <= w.te) || (w.te <= 0.0));
 }
 
+// The heterodyned sums (Likelihood.cu:91-103) per segment j of nodes j-1, j
+// with r1 = (r_j - r_{j-1}) / (f_j - f_{j-1}) and r0 = r at the segment's
+// midpoint, (r_j + r_{j-1}) / 2, are
+//   d_h = sum_j A0 conj(r0) + A1 conj(r1),
+//   h_h = sum_j B0 |r0|^2 + 2 B1 Re(r0 conj(r1)),
+// and since Re(r0 conj(r1)) = (|r_j|^2 - |r_{j-1}|^2) / (2 (f_j - f_{j-1})) and
+// |r0|^2 = (|r_j|^2 + |r_{j-1}|^2 + 2 Re(r_j conj(r_{j-1}))) / 4, per node n
+//   d_h = sum_n conj(r_n) W_n,
+//   h_h = sum_n |r_n|^2 V_n + Re(r_n conj(r_{n-1})) U_n
+// with W_n = [A0/2 + A1 i]_n + [A0/2 - A1 i]_{n+1}, V_n = [B0/4 + B1 i]_n +
+// [B0/4 - B1 i]_{n+1}, U_n = [B0/2]_n, i = 1 / (f_j - f_{j-1}), each bracket the
+// segment's term (0 for a segment the sums skip). The weights depend on the
+// nodes and data alone, so they are formed once per call; q = 1 / h0 of the
+// node (r = h q). One entry per (channel c, node n) at c * K + n.
+struct NodeW {
+    cd w, v, u, q;
+};
+
+// Entry idx (< 3 K) of the node weights from the heterodyne constants
+// dc[(q * 3 + c) * K + j] (A0, A1, B0, B1 of segment j) and the reference h0;
+// segments j < jend whose lower node is not at f = 0 are summed.
+__device__ inline void node_weight(NodeW* __restrict__ nw, const double* __restrict__ freqs,
+                                   const cd* __restrict__ h0, const cd* __restrict__ dc, int K, int jend, int idx)
+{
+    if (idx >= 3 * K) return;
+    const int c = idx / K, n = idx - c * K;
+    auto valid = [&](int j) { return j >= 1 && j < jend && freqs[j - 1] != 0.0; };
+    cd W = mk(0.0, 0.0), V = mk(0.0, 0.0), U = mk(0.0, 0.0);
+    if (valid(n)) {
+        const double inv = rcp_pos(freqs[n] - freqs[n - 1]);
+        const cd A0 = dc[(0 * 3 + c) * K + n], A1 = dc[(1 * 3 + c) * K + n];
+        const cd B0 = dc[(2 * 3 + c) * K + n], B1 = dc[(3 * 3 + c) * K + n];
+        W = 0.5 * A0 + A1 * inv;
+        V = 0.25 * B0 + B1 * inv;
+        U = 0.5 * B0;
+    }
+    if (n + 1 < K && valid(n + 1)) {
+        const double inv = rcp_pos(freqs[n + 1] - freqs[n]);
+        const cd A0 = dc[(0 * 3 + c) * K + n + 1], A1 = dc[(1 * 3 + c) * K + n + 1];
+        const cd B0 = dc[(2 * 3 + c) * K + n + 1], B1 = dc[(3 * 3 + c) * K + n + 1];
+        W = W + (0.5 * A0 - A1 * inv);
+        V = V + (0.25 * B0 - B1 * inv);
+    }
+    const cd d = h0[c * K + n];
+    const double iden = rcp_pos(d.re * d.re + d.im * d.im);
+    nw[idx] = NodeW{W, V, U, mk(d.re * iden, -d.im * iden)};
+}
+
 // Residual pass of one template by one workgroup: thread t owns nodes t,
 // t + blockDim.x, ...; r = h / h0 is written to rs[c * K + j] (LDS).
 // Per node the modes are summed before the TDI combination: with link terms
@@ -133,7 +181,7 @@ __device__ inline bool in_window(const Window& w, double tf)
 // two modes' Y factors weighted by their amplitudes.
 template <class E>
 __device__ inline void residuals(cd* rs, const FusedRec* __restrict__ rb, const ModeList& ml, int M,
-                                 const Window& win, const double* __restrict__ freqs, const cd* __restrict__ h0,
+                                 const Window& win, const double* __restrict__ freqs, const NodeW* __restrict__ nw,
                                  int K, const resp::OrbitTab& orb, int TDItag, int rescaled, int tdi2,
                                  const E* __restrict__ tab)
 {
@@ -182,13 +230,8 @@ __device__ inline void residuals(cd* rs, const FusedRec* __restrict__ rb, const 
         }
         cd T[3];
         resp::tdi_combine(S, nr, TDItag, rescaled, tdi2, T);
-        // r = h / h0 with h = conj(T) (likelihood.py:603)
-        for (int c = 0; c < 3; c++) {
-            const cd d = h0[c * K + j];
-            const cd h = conj(T[c]);
-            const double iden = rcp_pos(d.re * d.re + d.im * d.im);
-            rs[c * K + j] = mk((h.re * d.re + h.im * d.im) * iden, (h.im * d.re - h.re * d.im) * iden);
-        }
+        // r = h / h0 with h = conj(T) (likelihood.py:603), as h (1 / h0)
+        for (int c = 0; c < 3; c++) rs[c * K + j] = conj(T[c]) * nw[c * K + j].q;
     }
 }
 
@@ -211,49 +254,43 @@ __device__ inline void etab_to_lds(double2* tab)
     for (int j = threadIdx.x; j < 129; j += blockDim.x) tab[j] = make_double2(ETAB[j].x, ETAB[j].y);
 }
 
-// Heterodyned sums (Likelihood.cu:52-111) over the segments j = t+1,
-// t+1 + blockDim.x, ... < jend of thread t; segment j joins nodes j-1 and j.
-// Returns the workgroup totals (d_h re, im, h_h re, im) in every thread: each
-// wavefront reduces its lanes, then the (at most 8) wavefront totals are added
-// in wavefront order.
-__device__ inline void segment_sums(const cd* rs, const double* __restrict__ freqs, const cd* __restrict__ dc, int K,
-                                    int jend, double out[4])
+// Heterodyned sums (Likelihood.cu:52-111) in the per-node form of NodeW over
+// the nodes j = t, t + blockDim.x, ... of thread t. Returns the workgroup
+// totals (d_h re, im, h_h re, im) in every thread: each wavefront reduces its
+// lanes, then the (at most 8) wavefront totals are added in wavefront order.
+__device__ inline void node_sums(const cd* rs, const NodeW* __restrict__ nw, int K, double out[4])
 {
     double l1r = 0.0, l1i = 0.0, l2r = 0.0, l2i = 0.0;
-    for (int j = threadIdx.x + 1; j < jend; j += blockDim.x) {
-        const double fj = freqs[j], fprev = freqs[j - 1];
-        if (fprev == 0.0) continue;
-        const double inv = rcp_pos(fj - fprev);  // nodes increase
-        const double fm = (fj + fprev) / 2.0;
+    for (int j = threadIdx.x; j < K; j += blockDim.x) {
         for (int c = 0; c < 3; c++) {
-            const cd cur = rs[c * K + j], prev = rs[c * K + j - 1];
-            const cd A0 = dc[(0 * 3 + c) * K + j], A1 = dc[(1 * 3 + c) * K + j];
-            const cd B0 = dc[(2 * 3 + c) * K + j], B1 = dc[(3 * 3 + c) * K + j];
-            const cd r1 = mk((cur.re - prev.re) * inv, (cur.im - prev.im) * inv);
-            const cd r0 = cur - r1 * (fj - fm);
-            const cd r1c = conj(r1);
-            const cd t1 = A0 * conj(r0) + A1 * r1c;
-            const double mag2 = r0.re * r0.re + r0.im * r0.im;
-            const cd t2 = B0 * mag2 + 2. * B1 * (r0 * r1c).re;
-            l1r += t1.re;
-            l1i += t1.im;
-            l2r += t2.re;
-            l2i += t2.im;
+            const NodeW& w = nw[c * K + j];
+            const cd r = rs[c * K + j];
+            // conj(r) W
+            l1r += fma(r.re, w.w.re, r.im * w.w.im);
+            l1i += fma(r.re, w.w.im, -(r.im * w.w.re));
+            const double m2 = fma(r.re, r.re, r.im * r.im);
+            double x = 0.0;
+            if (j > 0) {
+                const cd p = rs[c * K + j - 1];
+                x = fma(r.re, p.re, r.im * p.im);
+            }
+            l2r += fma(m2, w.v.re, x * w.u.re);
+            l2i += fma(m2, w.v.im, x * w.u.im);
         }
     }
     out[0] = wsum(l1r);
     out[1] = wsum(l1i);
     out[2] = wsum(l2r);
     out[3] = wsum(l2i);
-    const int nw = blockDim.x >> 6;
-    if (nw > 1) {
+    const int nwave = blockDim.x >> 6;
+    if (nwave > 1) {
         __shared__ double part[4][MAX_RESIDUAL_WAVES];
         if ((threadIdx.x & 63) == 0)
             for (int k = 0; k < 4; k++) part[k][threadIdx.x >> 6] = out[k];
         __syncthreads();
         for (int k = 0; k < 4; k++) {
             double s = part[k][0];
-            for (int i = 1; i < nw; i++) s += part[k][i];
+            for (int i = 1; i < nwave; i++) s += part[k][i];
             out[k] = s;
         }
     }

@@ -35,27 +35,36 @@ bool response_orbit_tab(const bbhx_response* r, resp::OrbitTab& o);
 
 namespace {
 
-__global__ __launch_bounds__(64) void k_fused_setup(FusedRec* __restrict__ recs, const double* __restrict__ params, int B, ModeList modes,
-                              int M, const double* __restrict__ qnm, int table_len, double dspin)
+// The records over SETUP_PARTS * ceil(B M / 64) blocks, then the node weights
+// over ceil(3 K / 64) more.
+__global__ __launch_bounds__(64) void k_fused_setup(FusedRec* __restrict__ recs, NodeW* __restrict__ nw,
+                                                    const double* __restrict__ params, int B, ModeList modes, int M,
+                                                    const double* __restrict__ qnm, int table_len, double dspin,
+                                                    const double* __restrict__ freqs, const cd* __restrict__ h0,
+                                                    const cd* __restrict__ dc, int K, int jend)
 {
+    const int nsb = SETUP_PARTS * ((B * M + 63) / 64);
+    if ((int)blockIdx.x >= nsb) {
+        node_weight(nw, freqs, h0, dc, K, jend, ((int)blockIdx.x - nsb) * 64 + threadIdx.x);
+        return;
+    }
     setup_split(recs, params, B, modes, M, qnm, table_len, dspin);
 }
 
 // One workgroup per template, of residual_waves(B, K) wavefronts (one at large
 // batches; several at small ones, so that the chip still holds two per SIMD).
 // Dynamic LDS holds the residuals r[3][K] of this template; static LDS its
-// records and the e^{i theta} table. The register budget is capped so that two
-// wavefronts share each SIMD (256 VGPRs each): 15 values live across the node
-// loop spill to scratch (12 MB written per launch at 4096 templates), which
-// costs less than the scalar-load waits of reading the records through the
-// scalar cache (no spills; 0.140 against 0.129 ms at 4096 x 128 nodes).
-// Capped at three per SIMD (168 VGPRs) 79 values spill and it takes 0.245 ms.
+// records and the e^{i theta} table (the records read with LDS broadcast loads:
+// measured faster than scalar loads, whose waits the SGPR budget serialises).
+// The register budget is capped so that two wavefronts share each SIMD (256
+// VGPRs each, no spills); capped at three per SIMD (168 VGPRs) values spill and
+// it takes twice as long.
 __global__ __launch_bounds__(64 * MAX_RESIDUAL_WAVES) __attribute__((amdgpu_waves_per_eu(2))) void k_fused_eval(
     double* __restrict__ d_h, double* __restrict__ h_h, double* __restrict__ ll, double d_d,
     const FusedRec* __restrict__ recs, const double* __restrict__ params, const double* __restrict__ t_start,
     const double* __restrict__ t_end, int B, ModeList ml, int M, const double* __restrict__ freqs,
-    const cd* __restrict__ h0, const cd* __restrict__ dc, int K, resp::OrbitTab orb, int TDItag, int rescaled,
-    int tdi2, int jend, double initial_t_val)
+    const NodeW* __restrict__ nw, int K, resp::OrbitTab orb, int TDItag, int rescaled, int tdi2,
+    double initial_t_val)
 {
     extern __shared__ cd rs[];  // [3][K]
     __shared__ double2 tab[129];
@@ -69,10 +78,10 @@ __global__ __launch_bounds__(64 * MAX_RESIDUAL_WAVES) __attribute__((amdgpu_wave
     __shared__ FusedRec srec[MAX_MODES];
     records_to_lds(srec, recs + (long)b * M, M);
     __syncthreads();
-    residuals(rs, srec, ml, M, win, freqs, h0, K, orb, TDItag, rescaled, tdi2, tab);
+    residuals(rs, srec, ml, M, win, freqs, nw, K, orb, TDItag, rescaled, tdi2, tab);
     __syncthreads();
     double sums[4];
-    segment_sums(rs, freqs, dc, K, jend, sums);
+    node_sums(rs, nw, K, sums);
     if (threadIdx.x == 0) {
         d_h[2 * b] = sums[0];
         d_h[2 * b + 1] = sums[1];
@@ -182,10 +191,10 @@ extern "C" int bbhx_hip_timing_read(double* ms_setup, double* ms_eval, int* ncal
 namespace {
 
 // The record set-up and evaluation launches of the fused forward, on records
-// 'recs' (numBinAll * numModes of them) that the caller provides; arguments
-// as bbhx_hip_hdyn_fused, already checked. 'timing' allows the optional event
-// timing (which may create events).
-int hdyn_fused_launch(FusedRec* recs, double* d_h, double* h_h, const double* params, const double* t_start,
+// 'recs' (numBinAll * numModes of them) and node weights 'nw' (3 K) that the
+// caller provides; arguments as bbhx_hip_hdyn_fused, already checked. 'timing'
+// allows the optional event timing (which may create events).
+int hdyn_fused_launch(FusedRec* recs, NodeW* nw, double* d_h, double* h_h, const double* params, const double* t_start,
                       const double* t_end, int numBinAll, const ModeList& ml, int numModes, const double* freqs,
                       const double* h0, const double* dataConstants, int K, const double* qnm, int table_len,
                       double dspin, double initial_t_val, const resp::OrbitTab& orb, int TDItag, int rescaled,
@@ -196,17 +205,16 @@ int hdyn_fused_launch(FusedRec* recs, double* d_h, double* h_h, const double* pa
     const bool timed = timing && timer_take3(ev);
     if (timing && g_timer.on && !timed) return ERR_LAUNCH;
     if (timed) (void)hipEventRecord(ev[0], st);
-    hipLaunchKernelGGL(k_fused_setup, dim3(SETUP_PARTS * ceil_div(nrec, 64)), dim3(64), 0, st, recs, params,
-                       numBinAll, ml, numModes, qnm, table_len, dspin);
+    const int jend = include_last_segment ? K : K - 1;
+    hipLaunchKernelGGL(k_fused_setup, dim3(SETUP_PARTS * ceil_div(nrec, 64) + ceil_div(3 * K, 64)), dim3(64), 0, st,
+                       recs, nw, params, numBinAll, ml, numModes, qnm, table_len, dspin, freqs,
+                       reinterpret_cast<const cd*>(h0), reinterpret_cast<const cd*>(dataConstants), K, jend);
     if (hipGetLastError() != hipSuccess) return ERR_LAUNCH;
     if (timed) (void)hipEventRecord(ev[1], st);
-    const int jend = include_last_segment ? K : K - 1;
     const size_t lds = sizeof(cd) * 3 * (size_t)K;
     hipLaunchKernelGGL(k_fused_eval, dim3(numBinAll), dim3(64 * residual_waves(numBinAll, K)), lds, st, d_h, h_h,
-                       logL, d_d, recs, params, t_start, t_end,
-                       numBinAll, ml, numModes, freqs, reinterpret_cast<const cd*>(h0),
-                       reinterpret_cast<const cd*>(dataConstants), K, orb, TDItag, rescaled, tdi2, jend,
-                       initial_t_val);
+                       logL, d_d, recs, params, t_start, t_end, numBinAll, ml, numModes, freqs, nw, K, orb, TDItag,
+                       rescaled, tdi2, initial_t_val);
     if (timed) (void)hipEventRecord(ev[2], st);
     return hipGetLastError() == hipSuccess ? OK : ERR_LAUNCH;
 }
@@ -234,7 +242,9 @@ constexpr long NOALLOC_MAX_RECORDS = BBHX_HIP_NOALLOC_MAX_RECORDS;
 
 // One copy per device, held until the process ends. In host code the name is
 // only a stand-in: its device address comes from hipGetSymbolAddress.
-static __device__ __align__(256) unsigned char scratch[sizeof(FusedRec) * NOALLOC_MAX_RECORDS];
+// The records, then the node weights for up to 1024 nodes.
+constexpr size_t NOALLOC_NW_OFFSET = sizeof(FusedRec) * NOALLOC_MAX_RECORDS;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_NW_OFFSET + sizeof(NodeW) * 3 * 1024];
 
 std::mutex g_scratch_mutex;
 std::map<int, void*> g_scratch_addr;  // device -> address of 'scratch' on that device
@@ -274,9 +284,13 @@ extern "C" int bbhx_hip_hdyn_fused(double* d_h, double* h_h, const double* param
     if (chk != OK) return chk < 0 ? OK : chk;
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     const int nrec = numBinAll * numModes;
-    FusedRec* recs = static_cast<FusedRec*>(workspace_get(WS_FUSED, sizeof(FusedRec) * (size_t)nrec, st));
-    if (!recs) return ERR_ALLOC;
-    return hdyn_fused_launch(recs, d_h, h_h, params, t_start, t_end, numBinAll, ml, numModes, freqs, h0,
+    // the records, then the node weights
+    char* ws = static_cast<char*>(
+        workspace_get(WS_FUSED, sizeof(FusedRec) * (size_t)nrec + sizeof(NodeW) * 3 * (size_t)K, st));
+    if (!ws) return ERR_ALLOC;
+    FusedRec* recs = reinterpret_cast<FusedRec*>(ws);
+    NodeW* nw = reinterpret_cast<NodeW*>(ws + sizeof(FusedRec) * (size_t)nrec);
+    return hdyn_fused_launch(recs, nw, d_h, h_h, params, t_start, t_end, numBinAll, ml, numModes, freqs, h0,
                              dataConstants, K, qnm, table_len, dspin, initial_t_val, orb, TDItag, rescaled, tdi2,
                              include_last_segment, logL, d_d, st, true);
 }
@@ -295,9 +309,11 @@ extern "C" int bbhx_hip_hdyn_fused_noalloc(double* d_h, double* h_h, const doubl
                                      freqs, h0, dataConstants, K, qnm, table_len, orbits, precision, orb, ml);
     if (chk != OK) return chk < 0 ? OK : chk;
     if ((long)numBinAll * numModes > NOALLOC_MAX_RECORDS) return ERR_CAPACITY;
-    FusedRec* recs = static_cast<FusedRec*>(scratch_address());
-    if (!recs) return ERR_LAUNCH;
-    return hdyn_fused_launch(recs, d_h, h_h, params, t_start, t_end, numBinAll, ml, numModes, freqs, h0,
+    char* base = static_cast<char*>(scratch_address());
+    if (!base) return ERR_LAUNCH;
+    FusedRec* recs = reinterpret_cast<FusedRec*>(base);
+    NodeW* nw = reinterpret_cast<NodeW*>(base + NOALLOC_NW_OFFSET);
+    return hdyn_fused_launch(recs, nw, d_h, h_h, params, t_start, t_end, numBinAll, ml, numModes, freqs, h0,
                              dataConstants, K, qnm, table_len, dspin, initial_t_val, orb, TDItag, rescaled, tdi2,
                              include_last_segment, logL, d_d, reinterpret_cast<hipStream_t>(stream), false);
 }

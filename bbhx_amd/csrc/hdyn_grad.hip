@@ -50,11 +50,19 @@ bool response_orbit_tab(const bbhx_response* r, resp::OrbitTab& o);
 
 namespace {
 
-// the double records of the forward pass, one lane per (template, mode)
-__global__ __launch_bounds__(64) void k_grad_setup_fwd(FusedRec* __restrict__ recs, const double* __restrict__ params,
-                                                       int B, ModeList modes, int M, const double* __restrict__ qnm,
-                                                       int table_len, double dspin)
+// the double records of the forward pass, one lane per (template, mode), then
+// the node weights (as k_fused_setup)
+__global__ __launch_bounds__(64) void k_grad_setup_fwd(FusedRec* __restrict__ recs, NodeW* __restrict__ nw,
+                                                       const double* __restrict__ params, int B, ModeList modes, int M,
+                                                       const double* __restrict__ qnm, int table_len, double dspin,
+                                                       const double* __restrict__ freqs, const cd* __restrict__ h0,
+                                                       const cd* __restrict__ dc, int K, int jend)
 {
+    const int nsb = SETUP_PARTS * ((B * M + 63) / 64);
+    if ((int)blockIdx.x >= nsb) {
+        node_weight(nw, freqs, h0, dc, K, jend, ((int)blockIdx.x - nsb) * 64 + threadIdx.x);
+        return;
+    }
     setup_split(recs, params, B, modes, M, qnm, table_len, dspin);
 }
 
@@ -142,8 +150,8 @@ __global__ __launch_bounds__(64 * MAX_RESIDUAL_WAVES) __attribute__((amdgpu_wave
     double* __restrict__ d_h, double* __restrict__ h_h, double* __restrict__ ll, double d_d, cd* __restrict__ gw,
     const FusedRec* __restrict__ recs, const double* __restrict__ params, const double* __restrict__ t_start,
     const double* __restrict__ t_end, int B, ModeList ml, int M, const double* __restrict__ freqs,
-    const cd* __restrict__ h0, const cd* __restrict__ dc, int K, resp::OrbitTab orb, int TDItag, int rescaled,
-    int tdi2, int jend, double initial_t_val)
+    const NodeW* __restrict__ nw, const cd* __restrict__ h0, const cd* __restrict__ dc, int K, resp::OrbitTab orb,
+    int TDItag, int rescaled, int tdi2, int jend, double initial_t_val)
 {
     extern __shared__ cd rs[];  // residuals [3][K]
     __shared__ double2 tab[129];
@@ -157,10 +165,10 @@ __global__ __launch_bounds__(64 * MAX_RESIDUAL_WAVES) __attribute__((amdgpu_wave
     __shared__ FusedRec srec[MAX_MODES];
     records_to_lds(srec, recs + (long)b * M, M);
     __syncthreads();
-    residuals(rs, srec, ml, M, win, freqs, h0, K, orb, TDItag, rescaled, tdi2, tab);
+    residuals(rs, srec, ml, M, win, freqs, nw, K, orb, TDItag, rescaled, tdi2, tab);
     __syncthreads();
     double sums[4];
-    segment_sums(rs, freqs, dc, K, jend, sums);
+    node_sums(rs, nw, K, sums);
     adjoint_weights(gw + (long)b * 3 * K, rs, freqs, h0, dc, K, jend);
     if (threadIdx.x == 0) {
         d_h[2 * b] = sums[0];
@@ -260,13 +268,17 @@ extern "C" int bbhx_hip_hdyn_fused_grad(double* d_h, double* h_h, double* grad, 
         return ERR_ARGS;
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     const int nrec = numBinAll * numModes;
-    FusedRec* recs = static_cast<FusedRec*>(workspace_get(WS_FUSED, sizeof(FusedRec) * (size_t)nrec, st));
+    // the double records, then the node weights (as the forward's workspace)
+    char* fws = static_cast<char*>(
+        workspace_get(WS_FUSED, sizeof(FusedRec) * (size_t)nrec + sizeof(NodeW) * 3 * (size_t)K, st));
+    FusedRec* recs = reinterpret_cast<FusedRec*>(fws);
+    NodeW* nw = reinterpret_cast<NodeW*>(fws + sizeof(FusedRec) * (size_t)nrec);
     GradRec* grecs = static_cast<GradRec*>(workspace_get(WS_GRAD, sizeof(GradRec) * (size_t)nrec, st));
     // adjoint weights (B, 3, K) complex, then partial gradients (B, M, 11)
     const size_t gw_bytes = sizeof(cd) * 3 * (size_t)K * numBinAll;
     char* scratch = static_cast<char*>(
         workspace_get(WS_GRAD_AUX, gw_bytes + sizeof(double) * NGRAD * (size_t)nrec, st));
-    if (!recs || !grecs || !scratch) return ERR_ALLOC;
+    if (!fws || !grecs || !scratch) return ERR_ALLOC;
     cd* gw = reinterpret_cast<cd*>(scratch);
     double* partial = reinterpret_cast<double*>(scratch + gw_bytes);
     // The records with tangents are needed only by k_grad_points; their set-up
@@ -284,15 +296,15 @@ extern "C" int bbhx_hip_hdyn_fused_grad(double* d_h, double* h_h, double* grad, 
                        params,
                        numBinAll, ml, numModes, qnm, table_len, dspin);
     if (hipEventRecord(side->join, side->stream) != hipSuccess) return ERR_LAUNCH;
-    hipLaunchKernelGGL(k_grad_setup_fwd, dim3(SETUP_PARTS * ceil_div((long)nrec, 64)), dim3(64), 0, st, recs, params,
-                       numBinAll, ml,
-                       numModes, qnm, table_len, dspin);
+    const int jend = include_last_segment ? K : K - 1;
+    hipLaunchKernelGGL(k_grad_setup_fwd, dim3(SETUP_PARTS * ceil_div((long)nrec, 64) + ceil_div(3 * K, 64)), dim3(64),
+                       0, st, recs, nw, params, numBinAll, ml, numModes, qnm, table_len, dspin, freqs,
+                       reinterpret_cast<const cd*>(h0), reinterpret_cast<const cd*>(dataConstants), K, jend);
     if (hipGetLastError() != hipSuccess) return ERR_LAUNCH;
     if (timed) (void)hipEventRecord(ev[1], st);
-    const int jend = include_last_segment ? K : K - 1;
     hipLaunchKernelGGL(k_grad_weights, dim3(numBinAll), dim3(64 * residual_waves(numBinAll, K)),
-                       sizeof(cd) * 3 * (size_t)K, st, d_h, h_h, logL, d_d, gw, recs,
-                       params, t_start, t_end, numBinAll, ml, numModes, freqs, reinterpret_cast<const cd*>(h0),
+                       sizeof(cd) * 3 * (size_t)K, st, d_h, h_h, logL, d_d, gw, recs, params, t_start, t_end,
+                       numBinAll, ml, numModes, freqs, nw, reinterpret_cast<const cd*>(h0),
                        reinterpret_cast<const cd*>(dataConstants), K, orb, TDItag, rescaled, tdi2, jend,
                        initial_t_val);
     if (hipStreamWaitEvent(st, side->join, 0) != hipSuccess) return ERR_LAUNCH;
