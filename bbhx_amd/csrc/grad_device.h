@@ -344,19 +344,22 @@ __device__ inline void phm_eval_tan(const phm::PhmRecT<D4>& r, double f, D4& amp
     A *= hm_amp_factor_tan(r, Mf_wf, Mf_amp);
     amp = A * r.amp_scale;
 
-    D4 sa, sb, c1, c2;
+    D4 sa, isa, sb, c1, c2;
     if (!(Mf_wf.v > r.fi.v)) {
         sa = r.ai;
+        isa = r.iai;
         sb = r.bi;
         c1 = D4(0.0);
         c2 = D4(0.0);
     } else if (!(Mf_wf.v > r.fr.v)) {
         sa = r.am;
+        isa = r.iam;
         sb = r.bm;
         c1 = r.PhDBconst;
         c2 = r.PhDBAterm;
     } else {
         sa = r.ar;
+        isa = r.iar;
         sb = r.br;
         c1 = r.PhDCconst;
         c2 = r.phaseC;
@@ -364,7 +367,7 @@ __device__ inline void phm_eval_tan(const phm::PhmRecT<D4>& r, double f, D4& amp
     const D4 Mf_ph = sa * Mf_wf + sb;
     D4 ph, dph;
     phase_imr_tan(r, Mf_ph, X1::seed(Mf_ph.v, 0), &p, Mf_ph.v == Mf_amp.v, ph, dph);
-    const D4 phi = r.cshift + ((ph * rcp_pos(sa) - c1) + c2);
+    const D4 phi = r.cshift + ((ph * isa - c1) + c2);
     phase = -r.t0 * (Mf_wf - r.Mf_ref) + (phi - (r.mm * r.phi0));
     const double INV_TWO_PI = 0.15915494309189535;
     tf = (r.M_sec * (dph * INV_TWO_PI)) - ((r.t0 * INV_TWO_PI) * r.M_sec);

@@ -154,6 +154,7 @@ struct PhmRecT {
     S q_init, q_23, q_13, q_13log, q_logv, q_m13, q_m23, q_m1, q_m53, q_1, q_43, q_53, q_2;
     // HM frequency maps: phase (a*, b*, fi, fr) and amplitude (amp_*)
     S ai, bi, am, bm, ar, br, fi, fr, PhDBconst, PhDCconst, PhDBAterm, phaseC;
+    S iai, iam, iar;  // 1 / ai, 1 / am, 1 / ar
     S amp_fi, amp_fr, amp_Ai, amp_Am, amp_Bm, amp_Br;
     S cshift;
     int ell, mm, phenomd, pad;
@@ -838,6 +839,7 @@ __device__ inline bool phm_setup(PhmRecT<S>& r, nd<S> m1_SI, nd<S> m2_SI, nd<S> 
 
     if (phenomd) {
         r.ai = 1.0; r.bi = 0.0; r.am = 1.0; r.bm = 0.0; r.ar = 1.0; r.br = 0.0;
+        r.iai = 1.0; r.iam = 1.0; r.iar = 1.0;
         r.fi = 1e300; r.fr = 1e300; r.PhDBconst = 0.0; r.PhDCconst = 0.0; r.PhDBAterm = 0.0; r.phaseC = 0.0;
         r.amp_fi = 1e300; r.amp_fr = 1e300; r.amp_Ai = 1.0; r.amp_Am = 1.0; r.amp_Bm = 0.0; r.amp_Br = 0.0;
         return ok;
@@ -856,6 +858,9 @@ __device__ inline bool phm_setup(PhmRecT<S>& r, nd<S> m1_SI, nd<S> m2_SI, nd<S> 
     hm_map(Rholm, Mf_RD_lm, Mf_RD_22, mm, fr + shift, false, r.ar, r.br, fi, fr);
     r.fi = fi;
     r.fr = fr;
+    r.iai = rcp_pos(r.ai);
+    r.iam = rcp_pos(r.am);
+    r.iar = rcp_pos(r.ar);
     r.PhDBconst = phase_only(r, r.am * fi + r.bm) / r.am;
     r.PhDCconst = phase_only(r, r.ar * fr + r.br) / r.ar;
     r.PhDBAterm = phase_only(r, r.ai * fi + r.bi) / r.ai;
@@ -906,20 +911,20 @@ __device__ inline void phm_eval(const PhmRecT<S>& r, double f, S& amp, S& phase,
 
     // phase and t(f): one of three linear frequency maps, then one PhenomD phase
     // evaluation; phi = ph / a - c1 + c2 with c1 = c2 = 0 on the first piece
-    S sa, sb, c1, c2;
+    S sa, isa, sb, c1, c2;
     if (!(Mf_wf > r.fi)) {
-        sa = r.ai; sb = r.bi; c1 = 0.0; c2 = 0.0;
+        sa = r.ai; isa = r.iai; sb = r.bi; c1 = 0.0; c2 = 0.0;
     } else if (!(Mf_wf > r.fr)) {
-        sa = r.am; sb = r.bm; c1 = r.PhDBconst; c2 = r.PhDBAterm;
+        sa = r.am; isa = r.iam; sb = r.bm; c1 = r.PhDBconst; c2 = r.PhDBAterm;
     } else {
-        sa = r.ar; sb = r.br; c1 = r.PhDCconst; c2 = r.phaseC;
+        sa = r.ar; isa = r.iar; sb = r.br; c1 = r.PhDCconst; c2 = r.phaseC;
     }
     S ph, dph;
     // below both joins the amplitude and phase maps are the same (2/m) Mf
     // (PhenomHMWaveform.cu:2200-2212, :2107-2150): the powers of Mf_amp serve both
     const S Mf_ph = sa * Mf_wf + sb;
     phase_imr(r, Mf_ph, ph, &dph, &p, value(Mf_ph) == value(Mf_amp));
-    const S phi = r.cshift + ((ph * rcp_pos(sa) - c1) + c2);
+    const S phi = r.cshift + ((ph * isa - c1) + c2);
     phase = -r.t0 * (Mf_wf - r.Mf_ref) + (phi - (r.mm * r.phi0));
     const double INV_TWO_PI = 0.15915494309189535;
     tf = (r.M_sec * (dph * INV_TWO_PI)) - ((r.t0 * INV_TWO_PI) * r.M_sec);
