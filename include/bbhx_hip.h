@@ -182,8 +182,9 @@ int bbhx_hip_hdyn_fused(double* d_h, double* h_h, const double* params, const do
  * callers that cannot allocate device memory: code under HIP graph capture
  * that must not call hipMalloc, tight serving loops, harnesses that forbid
  * allocation. It calls no hipMalloc / hipFree and no allocating library
- * routine: the per-(template, mode) records live in a fixed array of the
- * library, BBHX_HIP_NOALLOC_MAX_RECORDS records (about 22 MB) per device,
+ * routine: the per-(template, mode) records and the per-node weights live in
+ * a fixed array of the library, BBHX_HIP_NOALLOC_MAX_RECORDS records and
+ * weights for up to 1024 nodes (about 22.6 MB) per device,
  * reserved when the library loads on a device and held until the process
  * ends. numBinAll * numModes above that returns status 5. Every call on one
  * device shares the array, which keeps the last call's records: calls that
