@@ -1479,7 +1479,7 @@ int direct_launch(bbhx_direct_plan* p, double* d_h, double* h_h, double* grad, i
     hipEvent_t ev[3];
     const bool timed = timer_take3(ev);
     if (timed) (void)hipEventRecord(ev[0], st);
-    hipLaunchKernelGGL(k_dir_setup, dim3(SETUP_PARTS * ceil_div((long)B * M, 64)), dim3(64), 0, st, p->recs, params, B,
+    hipLaunchKernelGGL(k_dir_setup, dim3(setup_blocks(B, M)), dim3(64), 0, st, p->recs, params, B,
                        ml, M, qnm,
                        table_len, dspin);
     hipLaunchKernelGGL(k_dir_grid, dim3(B * M), dim3(64), sizeof(double) * (size_t)p->Lmax, st, p->X, p->Yc,

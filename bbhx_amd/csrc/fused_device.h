@@ -53,10 +53,10 @@ __device__ inline void phenomd_ringdown(S finmass, S finspin, const double* __re
 // One of the three independent dependency chains of a record's set-up (they
 // write disjoint fields of the record): PART 0, the mode-dependent PhenomHM
 // coefficients; PART 1, the (2,2) reference (PhenomD amplitude, Mf_ref, phi0,
-// t0); PART 2, the response record. A set-up kernel gives each part its own
-// wavefronts (a branch inside one wavefront would run them one after another),
-// so the set-up takes the longest chain instead of their sum.
-constexpr int SETUP_PARTS = 3;
+// t0), the same for every mode of a binary; PART 2, the response record. A
+// set-up kernel gives each part its own wavefronts (a branch inside one
+// wavefront would run them one after another), so the set-up takes the longest
+// chain instead of their sum.
 
 template <int PART>
 __device__ inline void setup_part(FusedRec& r, const double* __restrict__ params, int B, int b, int l, int m,
@@ -78,20 +78,41 @@ __device__ inline void setup_part(FusedRec& r, const double* __restrict__ params
     }
 }
 
-// Body of a set-up kernel of 64-thread workgroups over SETUP_PARTS * nb blocks,
-// nb = ceil(B * M / 64): block k does part k / nb of records 64 (k % nb) + tid.
+// PART 1 (the (2,2) reference) depends on the binary alone: computed once per
+// template and written into each of its M records.
+__device__ inline void setup_reference(FusedRec* __restrict__ rb, int M, const double* __restrict__ params, int B,
+                                       int b, const double* __restrict__ qnm, int table_len, double dspin)
+{
+    FusedRec t;
+    setup_part<1>(t, params, B, b, 2, 2, qnm, table_len, dspin);
+    for (int k = 0; k < M; k++) phm::copy_reference(rb[k].w, t.w);
+}
+
+// Blocks of 64 lanes of a set-up kernel for B templates of M modes: PART 0 and
+// PART 2 one lane per record (nb = ceil(B M / 64) blocks each), PART 1 one lane
+// per template (ceil(B / 64) blocks).
+__host__ __device__ inline int setup_blocks(int B, int M) { return 2 * ((B * M + 63) / 64) + (B + 63) / 64; }
+
+// Body of a set-up kernel of 64-thread workgroups over setup_blocks(B, M)
+// blocks: [0, nb) PART 0, [nb, nb + nt) PART 1, then PART 2 (nb = ceil(B M /
+// 64), nt = ceil(B / 64)).
 __device__ inline void setup_split(FusedRec* __restrict__ recs, const double* __restrict__ params, int B,
                                    const ModeList& modes, int M, const double* __restrict__ qnm, int table_len,
                                    double dspin)
 {
-    const int nb = (B * M + 63) / 64;
-    const int part = blockIdx.x / nb;
-    const int idx = (blockIdx.x - part * nb) * 64 + threadIdx.x;
+    const int nb = (B * M + 63) / 64, nt = (B + 63) / 64;
+    const int blk = blockIdx.x;
+    if (blk >= nb && blk < nb + nt) {
+        const int b = (blk - nb) * 64 + threadIdx.x;
+        if (b < B) setup_reference(recs + (long)b * M, M, params, B, b, qnm, table_len, dspin);
+        return;
+    }
+    const int part = blk < nb ? 0 : 2;
+    const int idx = (part == 0 ? blk : blk - nb - nt) * 64 + threadIdx.x;
     if (idx >= B * M) return;
     const int b = idx / M, mi = idx % M;
     const int l = modes.ell[mi], m = modes.mm[mi];
     if (part == 0) setup_part<0>(recs[idx], params, B, b, l, m, qnm, table_len, dspin);
-    else if (part == 1) setup_part<1>(recs[idx], params, B, b, l, m, qnm, table_len, dspin);
     else setup_part<2>(recs[idx], params, B, b, l, m, qnm, table_len, dspin);
 }
 

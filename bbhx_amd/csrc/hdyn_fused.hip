@@ -35,7 +35,7 @@ bool response_orbit_tab(const bbhx_response* r, resp::OrbitTab& o);
 
 namespace {
 
-// The records over SETUP_PARTS * ceil(B M / 64) blocks, then the node weights
+// The records over setup_blocks(B, M) blocks, then the node weights
 // over ceil(3 K / 64) more.
 __global__ __launch_bounds__(64) void k_fused_setup(FusedRec* __restrict__ recs, NodeW* __restrict__ nw,
                                                     const double* __restrict__ params, int B, ModeList modes, int M,
@@ -43,7 +43,7 @@ __global__ __launch_bounds__(64) void k_fused_setup(FusedRec* __restrict__ recs,
                                                     const double* __restrict__ freqs, const cd* __restrict__ h0,
                                                     const cd* __restrict__ dc, int K, int jend)
 {
-    const int nsb = SETUP_PARTS * ((B * M + 63) / 64);
+    const int nsb = setup_blocks(B, M);
     if ((int)blockIdx.x >= nsb) {
         node_weight(nw, freqs, h0, dc, K, jend, ((int)blockIdx.x - nsb) * 64 + threadIdx.x);
         return;
@@ -206,7 +206,7 @@ int hdyn_fused_launch(FusedRec* recs, NodeW* nw, double* d_h, double* h_h, const
     if (timing && g_timer.on && !timed) return ERR_LAUNCH;
     if (timed) (void)hipEventRecord(ev[0], st);
     const int jend = include_last_segment ? K : K - 1;
-    hipLaunchKernelGGL(k_fused_setup, dim3(SETUP_PARTS * ceil_div(nrec, 64) + ceil_div(3 * K, 64)), dim3(64), 0, st,
+    hipLaunchKernelGGL(k_fused_setup, dim3(setup_blocks(numBinAll, numModes) + ceil_div(3 * K, 64)), dim3(64), 0, st,
                        recs, nw, params, numBinAll, ml, numModes, qnm, table_len, dspin, freqs,
                        reinterpret_cast<const cd*>(h0), reinterpret_cast<const cd*>(dataConstants), K, jend);
     if (hipGetLastError() != hipSuccess) return ERR_LAUNCH;

@@ -58,7 +58,7 @@ __global__ __launch_bounds__(64) void k_grad_setup_fwd(FusedRec* __restrict__ re
                                                        const double* __restrict__ freqs, const cd* __restrict__ h0,
                                                        const cd* __restrict__ dc, int K, int jend)
 {
-    const int nsb = SETUP_PARTS * ((B * M + 63) / 64);
+    const int nsb = setup_blocks(B, M);
     if ((int)blockIdx.x >= nsb) {
         node_weight(nw, freqs, h0, dc, K, jend, ((int)blockIdx.x - nsb) * 64 + threadIdx.x);
         return;
@@ -66,21 +66,30 @@ __global__ __launch_bounds__(64) void k_grad_setup_fwd(FusedRec* __restrict__ re
     setup_split(recs, params, B, modes, M, qnm, table_len, dspin);
 }
 
-// the records with tangents, one lane per (template, mode)
-// records with tangents over SETUP_PARTS * nb blocks of 64 lanes, nb =
-// ceil(B * M / 64): block k does part k / nb of records 64 (k % nb) + tid
+// the records with tangents over setup_blocks(B, M) blocks of 64 lanes, laid
+// out as setup_split's: PART 0 and PART 2 one lane per (template, mode), PART 1
+// (the (2,2) reference, which depends on the binary alone) one lane per
+// template, written into each of its M records
 __global__ __launch_bounds__(64) void k_grad_setup(GradRec* __restrict__ grecs, const double* __restrict__ params,
                                                    int B, ModeList modes, int M, const double* __restrict__ qnm,
                                                    int table_len, double dspin)
 {
-    const int nb = (B * M + 63) / 64;
-    const int part = blockIdx.x / nb;
-    const int idx = (blockIdx.x - part * nb) * 64 + threadIdx.x;
+    const int nb = (B * M + 63) / 64, nt = (B + 63) / 64;
+    const int blk = blockIdx.x;
+    if (blk >= nb && blk < nb + nt) {
+        const int b = (blk - nb) * 64 + threadIdx.x;
+        if (b >= B) return;
+        GradRec t;
+        grad_setup_part<1>(t, params, B, b, 2, 2, qnm, table_len, dspin);
+        for (int k = 0; k < M; k++) phm::copy_reference(grecs[(long)b * M + k].w, t.w);
+        return;
+    }
+    const int part = blk < nb ? 0 : 2;
+    const int idx = (part == 0 ? blk : blk - nb - nt) * 64 + threadIdx.x;
     if (idx >= B * M) return;
     const int b = idx / M, mi = idx % M;
     const int l = modes.ell[mi], m = modes.mm[mi];
     if (part == 0) grad_setup_part<0>(grecs[idx], params, B, b, l, m, qnm, table_len, dspin);
-    else if (part == 1) grad_setup_part<1>(grecs[idx], params, B, b, l, m, qnm, table_len, dspin);
     else grad_setup_part<2>(grecs[idx], params, B, b, l, m, qnm, table_len, dspin);
 }
 
@@ -292,12 +301,12 @@ extern "C" int bbhx_hip_hdyn_fused_grad(double* d_h, double* h_h, double* grad, 
     if (timed) (void)hipEventRecord(ev[0], st);
     if (hipEventRecord(side->fork, st) != hipSuccess || hipStreamWaitEvent(side->stream, side->fork, 0) != hipSuccess)
         return ERR_LAUNCH;
-    hipLaunchKernelGGL(k_grad_setup, dim3(SETUP_PARTS * ceil_div((long)nrec, 64)), dim3(64), 0, side->stream, grecs,
+    hipLaunchKernelGGL(k_grad_setup, dim3(setup_blocks(numBinAll, numModes)), dim3(64), 0, side->stream, grecs,
                        params,
                        numBinAll, ml, numModes, qnm, table_len, dspin);
     if (hipEventRecord(side->join, side->stream) != hipSuccess) return ERR_LAUNCH;
     const int jend = include_last_segment ? K : K - 1;
-    hipLaunchKernelGGL(k_grad_setup_fwd, dim3(SETUP_PARTS * ceil_div((long)nrec, 64) + ceil_div(3 * K, 64)), dim3(64),
+    hipLaunchKernelGGL(k_grad_setup_fwd, dim3(setup_blocks(numBinAll, numModes) + ceil_div(3 * K, 64)), dim3(64),
                        0, st, recs, nw, params, numBinAll, ml, numModes, qnm, table_len, dspin, freqs,
                        reinterpret_cast<const cd*>(h0), reinterpret_cast<const cd*>(dataConstants), K, jend);
     if (hipGetLastError() != hipSuccess) return ERR_LAUNCH;

@@ -756,6 +756,36 @@ __device__ inline void hm_map(S Rholm, S Mf_RD_lm, S Mf_RD_22, int mm, nd<S> flm
     }
 }
 
+// The fields that phm_setup's PART 1 (the (2,2) reference) writes, copied
+// from one record to another (they depend on the binary alone).
+template <class S>
+__device__ inline void copy_reference(PhmRecT<S>& r, const PhmRecT<S>& w)
+{
+    r.Mf_ref = w.Mf_ref;
+    r.phi0 = w.phi0;
+    r.t0 = w.t0;
+    r.a_fRD = w.a_fRD;
+    r.a_fDM = w.a_fDM;
+    r.a_g1 = w.a_g1;
+    r.a_g2 = w.a_g2;
+    r.a_g3 = w.a_g3;
+    r.a_fmax = w.a_fmax;
+    r.a_rho1 = w.a_rho1;
+    r.a_rho2 = w.a_rho2;
+    r.a_rho3 = w.a_rho3;
+    r.a_d0 = w.a_d0;
+    r.a_d1 = w.a_d1;
+    r.a_d2 = w.a_d2;
+    r.a_d3 = w.a_d3;
+    r.a_d4 = w.a_d4;
+    r.a_pf23 = w.a_pf23;
+    r.a_pf1 = w.a_pf1;
+    r.a_pf43 = w.a_pf43;
+    r.a_pf53 = w.a_pf53;
+    r.a_pf2 = w.a_pf2;
+    r.a_amp0 = w.a_amp0;
+}
+
 // Per-(binary, mode) setup. Masses in solar masses (m1_SI/MSUN), distance in m.
 // Mf_RD_lm/Mf_DM_lm: this mode's QNM frequencies; Mf_RD_D/Mf_DM_D: the PhenomD
 // (2,2) ringdown frequencies from the QNM table (IMRPhenomHMCore, PhenomHMWaveform.cu:2671-2844).

@@ -58,23 +58,28 @@ struct OrbitTab {
 
 // Real factor of the spin-weight -2 spherical harmonic -2Y_lm(theta, phi) =
 // fac(theta) e^{i m phi} for the PhenomHM (l, m) (Response.cu:383-424).
+// Written in the half-angle cosine ch and sine sh of theta (1 + cos = 2 ch^2,
+// 1 - cos = 2 sh^2, sin = 2 sh ch, cos = ch^2 - sh^2) with integer powers as
+// products: two trigonometric calls and no pow.
 template <class S>
 __device__ inline S ylm_fac(int l, int m, S theta)
 {
     S fac = 0.0;
-    const S c = cos(theta), ch = cos(theta / 2.0), sh = sin(theta / 2.0), st = sin(theta);
-    if (l == 2 && m == -2) fac = sqrt(5.0 / (64.0 * PI)) * (1.0 - c) * (1.0 - c);
-    else if (l == 2 && m == -1) fac = sqrt(5.0 / (16.0 * PI)) * st * (1.0 - c);
-    else if (l == 2 && m == 1) fac = sqrt(5.0 / (16.0 * PI)) * st * (1.0 + c);
-    else if (l == 2 && m == 2) fac = sqrt(5.0 / (64.0 * PI)) * (1.0 + c) * (1.0 + c);
-    else if (l == 3 && m == -3) fac = sqrt(21.0 / (2.0 * PI)) * ch * pow(sh, 5.0);
-    else if (l == 3 && m == -2) fac = sqrt(7.0 / (4.0 * PI)) * (2.0 + 3.0 * c) * pow(sh, 4.0);
-    else if (l == 3 && m == 2) fac = sqrt(7.0 / PI) * pow(ch, 4.0) * (-2.0 + 3.0 * c) / 2.0;
-    else if (l == 3 && m == 3) fac = -sqrt(21.0 / (2.0 * PI)) * pow(ch, 5.0) * sh;
-    else if (l == 4 && m == -4) fac = 3.0 * sqrt(7.0 / PI) * pow(ch, 2.0) * pow(sh, 6.0);
-    else if (l == 4 && m == -3) fac = 3.0 * sqrt(7.0 / (2.0 * PI)) * ch * (1.0 + 2.0 * c) * pow(sh, 5.0);
-    else if (l == 4 && m == 3) fac = -3.0 * sqrt(7.0 / (2.0 * PI)) * pow(ch, 5.0) * (-1.0 + 2.0 * c) * sh;
-    else if (l == 4 && m == 4) fac = 3.0 * sqrt(7.0 / PI) * pow(ch, 6.0) * pow(sh, 2.0);
+    const S ch = cos(theta / 2.0), sh = sin(theta / 2.0);
+    const S ch2 = ch * ch, sh2 = sh * sh;
+    const S c = ch2 - sh2;
+    if (l == 2 && m == -2) fac = (4.0 * sqrt(5.0 / (64.0 * PI))) * (sh2 * sh2);
+    else if (l == 2 && m == -1) fac = (4.0 * sqrt(5.0 / (16.0 * PI))) * (sh2 * sh * ch);
+    else if (l == 2 && m == 1) fac = (4.0 * sqrt(5.0 / (16.0 * PI))) * (ch2 * ch * sh);
+    else if (l == 2 && m == 2) fac = (4.0 * sqrt(5.0 / (64.0 * PI))) * (ch2 * ch2);
+    else if (l == 3 && m == -3) fac = sqrt(21.0 / (2.0 * PI)) * ch * (sh2 * sh2 * sh);
+    else if (l == 3 && m == -2) fac = sqrt(7.0 / (4.0 * PI)) * (2.0 + 3.0 * c) * (sh2 * sh2);
+    else if (l == 3 && m == 2) fac = sqrt(7.0 / PI) * (ch2 * ch2) * (-2.0 + 3.0 * c) / 2.0;
+    else if (l == 3 && m == 3) fac = -sqrt(21.0 / (2.0 * PI)) * (ch2 * ch2 * ch) * sh;
+    else if (l == 4 && m == -4) fac = 3.0 * sqrt(7.0 / PI) * ch2 * (sh2 * sh2 * sh2);
+    else if (l == 4 && m == -3) fac = 3.0 * sqrt(7.0 / (2.0 * PI)) * ch * (1.0 + 2.0 * c) * (sh2 * sh2 * sh);
+    else if (l == 4 && m == 3) fac = -3.0 * sqrt(7.0 / (2.0 * PI)) * (ch2 * ch2 * ch) * (-1.0 + 2.0 * c) * sh;
+    else if (l == 4 && m == 4) fac = 3.0 * sqrt(7.0 / PI) * (ch2 * ch2 * ch2) * sh2;
     return fac;
 }
 
