@@ -259,8 +259,10 @@ __device__ inline void residuals(cd* rs, const FusedRec* __restrict__ rb, const 
 // The records of one template copied into LDS ('srec', M of them) by the
 // threads of a workgroup; a barrier must follow. The residual pass then reads
 // them with LDS loads (wave-uniform addresses, one broadcast each) instead of
-// scalar loads: measured 140 -> 129 us for 4096 templates x 128 nodes, the
-// scalar-load waits per mode outweighing the few values that then spill.
+// scalar loads: measured 140 -> 129 us for 4096 templates x 128 nodes (the
+// scalar-load waits per mode, which the SGPR budget serialises, gone). The
+// copy itself costs about 3 us of the kernel at that size (measured by doing
+// it twice).
 __device__ inline void records_to_lds(FusedRec* srec, const FusedRec* __restrict__ rb, int M)
 {
     const double* src = reinterpret_cast<const double*>(rb);
