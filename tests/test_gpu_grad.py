@@ -31,8 +31,17 @@ def to_sampling(p):
                      p[11]], axis=1)
 
 
-@pytest.fixture(scope="module")
-def grad_problem():
+# TDI forms: the benchmark's TDI-2 A/E/T, and X/Y/Z and rescaled first-generation
+# A/E/T, whose TDI combinations (and so the adjoint weights of the link terms)
+# differ. The latter two put the reference at 100 times the distance: weighted
+# by the A/E/T noise model their channels are so loud (|logL| ~ 1e7) that FP64
+# rounding alone would exceed the logL tolerances below.
+TDI_FORMS = [("AET", True, False), ("XYZ", False, False), ("AET", False, True)]
+
+
+@pytest.fixture(scope="module", params=TDI_FORMS, ids=["AET-tdi2", "XYZ-tdi1", "AET-tdi1-rescaled"])
+def grad_problem(request):
+    TDItag, tdi2, rescaled = request.param
     from bbhx_amd import get_backend
     from bbhx_amd.likelihood import HeterodynedLikelihood
     from bbhx_amd.utils.orbits import EqualArmlengthOrbits
@@ -44,8 +53,10 @@ def grad_problem():
     ref = draw(1, 1001)[:, 0]
     ref[0], ref[1] = 1.3e6, 4.1e5
     ref[11] = 0.62 * YRSID_SI
+    if (TDItag, tdi2, rescaled) != TDI_FORMS[0]:
+        ref[4] *= 100.0
     f = np.arange(1, 2**15 + 1) * (4.0 / YRSID_SI)
-    owave = OracleWaveform(orbits.as_host_dict(), tdi2=True)
+    owave = OracleWaveform(orbits.as_host_dict(), TDItag=TDItag, tdi2=tdi2, rescaled=rescaled)
     inj = owave(*ref, freqs=f, length=1024, modes=MODES, direct=False, fill=True, squeeze=True)[0]
     sens = AET1SensitivityMatrix(f)
     psd = np.array([sens[0], sens[1], sens[2]])
@@ -53,7 +64,8 @@ def grad_problem():
     df = f[1] - f[0]
     data = inj + np.sqrt(psd / (4 * df)) * (rng.standard_normal(psd.shape) + 1j * rng.standard_normal(psd.shape))
     K = 96
-    wave = BBHWaveformFD(response_kwargs=dict(orbits=orbits, tdi2=True), force_backend=backend)
+    wave = BBHWaveformFD(response_kwargs=dict(orbits=orbits, TDItag=TDItag, tdi2=tdi2, rescaled=rescaled),
+                         force_backend=backend)
     het = HeterodynedLikelihood(wave, f, data, ref, K, sens_mat=sens, template_gen_kwargs=dict(modes=MODES),
                                 reference_gen_kwargs=dict(length=1024, modes=MODES), force_backend=backend)
     psd_fn = lambda x: [AET1SensitivityMatrix(x)[i] for i in range(3)]  # noqa: E731

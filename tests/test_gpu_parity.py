@@ -558,3 +558,44 @@ def test_heterodyned_likelihood(backend, orbits, small_problem, K, last):
     dh, hh = host(het.hdyn_d_h), host(het.hdyn_h_h)
     assert np.array_equal(ll_d, -1 / 2.0 * (het.reference_d_d + hh - 2 * dh).real)
     assert np.max(np.abs(ll_d - ll_f)) < 1e-9
+
+
+@pytest.mark.parametrize("TDItag,tdi2,rescaled", [("XYZ", False, False), ("AET", False, True), ("AET", True, True)])
+def test_heterodyned_likelihood_tdi_forms(backend, orbits, small_problem, TDItag, tdi2, rescaled):
+    """The fused and kernel-by-kernel heterodyned logL in the other TDI forms
+    (X/Y/Z, rescaled, first and second generation) against the oracle of the
+    same form (the reference applies the second-generation factor to A/E/T
+    only, Response.cu:247-258),
+    on data of that form: the small problem's noise plus the injection in that
+    form at 100 times the distance (the X/Y/Z channels, weighted by the A/E/T
+    noise model, would otherwise give |logL| ~ 1e8, where FP64 rounding alone
+    exceeds 1e-6)."""
+    from bbhx_amd.likelihood import HeterodynedLikelihood
+    from bbhx_amd.utils.sensitivity import AET1SensitivityMatrix
+    from bbhx_amd.waveformbuild import BBHWaveformFD
+
+    sp = small_problem
+    K = 128
+    rkw = dict(TDItag=TDItag, tdi2=tdi2, rescaled=rescaled)
+    wave = BBHWaveformFD(response_kwargs=dict(orbits=orbits, **rkw), force_backend=backend)
+    owave = OracleWaveform(orbits.as_host_dict(), **rkw)
+    kw = dict(freqs=sp["f"], length=1024, modes=MODES, direct=False, fill=True, squeeze=True)
+    noise = sp["data"] - sp["owave"](*sp["ref"], **kw)[0]
+    ref = sp["ref"].copy()
+    ref[4] *= 100.0
+    data = owave(*ref, **kw)[0] + noise
+    sens = AET1SensitivityMatrix(sp["f"])
+    het = HeterodynedLikelihood(wave, sp["f"], data, ref, K, sens_mat=sens,
+                                reference_gen_kwargs=dict(length=1024), force_backend=backend)
+    psd_fn = lambda x: [AET1SensitivityMatrix(x)[i] for i in range(3)]  # noqa: E731
+    ohet = OracleHeterodyne(owave, sp["f"], data, ref, K, psd_fn, reference_length=1024)
+    assert abs(het.reference_ll - ohet.reference_ll) < 1e-6
+    p = perturb(ref, 9, 29, scale=1e-4)
+    ll_o = ohet.get_ll(p)[0]
+    ll_f = het.get_ll(p, fused=True)
+    ll_p = het.get_ll(p, fused=False)
+    print(TDItag, "tdi2" if tdi2 else "tdi1", "rescaled" if rescaled else "", "logL", ll_o, "max |dlogL| fused",
+          np.max(np.abs(ll_f - ll_o)), "plain", np.max(np.abs(ll_p - ll_o)), flush=True)
+    assert np.all(np.isfinite(ll_f))
+    assert np.max(np.abs(ll_p - ll_o)) < 1e-6, (ll_p, ll_o)
+    assert np.max(np.abs(ll_f - ll_o)) < 1e-6, (ll_f, ll_o)
