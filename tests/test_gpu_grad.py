@@ -188,3 +188,58 @@ def test_grad_graph_replays(grad_problem):
         print("replay", rep, ll_s.cpu().numpy(), ll_e, flush=True)
         assert np.array_equal(ll_s.cpu().numpy(), ll_e)
         assert np.array_equal(grad_s.cpu().numpy(), grad_e)
+
+
+@pytest.mark.parametrize("modes", [[(2, 2)], [(4, 3), (2, 1), (3, 3), (4, 4)]], ids=["22", "43-21-33-44"])
+def test_grad_mode_lists(modes):
+    """The gradient with other mode lists (one mode; four modes listed out of
+    order, one pair of equal m apart) against oracle finite differences, and its
+    logL against the forward's."""
+    from bbhx_amd import get_backend
+    from bbhx_amd.likelihood import HeterodynedLikelihood
+    from bbhx_amd.utils.orbits import EqualArmlengthOrbits
+    from bbhx_amd.utils.sensitivity import AET1SensitivityMatrix
+    from bbhx_amd.waveformbuild import BBHWaveformFD
+
+    backend = get_backend("hip")
+    orbits = EqualArmlengthOrbits(dt=1e4, t_max=3.0 * YRSID_SI)
+    ref = draw(1, 1001)[:, 0]
+    ref[0], ref[1] = 1.3e6, 4.1e5
+    ref[11] = 0.62 * YRSID_SI
+    f = np.arange(1, 2**15 + 1) * (4.0 / YRSID_SI)
+    owave = OracleWaveform(orbits.as_host_dict(), tdi2=True)
+    inj = owave(*ref, freqs=f, length=1024, modes=modes, direct=False, fill=True, squeeze=True)[0]
+    sens = AET1SensitivityMatrix(f)
+    psd = np.array([sens[0], sens[1], sens[2]])
+    rng = np.random.default_rng(1002)
+    df = f[1] - f[0]
+    data = inj + np.sqrt(psd / (4 * df)) * (rng.standard_normal(psd.shape) + 1j * rng.standard_normal(psd.shape))
+    K = 96
+    wave = BBHWaveformFD(response_kwargs=dict(orbits=orbits, tdi2=True), force_backend=backend)
+    het = HeterodynedLikelihood(wave, f, data, ref, K, sens_mat=sens, template_gen_kwargs=dict(modes=modes),
+                                reference_gen_kwargs=dict(length=1024, modes=modes), force_backend=backend)
+    psd_fn = lambda x: [AET1SensitivityMatrix(x)[i] for i in range(3)]  # noqa: E731
+    ohet = OracleHeterodyne(owave, f, data, ref, K, psd_fn, reference_length=1024, modes=modes)
+    x0 = to_sampling(ref[:, None])[0]
+    rng = np.random.default_rng(6)
+    dx = np.array([3e-5, 1e-3, 3e-3, 3e-3, 0.0, 0.05, 0.02, 0.01, 0.01, 0.05, 3.0])
+    X = x0[None, :] + dx[None, :] * rng.standard_normal((2, 11))
+    X[:, 4] = x0[4] * (1 + 0.01 * rng.standard_normal(2))
+    ll_g, grad = het.get_ll_and_grad(X)
+    ll_f = het.get_ll(het.sampling_to_params(X), fused=True)
+    assert np.max(np.abs(ll_g - ll_f)) < 1e-7
+    assert np.max(np.abs(ll_g - oracle_ll(ohet, X))) < 1e-6
+    B = X.shape[0]
+    steps = np.tile(STEPS, (B, 1))
+    steps[:, 4] = REL_DIST_STEP * X[:, 4]
+    pts = []
+    for k in range(11):
+        for s in (2, 1, -1, -2):
+            Y = X.copy()
+            Y[:, k] += s * steps[:, k]
+            pts.append(Y)
+    ll = oracle_ll(ohet, np.concatenate(pts)).reshape(11, 4, B)
+    fd = ((-ll[:, 0] + 8 * ll[:, 1] - 8 * ll[:, 2] + ll[:, 3]) / (12 * steps.T)).T
+    rel = np.abs(grad - fd) / np.maximum(np.abs(fd), 1e-300)
+    print(modes, "max relative gradient error", rel.max(), flush=True)
+    assert np.all(np.abs(grad - fd) <= 1e-5 * np.abs(fd)), np.argwhere(np.abs(grad - fd) > 1e-5 * np.abs(fd))
