@@ -7,13 +7,19 @@
 // all 3K nodes). Here the whole chain is two launches and only d_h, h_h leave
 // the chip:
 //
-//   k_fused_setup  one lane per (template, mode): QNM ringdown frequencies,
-//                  PhenomHM coefficient record, response H matrix
-//   k_fused_eval   one 64-lane wavefront per template; lane l owns sparse nodes
-//                  j = l, l+64, ...; the mode loop accumulates the three TDI
-//                  channels in registers; r = h/h0 goes to LDS; after one
-//                  barrier each lane sums its segments (j-1, j) from LDS and
-//                  the heterodyned sums are reduced across the wavefront.
+//   k_fused_setup  the (template, mode) records in three independent parts
+//                  (mode-dependent PhenomHM coefficients and QNM ringdown, one
+//                  lane per record; the (2,2) reference, one lane per
+//                  template; the response basis and Y factors, one lane per
+//                  record), and in extra blocks the per-node weights of the
+//                  heterodyned sums and 1 / h0 (NodeW)
+//   k_fused_eval   one workgroup per template (one 64-lane wavefront at large
+//                  batches); lane l owns sparse nodes j = l, l + 64, ...; the
+//                  records are copied into LDS; per node the modes' link terms
+//                  are summed before one TDI combination; r = h / h0 goes to
+//                  LDS; after one barrier each lane adds its nodes' terms of
+//                  the heterodyned sums (per-node form) and the sums are
+//                  reduced across the workgroup in a fixed order.
 //
 // FP64 throughout ("plain" form). No atomics; the reduction order is fixed.
 
