@@ -5,16 +5,18 @@
 // differentiated is the one HeterodynedLikelihood.get_ll returns
 // (likelihood.py:550-648): logL = Re d_h - 1/2 Re h_h - 1/2 d_d.
 //
-// k_grad_setup_fwd one lane per (template, mode): the double record of the
-//                forward pass;
+// k_grad_setup_fwd the double records of the forward pass (as k_fused_setup)
+//                and the per-node weights of the sums;
 // k_grad_setup   (on a side stream, concurrently with the two kernels that
-//                follow) three lanes per (template, mode), one per independent
-//                part, each in a wavefront of its own: a PhenomHM record whose
-//                every coefficient carries its derivatives with respect to
+//                follow) the records with tangents in three independent parts,
+//                each in wavefronts of its own: a PhenomHM record whose every
+//                coefficient carries its derivatives with respect to
 //                (ln M, q, chi1, chi2) (Dual<4> instantiation of the set-up
-//                code; its mode-dependent part and its (2,2) reference part),
-//                and the polarisation basis, wave vector and Y factors with
-//                their derivatives in lam, beta (analytic) and cos inc.
+//                code; its mode-dependent part one lane per (template, mode),
+//                its (2,2) reference part one lane per template), and the
+//                polarisation basis, wave vector and Y factors with the frame's
+//                rotation coefficients in lam, beta and the Y factors'
+//                derivatives in cos inc.
 // k_grad_weights one workgroup per template (as the forward kernel's):
 //   1. forward: residuals r = h / h0 on the K nodes (LDS) and d_h, h_h,
 //      exactly as the forward kernel;
@@ -23,11 +25,12 @@
 //      of Likelihood.cu:91-103 and r = h / h0), written out (48 K bytes);
 // k_grad_points  one 64-lane wavefront per (template, mode), lane per node:
 //   3. per (node, mode) tangents: PhenomHM amplitude, phase and t(f) with
-//      respect to the intrinsic parameters (Dual<4> evaluation of the same
+//      respect to the intrinsic parameters (structured tangents of the same
 //      phm_eval code), the response by an adjoint of the (linear) TDI
-//      combination and forward tangents of the six single-link terms
-//      G_ij in t, lam, beta, psi and cos inc; contracted with g at once, so
-//      each lane keeps 11 accumulators; the wavefront reduces them;
+//      combination and derivatives of the six single-link terms G_ij in t
+//      and, through the frame's rotation, in lam, beta and psi, and in cos
+//      inc through the Y factors; contracted with g at once, so each lane
+//      keeps 11 accumulators; the wavefront reduces them;
 // k_grad_reduce  the modes' partial gradients, summed in mode order.
 // Splitting the forward pass from the per-point tangents keeps each kernel's
 // register footprint (and so its occupancy) to its own needs.
