@@ -200,9 +200,13 @@ def fused_param_rows(template_gen, params, t_obs_start, t_obs_end, device):
                          "beta, psi, t_ref)")
     m1, m2, c1, c2 = p[0].copy(), p[1].copy(), p[2].copy(), p[3].copy()
     template_gen.amp_phase_gen._sanity_check_params(m1, m2, c1, c2)
-    rows = np.stack([m1, m2, c1, c2, p[4], p[5], p[7], p[8], p[9], p[10], p[11]])
     t_start, t_end = template_gen.time_window(p[8], p[9], t_obs_start, t_obs_end)
-    return to_dev(rows, device), to_dev(t_start, device), to_dev(t_end, device)
+    # one host-to-device copy: the 11 rows, t_start and t_end as one (13, B) block
+    B = p.shape[1]
+    block = np.stack([m1, m2, c1, c2, p[4], p[5], p[7], p[8], p[9], p[10], p[11],
+                      np.broadcast_to(t_start, (B,)), np.broadcast_to(t_end, (B,))])
+    dev = to_dev(block, device)
+    return dev[:11], dev[11], dev[12]
 
 
 def sampling_rows_device(x, t_obs_start=0.0, t_obs_end=1.0):
@@ -357,6 +361,11 @@ class HeterodynedLikelihood(BBHxParallelModule):
         self.phase_marginalize = phase_marginalize
         self.return_extracted_snr = return_extracted_snr
         waveform_kwargs = self._merge_kwargs(dict(waveform_kwargs))
+        if fused and not (phase_marginalize or return_extracted_snr):
+            # the logL the evaluation kernel writes (the same expression of its
+            # d_h, h_h), one device-to-host copy
+            rows, t_start, t_end = self.fused_inputs(params, waveform_kwargs)
+            return self.get_ll_device(rows, t_start, t_end, modes=waveform_kwargs.get("modes")).cpu().numpy()
         if fused:
             self.hdyn_d_h, self.hdyn_h_h = self._fused_inner_products(params, waveform_kwargs)
         else:
