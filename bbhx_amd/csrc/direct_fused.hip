@@ -300,9 +300,8 @@ __global__ __launch_bounds__(64) void k_dir_spline(const double* __restrict__ X,
     const int q = s - m * NQ;
     auto at = [&](int i) { return ((base + i) * M + m) * RS + RQ + q * 4; };
     auto yi = [&](int i) { return Yc[((base + i) * M + m) * NQ + q]; };
-    double* dp = Cf + 1;  // d' then the slopes (= c1)
-    double* C2 = Cf + 2;
-    double* cp = Cf + 3;  // c' then c3
+    const double* dp = Cf + 1;  // d' (the forward sweep's, read back)
+    const double* cp = Cf + 3;  // c'
     // the FP32-bulk record of point i (CfF != nullptr): this lane's quantity
     auto put_f = [&](int i, double x_lo, double x_hi, double c0, double c1, double c2, double c3) {
         SegRecF& rf = CfF[(base + i) * M + m];
@@ -329,6 +328,16 @@ __global__ __launch_bounds__(64) void k_dir_spline(const double* __restrict__ X,
         }
     };
 
+    // The forward sweep keeps its values (d' in slot c1, c' in slot c3) in the
+    // records and writes each quantity's whole 32-B group (with the knot value
+    // and a zero c2) and the bounds: every line of the records is then fully
+    // written, so no partially written line leaves the cache to be merged in
+    // memory; the backward sweep overwrites all of it with the final values.
+    auto put_fwd = [&](int i, double x_lo, double x_hi, double y, double d, double c) {
+        *reinterpret_cast<Cubic*>(Cf + at(i)) = Cubic{y, d, 0.0, c};
+        if (q == 0)
+            *reinterpret_cast<Cubic*>(Cf + ((base + i) * M + m) * RS) = Cubic{x_lo, x_hi, 0.0, 0.0};
+    };
     double x0 = xi(0), x1 = xi(1), x2 = xi(2);
     double y0 = yi(0), y1 = yi(1), y2 = yi(2);
     double cprev, dprev;
@@ -338,8 +347,7 @@ __global__ __launch_bounds__(64) void k_dir_spline(const double* __restrict__ X,
         double bb = ((dx1 + 2 * dd) * dx2 * s1 + dx1 * dx1 * s2) / dd;
         cprev = dd / dx2;
         dprev = bb / dx2;
-        cp[at(0)] = cprev;
-        dp[at(0)] = dprev;
+        put_fwd(0, x0, x1, y0, dprev, cprev);
     }
     // each sweep loads DSPL_BLOCK rows ahead of its recurrence (one memory
     // latency per block) and divides once per row
@@ -364,14 +372,14 @@ __global__ __launch_bounds__(64) void k_dir_spline(const double* __restrict__ X,
             const double im = 1.0 / (diag - lo * cprev);
             cprev = up * im;
             dprev = (rhs - lo * dprev) * im;
-            cp[at(i)] = cprev;
-            dp[at(i)] = dprev;
+            put_fwd(i, xb, xc, yb, dprev, cprev);
             xa = xb;
             xb = xc;
             ya = yb;
             yb = yc;
         }
     }
+    double snext;  // the last row's slope, the start of the backward sweep
     {
         const double xm3 = xi(n - 3), xm2 = xi(n - 2), xm1 = xi(n - 1);
         const double ym3 = yi(n - 3), ym2 = yi(n - 2), ym1 = yi(n - 1);
@@ -379,18 +387,11 @@ __global__ __launch_bounds__(64) void k_dir_spline(const double* __restrict__ X,
         const double t1 = (ym2 - ym3) / ex1, t2 = (ym1 - ym2) / ex2;
         const double rhs = ((ex2 * ex2 * t1 + (2 * ed + ex2) * ex1 * t2) / ed);
         const double mm = ex1 - ed * cprev;
-        dp[at(n - 1)] = (rhs - ed * dprev) / mm;
+        snext = (rhs - ed * dprev) / mm;
     }
-    double snext = dp[at(n - 1)];
     double xn = xi(n - 1), yn = yi(n - 1);
-    Cf[at(n - 1)] = yn;
-    dp[at(n - 1)] = 0.0;
-    C2[at(n - 1)] = 0.0;
-    cp[at(n - 1)] = 0.0;
-    if (q == 0) {
-        Cf[((base + n - 1) * M + m) * RS] = xn;
-        Cf[((base + n - 1) * M + m) * RS + 1] = xn;
-    }
+    *reinterpret_cast<Cubic*>(Cf + at(n - 1)) = Cubic{yn, 0.0, 0.0, 0.0};
+    if (q == 0) *reinterpret_cast<Cubic*>(Cf + ((base + n - 1) * M + m) * RS) = Cubic{xn, xn, 0.0, 0.0};
     if (CfF) put_f(n - 1, xn, xn, yn, 0.0, 0.0, 0.0);
     for (int i0 = n - 2; i0 >= 0; i0 -= DSPL_BLOCK) {
       double dv[DSPL_BLOCK], cv[DSPL_BLOCK], xv[DSPL_BLOCK], yv[DSPL_BLOCK];
@@ -412,15 +413,9 @@ __global__ __launch_bounds__(64) void k_dir_spline(const double* __restrict__ X,
         const double slope = (yn - y_i) * idx;
         const double t = (si + snext - 2 * slope) * idx;
         const double c2 = (slope - si) * idx - t, c3 = t * idx;
-        Cf[at(i)] = y_i;
-        dp[at(i)] = si;
-        C2[at(i)] = c2;
-        cp[at(i)] = c3;
+        *reinterpret_cast<Cubic*>(Cf + at(i)) = Cubic{y_i, si, c2, c3};
         if (CfF) put_f(i, x_i, xn, y_i, si, c2, c3);
-        if (q == 0) {
-            Cf[((base + i) * M + m) * RS] = x_i;
-            Cf[((base + i) * M + m) * RS + 1] = xn;
-        }
+        if (q == 0) *reinterpret_cast<Cubic*>(Cf + ((base + i) * M + m) * RS) = Cubic{x_i, xn, 0.0, 0.0};
         snext = si;
         xn = x_i;
         yn = y_i;
