@@ -1017,16 +1017,18 @@ __global__ __launch_bounds__(64) void k_dir_reduce(double* __restrict__ d_h, dou
 
 // ---------------------------------------------------------------- FP32-bulk routing
 //
-// The FP32-bulk form's error is at most ~1.3e-7 h_h per template (measured on
-// the uneven and the large batches), so templates with h_h above
-// FP32_FALLBACK_HH (error bound ~4e-4 in logL there, 2.5x inside the 1e-3
-// target) are evaluated with the FP64 records and kernel. Which ones is first
+// The FP32-bulk form's error, measured template by template against the FP64
+// form on the uneven (8192) and the large (512) batches with every template in
+// FP32, is at most 1.4e-8 h_h wherever h_h > 3000 (and below 3.3e-5 in logL
+// where h_h <= 3000), so templates with h_h above FP32_FALLBACK_HH (error up to
+// ~3e-4 in logL there, 3x inside the 1e-3 target) are evaluated with the FP64
+// records and kernel. Which ones is first
 // decided from an estimate of h_h made before the dense pass (k_dir_route);
 // after it, any binary the estimate sent to FP32 whose h_h exceeds the bound
 // is evaluated again in FP64 (its work items only), so the estimate decides
 // the cost and never the result.
 
-constexpr double FP32_FALLBACK_HH = 3000.0;
+constexpr double FP32_FALLBACK_HH = 2.0e4;
 
 __global__ void k_dir_loud_zero(int* __restrict__ loud_cnt)
 {

@@ -18,7 +18,7 @@ Direct, 4 yr of data at 5 s cadence (12.6 M dense bins), sparse length 4096
   * the same distance scaling of d_h, h_h (1e-12 relative);
   * the FP32-bulk form stays within 3e-7 of h_h of the FP64 form in d_h and
     h_h (measured up to 1.3e-7 on uneven batches), and within 1e-3 in logL on
-    every template (templates with h_h above 3000 are evaluated in FP64).
+    every template (templates with h_h above 2e4 are evaluated in FP64).
 
 Sparse lengths 4, 5 (the smallest the plan accepts) and 6144 (the largest)
 against the oracle run template by template: |delta logL| <= 1e-6 + 1e-12
@@ -185,7 +185,7 @@ def test_direct_fp32_bulk_error_bound_large(direct_large):
     rd, rh = np.abs(d32 - d64) / scale, np.abs(h32 - h64) / scale
     dll = np.abs(0.5 * (h32.real - h64.real) - (d32.real - d64.real))
     print("FP32 bulk: max |d d_h| / h_h", rd.max(), "max |d h_h| / h_h", rh.max(), "max |dlogL|", dll.max(),
-          "templates recomputed in FP64 (h_h > 3000):", int(np.sum(h64.real > 3000.0)), flush=True)
+          "templates recomputed in FP64 (h_h > 2e4):", int(np.sum(h64.real > 2.0e4)), flush=True)
     assert rd.max() < 3e-7 and rh.max() < 3e-7
     assert dll.max() < 1e-3
 
