@@ -436,7 +436,7 @@ def bench_direct_ragged(args):
                    "cadence_s": args.direct_cadence,
                    "lengths_min": int(lengths.min()), "lengths_max": int(lengths.max()),
                    "lengths_sum": int(lengths.sum()), "dense_bins": int(f.size), "modes": 6, "tdi": "AET tdi2",
-                   "launch": {k: tuned[k] for k in ("chunk", "block", "grid")}, "graph_vs_eager": diff,
+                   "launch": {k: tuned[k] for k in ("chunk", "block", "grid")}, "max_abs_dlogL_graph_vs_eager": diff,
                    "dense_bin_templates": dense, **split, "fp32_bulk": f32_info, "gradient": grad_info,
                    "unfused_inner_product": unfused},
         "roofline": roofline,
@@ -636,7 +636,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded injection + Gaussian noise, analytic A/E/T PSD)",
         "config": {"workload": "heterodyned logL, fused FP64, hipGraph replay", "templates": B,
-                   "sparse_nodes": args.nodes, "eager_ms_per_step": ms_eager, "graph_vs_eager": graph_diff,
+                   "sparse_nodes": args.nodes, "eager_ms_per_step": ms_eager, "max_abs_dlogL_graph_vs_eager": graph_diff,
                    "dense_bins": 2**args.dense_bits, "modes": 6, "tdi": "AET tdi2",
                    "plain_fp64_path_evals_per_s": plain_rate, "end_to_end_get_ll_ms": e2e_ms,
                    "end_to_end_evals_per_s": B / (e2e_ms * 1e-3), "accuracy_vs_cpu_oracle": acc,
