@@ -21,8 +21,9 @@
 //   1. forward: residuals r = h / h0 on the K nodes (LDS) and d_h, h_h,
 //      exactly as the forward kernel;
 //   2. adjoint of the heterodyned sums: for each node and channel the complex
-//      weight g with  d logL = sum Re(g dh)  (from the segment formulas
-//      of Likelihood.cu:91-103 and r = h / h0), written out (48 K bytes);
+//      weight g with  d logL = sum Re(g dh)  (from the per-node weights of
+//      the segment formulas of Likelihood.cu:91-103 and r = h / h0), written
+//      out (48 K bytes);
 // k_grad_points  one 64-lane wavefront per (template, mode), lane per node:
 //   3. per (node, mode) tangents: PhenomHM amplitude, phase and t(f) with
 //      respect to the intrinsic parameters (structured tangents of the same
@@ -96,40 +97,25 @@ __global__ __launch_bounds__(64) void k_grad_setup(GradRec* __restrict__ grecs, 
     else grad_setup_part<2>(grecs[idx], params, B, b, l, m, qnm, table_len, dspin);
 }
 
-// Adjoint weights of the heterodyned sums: for node j and channel c,
-// g[c][j] = gamma / conj(h0) with d logL = sum_j,c Re(conj(gamma) dr) and
-// r = h / h0, so that d logL = sum Re(g dh) for h = sum_m conj(X_m).
-__device__ inline void adjoint_weights(cd* gs, const cd* rs, const double* __restrict__ freqs,
-                                       const cd* __restrict__ h0, const cd* __restrict__ dc, int K, int jend)
+// Adjoint weights of the heterodyned sums: for node n and channel c,
+// g[c][n] = gamma / conj(h0) with d logL = sum_n,c Re(conj(gamma) dr) and
+// r = h / h0, so that d logL = sum Re(g dh) for h = sum_m conj(X_m). In the
+// per-node form of the sums (NodeW), logL = Re d_h - Re h_h / 2 - d_d / 2 with
+// d_h = sum conj(r_n) W_n and Re h_h = sum |r_n|^2 Re V_n +
+// Re(r_n conj(r_{n-1})) Re U_n, so
+//   gamma_n = W_n - Re V_n r_n - (Re U_n r_{n-1} + Re U_{n+1} r_{n+1}) / 2
+// (U_n = 0 where segment n is not summed, so node 0 has no r_{-1} term), and
+// 1 / conj(h0) = conj(q).
+__device__ inline void adjoint_weights(cd* gs, const cd* rs, const NodeW* __restrict__ nw, int K)
 {
     for (int j = threadIdx.x; j < K; j += blockDim.x) {
-        cd gam[3] = {mk(0.0, 0.0), mk(0.0, 0.0), mk(0.0, 0.0)};
-        // segment s joins prev = s-1 and cur = s; node j is cur of s = j and prev of s = j+1
-        for (int side = 0; side < 2; side++) {
-            const int sgm = j + side;
-            if (sgm < 1 || sgm >= jend) continue;
-            const double fj = freqs[sgm], fprev = freqs[sgm - 1];
-            if (fprev == 0.0) continue;
-            const double inv = 1.0 / (fj - fprev);
-            const double fm = (fj + fprev) / 2.0;
-            const double w0 = side == 0 ? 1.0 - inv * (fj - fm) : inv * (fj - fm);  // d r0 / d r_j
-            const double w1 = side == 0 ? inv : -inv;                                // d r1 / d r_j
-            for (int c = 0; c < 3; c++) {
-                const cd cur = rs[c * K + sgm], prev = rs[c * K + sgm - 1];
-                const cd A0 = dc[(0 * 3 + c) * K + sgm], A1 = dc[(1 * 3 + c) * K + sgm];
-                const double B0 = dc[(2 * 3 + c) * K + sgm].re, B1 = dc[(3 * 3 + c) * K + sgm].re;
-                const cd r1 = mk((cur.re - prev.re) * inv, (cur.im - prev.im) * inv);
-                const cd r0 = cur - r1 * (fj - fm);
-                // d logL = Re(conj(g0) dr0 + conj(g1) dr1)
-                const cd g0 = A0 - B0 * r0 - B1 * r1;
-                const cd g1 = A1 - B1 * r0;
-                gam[c] += w0 * g0 + w1 * g1;
-            }
-        }
         for (int c = 0; c < 3; c++) {
-            const cd d = h0[c * K + j];
-            const double iden = rcp_pos(d.re * d.re + d.im * d.im);
-            gs[c * K + j] = mk((gam[c].re * d.re - gam[c].im * d.im) * iden, (gam[c].re * d.im + gam[c].im * d.re) * iden);
+            const NodeW& w = nw[c * K + j];
+            const cd r = rs[c * K + j];
+            cd gam = w.w - w.v.re * r;
+            if (j > 0) gam = gam - (0.5 * w.u.re) * rs[c * K + j - 1];
+            if (j + 1 < K) gam = gam - (0.5 * nw[c * K + j + 1].u.re) * rs[c * K + j + 1];
+            gs[c * K + j] = gam * conj(w.q);
         }
     }
 }
@@ -162,8 +148,8 @@ __global__ __launch_bounds__(64 * MAX_RESIDUAL_WAVES) __attribute__((amdgpu_wave
     double* __restrict__ d_h, double* __restrict__ h_h, double* __restrict__ ll, double d_d, cd* __restrict__ gw,
     const FusedRec* __restrict__ recs, const double* __restrict__ params, const double* __restrict__ t_start,
     const double* __restrict__ t_end, int B, ModeList ml, int M, const double* __restrict__ freqs,
-    const NodeW* __restrict__ nw, const cd* __restrict__ h0, const cd* __restrict__ dc, int K, resp::OrbitTab orb,
-    int TDItag, int rescaled, int tdi2, int jend, double initial_t_val)
+    const NodeW* __restrict__ nw, int K, resp::OrbitTab orb, int TDItag, int rescaled, int tdi2,
+    double initial_t_val)
 {
     extern __shared__ cd rs[];  // residuals [3][K]
     __shared__ double2 tab[129];
@@ -181,7 +167,7 @@ __global__ __launch_bounds__(64 * MAX_RESIDUAL_WAVES) __attribute__((amdgpu_wave
     __syncthreads();
     double sums[4];
     node_sums(rs, nw, K, sums);
-    adjoint_weights(gw + (long)b * 3 * K, rs, freqs, h0, dc, K, jend);
+    adjoint_weights(gw + (long)b * 3 * K, rs, nw, K);
     if (threadIdx.x == 0) {
         d_h[2 * b] = sums[0];
         d_h[2 * b + 1] = sums[1];
@@ -316,9 +302,7 @@ extern "C" int bbhx_hip_hdyn_fused_grad(double* d_h, double* h_h, double* grad, 
     if (timed) (void)hipEventRecord(ev[1], st);
     hipLaunchKernelGGL(k_grad_weights, dim3(numBinAll), dim3(64 * residual_waves(numBinAll, K)),
                        sizeof(cd) * 3 * (size_t)K, st, d_h, h_h, logL, d_d, gw, recs, params, t_start, t_end,
-                       numBinAll, ml, numModes, freqs, nw, reinterpret_cast<const cd*>(h0),
-                       reinterpret_cast<const cd*>(dataConstants), K, orb, TDItag, rescaled, tdi2, jend,
-                       initial_t_val);
+                       numBinAll, ml, numModes, freqs, nw, K, orb, TDItag, rescaled, tdi2, initial_t_val);
     if (hipStreamWaitEvent(st, side->join, 0) != hipSuccess) return ERR_LAUNCH;
     hipLaunchKernelGGL(k_grad_points, dim3(nrec), dim3(64), 0, st, partial, grecs, gw, params, t_start, t_end,
                        numBinAll, numModes, freqs, K, orb, TDItag, rescaled, tdi2, initial_t_val);

@@ -71,3 +71,64 @@ def test_node_form_reference_identity():
     d_h, h_h = node_form(r, dc, freqs, K - 1)
     assert abs(d_h[0] - dc[0][:, 1:K - 1].sum()) <= 1e-12 * np.abs(dc[0]).sum()
     assert abs(h_h[0] - dc[2][:, 1:K - 1].sum()) <= 1e-12 * np.abs(dc[2]).sum()
+
+
+def node_adjoint(r, dc, freqs, jend):
+    """gamma (B, 3, K) with d logL = sum Re(conj(gamma) dr), logL = Re d_h - Re h_h / 2,
+    from the per-node weights (csrc/hdyn_grad.hip, adjoint_weights)."""
+    K = len(freqs)
+    A0, A1, B0, B1 = dc
+    W = np.zeros((3, K), complex)
+    V = np.zeros((3, K))
+    U = np.zeros((3, K))
+    valid = [1 <= j < jend and freqs[j - 1] != 0.0 for j in range(K)]
+    for n in range(K):
+        if valid[n]:
+            inv = 1.0 / (freqs[n] - freqs[n - 1])
+            W[:, n] += 0.5 * A0[:, n] + A1[:, n] * inv
+            V[:, n] += (0.25 * B0[:, n] + B1[:, n] * inv).real
+            U[:, n] = (0.5 * B0[:, n]).real
+        if n + 1 < K and valid[n + 1]:
+            inv = 1.0 / (freqs[n + 1] - freqs[n])
+            W[:, n] += 0.5 * A0[:, n + 1] - A1[:, n + 1] * inv
+            V[:, n] += (0.25 * B0[:, n + 1] - B1[:, n + 1] * inv).real
+    gam = W[None] - V[None] * r
+    gam[:, :, 1:] -= 0.5 * U[None, :, 1:] * r[:, :, :-1]
+    gam[:, :, :-1] -= 0.5 * U[None, :, 1:] * r[:, :, 1:]
+    return gam
+
+
+@pytest.mark.parametrize("zero_first", [False, True])
+@pytest.mark.parametrize("include_last", [False, True])
+def test_node_adjoint_matches_segment_differences(zero_first, include_last):
+    """The adjoint weights of the per-node form against central differences of
+    the oracle's segment sums (logL is quadratic in r, so the differences are
+    exact up to rounding), along random directions and along single nodes at
+    both ends, where a missing neighbour term or a wrong end rule shows."""
+    rng = np.random.default_rng(11 + 2 * zero_first + include_last)
+    B, K = 3, 24
+    freqs = np.sort(rng.uniform(1e-4, 1e-1, K))
+    if zero_first:
+        freqs[0] = 0.0
+    r = rng.standard_normal((B, 3, K)) + 1j * rng.standard_normal((B, 3, K))
+    dc = rng.standard_normal((4, 3, K)) + 1j * rng.standard_normal((4, 3, K))
+
+    def loglike(rr):
+        res = rr.transpose((2, 1, 0)).flatten()
+        d_h, h_h = oracle.hdyn(res, dc.flatten(), freqs, B, K, include_last_segment=include_last)
+        return d_h.real - 0.5 * h_h.real
+
+    gam = node_adjoint(r, dc, freqs, K if include_last else K - 1)
+    dirs = [rng.standard_normal((B, 3, K)) + 1j * rng.standard_normal((B, 3, K)) for _ in range(3)]
+    for n in (0, 1, K - 2, K - 1):
+        for c in range(3):
+            for unit in (1.0, 1j):
+                e = np.zeros((B, 3, K), complex)
+                e[:, c, n] = unit
+                dirs.append(e)
+    eps = 1e-3
+    for dr in dirs:
+        fd = (loglike(r + eps * dr) - loglike(r - eps * dr)) / (2 * eps)
+        an = np.sum((np.conj(gam) * dr).real, axis=(1, 2))
+        scale = np.sum(np.abs(gam) * np.abs(dr), axis=(1, 2)) + 1e-300
+        assert np.max(np.abs(fd - an) / scale) <= 1e-9, (dr.nonzero(), fd, an)
