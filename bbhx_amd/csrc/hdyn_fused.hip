@@ -324,6 +324,41 @@ extern "C" int bbhx_hip_hdyn_fused_noalloc(double* d_h, double* h_h, const doubl
                              include_last_segment, logL, d_d, reinterpret_cast<hipStream_t>(stream), false);
 }
 
+namespace {
+
+// Angular rate of the constellation's guiding centre in the SSB-frame time
+// transform (utils/constants.py ConstOmega of the reference, used by
+// transform.py:75-105); not the orbit's 2 pi / year of the response.
+constexpr double OMEGA_LFRAME = 1.99098659277e-7;
+
+// SSB-frame observation window of each binary (waveformbuild.py:477-484):
+// t = tL + R cos(beta) cos(p) - Omega/2 (R cos(beta))^2 sin(2 p), p = Omega tL - lam,
+// R = 1 AU / c (tSSBfromLframe with t0 = 0, transform.py:75-105), at tL = tL0 and tL1.
+__global__ void k_window_times(double* __restrict__ ts, double* __restrict__ te, const double* __restrict__ lam,
+                               const double* __restrict__ beta, int B, double tL0, double tL1)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
+    const double rc = (AU_SI / C_SI) * cos(beta[b]);
+    const double c2 = (0.5 * OMEGA_LFRAME) * (rc * rc);
+    const double l = lam[b];
+    const double p0 = OMEGA_LFRAME * tL0 - l, p1 = OMEGA_LFRAME * tL1 - l;
+    ts[b] = (tL0 + rc * cos(p0)) - c2 * sin(2.0 * p0);
+    te[b] = (tL1 + rc * cos(p1)) - c2 * sin(2.0 * p1);
+}
+
+}  // namespace
+
+extern "C" int bbhx_hip_window_times(double* t_start, double* t_end, const double* lam, const double* beta,
+                                     int numBinAll, double tL_start, double tL_end, void* stream)
+{
+    if (numBinAll < 0 || (numBinAll > 0 && (!t_start || !t_end || !lam || !beta))) return ERR_ARGS;
+    if (numBinAll == 0) return OK;
+    hipLaunchKernelGGL(k_window_times, dim3(ceil_div(numBinAll, 256)), dim3(256), 0,
+                       reinterpret_cast<hipStream_t>(stream), t_start, t_end, lam, beta, numBinAll, tL_start, tL_end);
+    return hipGetLastError() == hipSuccess ? OK : ERR_LAUNCH;
+}
+
 extern "C" int bbhx_hip_release_workspace(void)
 {
     workspace_release_all();

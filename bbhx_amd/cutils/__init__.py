@@ -96,6 +96,7 @@ def load_library():
                                        C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _vp],
         "bbhx_hip_direct_plan_enable_grad": [_vp],
         "bbhx_hip_direct_plan_set_precision": [_vp, C.c_int],
+        "bbhx_hip_window_times": [_vp, _vp, _vp, _vp, C.c_int, C.c_double, C.c_double, _vp],
         "bbhx_hip_release_workspace": [],
         "bbhx_hip_version": [],
         "bbhx_hip_timing_enable": [C.c_int],
@@ -120,7 +121,8 @@ def exported_symbols():
         "bbhx_hip_hdyn_fused_grad", "bbhx_hip_direct_plan_create", "bbhx_hip_direct_plan_destroy",
         "bbhx_hip_direct_plan_bytes", "bbhx_hip_direct_plan_band", "bbhx_hip_direct_fused", "bbhx_hip_hdyn_prep",
         "bbhx_hip_direct_fused_grad", "bbhx_hip_direct_plan_enable_grad", "bbhx_hip_direct_plan_set_precision",
-        "bbhx_hip_release_workspace", "bbhx_hip_version", "bbhx_hip_timing_enable", "bbhx_hip_timing_read",
+        "bbhx_hip_window_times", "bbhx_hip_release_workspace", "bbhx_hip_version", "bbhx_hip_timing_enable",
+        "bbhx_hip_timing_read",
     ]
 
 
@@ -524,6 +526,21 @@ def hdyn_prep(data_constants, totals, data_freqs, data_channels, h0, psd, nodes,
     _check(st, "hdyn_prep")
 
 
+def window_times(t_start, t_end, lam, beta, tL_start, tL_end):
+    """SSB-frame window (t_start, t_end) of each binary from device rows lam,
+    beta (tSSBfromLframe at the LISA-frame times tL_start, tL_end in seconds),
+    written on the device on the current stream (no reference counterpart in
+    BBHxBackendMethods; see include/bbhx_hip.h)."""
+    torch = _torch()
+    f64 = torch.float64
+    B = int(lam.numel())
+    if beta.numel() != B or t_start.numel() != B or t_end.numel() != B:
+        raise ValueError("lam, beta, t_start and t_end must have one entry per binary")
+    st = load_library().bbhx_hip_window_times(_dev(t_start, f64), _dev(t_end, f64), _dev(lam, f64), _dev(beta, f64),
+                                              B, float(tL_start), float(tL_end), _stream())
+    _check(st, "window_times")
+
+
 def timing_enable(on=True):
     """Start (or stop) event timing of the fused heterodyne launches."""
     _check(load_library().bbhx_hip_timing_enable(int(bool(on))), "timing_enable")
@@ -586,6 +603,7 @@ class BBHxHipBackend:
         self.direct_fused = direct_fused
         self.DirectPlan = DirectPlan
         self.hdyn_prep = hdyn_prep
+        self.window_times = window_times
         self.direct_fused_grad = direct_fused_grad
         self.device = torch.device("cuda", torch.cuda.current_device())
 

@@ -15,6 +15,7 @@ import torch
 
 from .utils.arrays import as_np1d, to_dev, to_host
 from .utils.parallelbase import BBHxParallelModule
+from .utils.constants import YRSID_SI
 from .utils.sensitivity import AET1SensitivityMatrix
 
 
@@ -200,13 +201,37 @@ def fused_param_rows(template_gen, params, t_obs_start, t_obs_end, device):
                          "beta, psi, t_ref)")
     m1, m2, c1, c2 = p[0].copy(), p[1].copy(), p[2].copy(), p[3].copy()
     template_gen.amp_phase_gen._sanity_check_params(m1, m2, c1, c2)
-    t_start, t_end = template_gen.time_window(p[8], p[9], t_obs_start, t_obs_end)
-    # one host-to-device copy: the 11 rows, t_start and t_end as one (13, B) block
     B = p.shape[1]
-    block = np.stack([m1, m2, c1, c2, p[4], p[5], p[7], p[8], p[9], p[10], p[11],
-                      np.broadcast_to(t_start, (B,)), np.broadcast_to(t_end, (B,))])
+    rows = [m1, m2, c1, c2, p[4], p[5], p[7], p[8], p[9], p[10], p[11]]
+    if np.isscalar(t_obs_start) and np.isscalar(t_obs_end):
+        # one host-to-device copy of the 11 rows; the window from lam, beta on
+        # the device (the reference evaluates it with its array module there)
+        dev = to_dev(np.stack(rows), device)
+        t_start = torch.empty(B, dtype=torch.float64, device=dev.device)
+        t_end = torch.empty(B, dtype=torch.float64, device=dev.device)
+        template_gen.backend.window_times(t_start, t_end, dev[7], dev[8], float(t_obs_start) * YRSID_SI,
+                                          float(t_obs_end) * YRSID_SI)
+        return dev, t_start, t_end
+    # per-binary windows: evaluated with the rows on the host (time_window
+    # checks their types), sent as one (13, B) block
+    t_start, t_end = template_gen.time_window(p[8], p[9], t_obs_start, t_obs_end)
+    block = np.stack(rows + [np.broadcast_to(t_start, (B,)), np.broadcast_to(t_end, (B,))])
     dev = to_dev(block, device)
     return dev[:11], dev[11], dev[12]
+
+
+def sampling_rows(x):
+    """(11, B) parameter rows of the fused kernels from sampling coordinates x
+    (B, 11) (as ``sampling_rows_device``), with torch operations on x's device:
+    m1 >= m2 enforced with the spins swapped along (phenomhm.py:175-199)."""
+    M = torch.exp(x[:, 0])
+    q = x[:, 1]
+    m1, m2 = M * q / (1 + q), M / (1 + q)
+    c1, c2 = x[:, 2], x[:, 3]
+    swap = m1 < m2
+    return torch.stack([torch.where(swap, m2, m1), torch.where(swap, m1, m2), torch.where(swap, c2, c1),
+                        torch.where(swap, c1, c2), x[:, 4], x[:, 5], torch.arccos(x[:, 6]), x[:, 7],
+                        torch.arcsin(x[:, 8]), x[:, 9], x[:, 10]]).contiguous()
 
 
 def sampling_rows_device(x, t_obs_start=0.0, t_obs_end=1.0):
@@ -214,30 +239,17 @@ def sampling_rows_device(x, t_obs_start=0.0, t_obs_end=1.0):
     (ln(m1 + m2), q = m1/m2, chi1z, chi2z, distance, phi_ref, cos inc, lam,
     sin beta, psi, t_ref; a float64 tensor on the GPU): (11, B) rows with
     m1 >= m2 enforced (phenomhm.py:175-199) and the SSB-frame window
-    (waveformbuild.py:477-484), computed with torch operations on the device
-    only, so that nothing waits for the host and the call can sit inside a
-    captured graph. Unlike ``fused_param_rows`` the spins are not checked on
+    (waveformbuild.py:477-484, the library's window kernel), computed on the
+    device only, so that nothing waits for the host and the call can sit
+    inside a captured graph. Unlike ``fused_param_rows`` the spins are not checked on
     the host (|chi| <= 1 is the caller's to keep)."""
-    from .utils.constants import AU_SI, C_SI, YRSID_SI, ConstOmega
+    from .cutils import window_times
 
-    M = torch.exp(x[:, 0])
-    q = x[:, 1]
-    m1, m2 = M * q / (1 + q), M / (1 + q)
-    c1, c2 = x[:, 2], x[:, 3]
-    swap = m1 < m2
-    lam = x[:, 7]
-    beta = torch.arcsin(x[:, 8])
-    rows = torch.stack([torch.where(swap, m2, m1), torch.where(swap, m1, m2), torch.where(swap, c2, c1),
-                        torch.where(swap, c1, c2), x[:, 4], x[:, 5], torch.arccos(x[:, 6]), lam, beta, x[:, 9],
-                        x[:, 10]]).contiguous()
-    RoC = AU_SI / C_SI
-
-    def t_ssb(tL):  # utils/transform.py tSSBfromLframe with t0 = 0
-        ph = ConstOmega * tL - lam
-        rc = RoC * torch.cos(beta)
-        return tL + rc * torch.cos(ph) - 0.5 * ConstOmega * rc * rc * torch.sin(2.0 * ph)
-
-    return rows, t_ssb(t_obs_start * YRSID_SI).contiguous(), t_ssb(t_obs_end * YRSID_SI).contiguous()
+    rows = sampling_rows(x)
+    t_start = torch.empty(rows.shape[1], dtype=torch.float64, device=rows.device)
+    t_end = torch.empty_like(t_start)
+    window_times(t_start, t_end, rows[7], rows[8], float(t_obs_start) * YRSID_SI, float(t_obs_end) * YRSID_SI)
+    return rows, t_start, t_end
 
 
 class HeterodynedLikelihood(BBHxParallelModule):

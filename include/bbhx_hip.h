@@ -299,6 +299,16 @@ int bbhx_hip_hdyn_prep(double* dataConstants, double* totals, const double* data
                        const double* h0, const double* psd, int data_length, const double* nodes, int K, double df,
                        void* stream);
 
+/* SSB-frame observation window of each binary, as BBHWaveformFD computes it
+ * before the template (waveformbuild.py:477-484, tSSBfromLframe of
+ * transform.py:75-105 with t0 = 0; the reference evaluates it with the array
+ * module on the device, so its FFI has no entry for it). lam, beta (B,) device
+ * arrays (the parameter rows of the fused entry points); tL_start, tL_end the
+ * LISA-frame window in seconds (t_obs_start, t_obs_end times YRSID_SI).
+ * Writes t_start, t_end (B,) on the device, asynchronously on the stream. */
+int bbhx_hip_window_times(double* t_start, double* t_end, const double* lam, const double* beta, int numBinAll,
+                          double tL_start, double tL_end, void* stream);
+
 /* Event timing of the fused entry points on the caller's stream, split in two:
  * set-up (records; for the direct form also grids, splines and bands) and
  * evaluation. bbhx_hip_hdyn_fused, bbhx_hip_hdyn_fused_grad and

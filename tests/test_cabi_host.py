@@ -82,6 +82,9 @@ def test_version_and_argument_errors_need_no_gpu():
     assert lib.bbhx_hip_hdyn_prep(None, None, None, None, None, None, 100, None, 1, 1e-8, None) == 1
     assert lib.bbhx_hip_hdyn_prep(None, None, None, None, None, None, 100, None, 8, 0.0, None) == 1
     assert lib.bbhx_hip_hdyn_prep(None, None, None, None, None, None, 100, None, 8, 1e-8, None) == 1
+    # window times: missing arrays, negative count
+    assert lib.bbhx_hip_window_times(None, None, None, None, 5, 0.0, 1.0, None) == 1
+    assert lib.bbhx_hip_window_times(None, None, None, None, -1, 0.0, 1.0, None) == 1
 
 
 def test_registry_without_gpu_fails_loudly(monkeypatch):
@@ -210,14 +213,12 @@ def test_qnm_table_file_matches_data_module():
     np.testing.assert_allclose(np.diff(r.QNMData_a), r.dspin, rtol=1e-9)
 
 
-def test_sampling_rows_device_matches_host_rows():
+def test_sampling_rows_match_host_rows():
     """The device-side parameter rows (torch operations, here on CPU tensors)
     equal the host path's (numpy) rows to rounding, with the m1 >= m2 swap."""
     import torch
 
-    from bbhx_amd.likelihood import HeterodynedLikelihood, sampling_rows_device
-    from bbhx_amd.utils.transform import tSSBfromLframe
-    from bbhx_amd.utils.constants import YRSID_SI
+    from bbhx_amd.likelihood import HeterodynedLikelihood, sampling_rows
 
     rng = np.random.default_rng(8)
     B = 6
@@ -225,13 +226,11 @@ def test_sampling_rows_device_matches_host_rows():
                   rng.uniform(1e25, 1e26, B), rng.uniform(0, 6, B), rng.uniform(-0.99, 0.99, B), rng.uniform(0, 6, B),
                   rng.uniform(-0.99, 0.99, B), rng.uniform(0, 3, B), rng.uniform(1e6, 3e7, B)], axis=1)
     x[0, 1] = 0.5  # q < 1: the swap
-    rows, ts, te = sampling_rows_device(torch.as_tensor(x))
+    rows = sampling_rows(torch.as_tensor(x))
     p = HeterodynedLikelihood.sampling_to_params(x)
     m1, m2, c1, c2 = p[0].copy(), p[1].copy(), p[2].copy(), p[3].copy()
     sw = m1 < m2
     m1[sw], m2[sw], c1[sw], c2[sw] = m2[sw], m1[sw], c2[sw], c1[sw]
     want = np.stack([m1, m2, c1, c2, p[4], p[5], p[7], p[8], p[9], p[10], p[11]])
     np.testing.assert_allclose(rows.numpy(), want, rtol=1e-15, atol=0)
-    np.testing.assert_allclose(ts.numpy(), tSSBfromLframe(0.0, p[8], p[9]), rtol=1e-15, atol=1e-15)
-    np.testing.assert_allclose(te.numpy(), tSSBfromLframe(YRSID_SI, p[8], p[9]), rtol=1e-15, atol=0)
     assert rows.numpy()[0, 0] >= rows.numpy()[1, 0]

@@ -53,6 +53,42 @@ def orbits():
     return EqualArmlengthOrbits(dt=1e4, t_max=3.0 * YRSID_SI)
 
 
+def test_window_times_kernel(backend):
+    """SSB-frame window of each binary on the device (bbhx_hip_window_times)
+    against the host transform (tSSBfromLframe, restating transform.py:75-105)
+    at two LISA-frame windows, and the fused entry points' host inputs built
+    with it equal to those built with the host window. A wrong sign, factor or
+    term in the formula moves t by 1e-2 s or more; rounding alone by < 1e-7 s."""
+    import torch
+
+    from bbhx_amd.utils.transform import tSSBfromLframe
+
+    rng = np.random.default_rng(21)
+    B = 1000
+    lam = rng.uniform(0.0, 2 * np.pi, B)
+    beta = rng.uniform(-np.pi / 2, np.pi / 2, B)
+    ts = torch.empty(B, dtype=torch.float64, device="cuda")
+    te = torch.empty_like(ts)
+    for t0, t1 in ((0.0, 1.0), (0.25, 3.5)):
+        backend.window_times(ts, te, dev(lam), dev(beta), t0 * YRSID_SI, t1 * YRSID_SI)
+        np.testing.assert_allclose(host(ts), tSSBfromLframe(t0 * YRSID_SI, lam, beta), rtol=0, atol=1e-6)
+        np.testing.assert_allclose(host(te), tSSBfromLframe(t1 * YRSID_SI, lam, beta), rtol=0, atol=1e-6)
+    empty = torch.empty(0, dtype=torch.float64, device="cuda")
+    backend.window_times(empty, empty, empty, empty, 0.0, 1.0)
+
+    from bbhx_amd.likelihood import fused_param_rows
+    from bbhx_amd.waveformbuild import BBHWaveformFD
+
+    gen = BBHWaveformFD(amp_phase_kwargs=dict(run_phenomd=False), response_kwargs=dict(TDItag="AET"))
+    p = draw(16, 5)
+    rows, t_start, t_end = fused_param_rows(gen, p, 0.0, 1.0, "cuda")
+    # per-binary windows take the host path: the same values to rounding
+    rows_h, ts_h, te_h = fused_param_rows(gen, p, np.zeros(16), np.ones(16), "cuda")
+    np.testing.assert_array_equal(host(rows), host(rows_h))
+    np.testing.assert_allclose(host(t_start), host(ts_h), rtol=0, atol=1e-6)
+    np.testing.assert_allclose(host(t_end), host(te_h), rtol=0, atol=1e-6)
+
+
 def test_ringdown_frequencies(backend):
     import torch
 
