@@ -1055,22 +1055,22 @@ __global__ __launch_bounds__(64) void k_dir_route(int* __restrict__ route, const
     const double f0 = freqs[0], f1 = freqs[N - 1];
     const double idf = f1 > f0 ? (N - 1) / (f1 - f0) : 0.0;
     double s = 0.0;
-    for (int m = 0; m < M; m++) {
-        for (int i = lane; i + 1 < L; i += 64) {
-            const long p = (base + i) * M + m;
-            const double x = X[p], xn = X[p + M];
-            if (!(x >= f0 && x <= f1 && xn > x)) continue;
-            const double* y = Yc + p * NQ;
-            const double tf = y[2];
-            if (!((tf >= ts) && ((tf <= te) || (te <= 0.0)))) continue;
-            const int k = min(N - 1, max(0, (int)((x - f0) * idf + 0.5)));
-            double t = 0.0;
-            for (int ch = 0; ch < 3; ch++) {
-                const double w = wts[(long)ch * N + k];
-                t += (y[3 + 2 * ch] * y[3 + 2 * ch] + y[4 + 2 * ch] * y[4 + 2 * ch]) * (w * w);
-            }
-            s += (y[0] * y[0]) * t * ((fmin(xn, f1) - x) * idf);
+    // entry j = i M + m (knot i of mode m): consecutive lanes read consecutive
+    // (knot, mode) entries of X and Yc
+    for (int j = lane; j < (L - 1) * M; j += 64) {
+        const long p = base * M + j;
+        const double x = X[p], xn = X[p + M];
+        if (!(x >= f0 && x <= f1 && xn > x)) continue;
+        const double* y = Yc + p * NQ;
+        const double tf = y[2];
+        if (!((tf >= ts) && ((tf <= te) || (te <= 0.0)))) continue;
+        const int k = min(N - 1, max(0, (int)((x - f0) * idf + 0.5)));
+        double t = 0.0;
+        for (int ch = 0; ch < 3; ch++) {
+            const double w = wts[(long)ch * N + k];
+            t += (y[3 + 2 * ch] * y[3 + 2 * ch] + y[4 + 2 * ch] * y[4 + 2 * ch]) * (w * w);
         }
+        s += (y[0] * y[0]) * t * ((fmin(xn, f1) - x) * idf);
     }
     s = wsum(s);
     if (lane == 0) route[b] = 4.0 * s > FP32_FALLBACK_HH ? 1 : 0;
