@@ -327,13 +327,14 @@ extern "C" int bbhx_hip_hdyn_fused_noalloc(double* d_h, double* h_h, const doubl
 namespace {
 
 // Angular rate of the constellation's guiding centre in the SSB-frame time
-// transform (utils/constants.py ConstOmega of the reference, used by
-// transform.py:75-105); not the orbit's 2 pi / year of the response.
+// transform (ConstOmega, utils/constants.py:48, used by transform.py:75-105);
+// not the orbit's 2 pi / year of the response.
 constexpr double OMEGA_LFRAME = 1.99098659277e-7;
 
 // SSB-frame observation window of each binary (waveformbuild.py:477-484):
 // t = tL + R cos(beta) cos(p) - Omega/2 (R cos(beta))^2 sin(2 p), p = Omega tL - lam,
-// R = 1 AU / c (tSSBfromLframe with t0 = 0, transform.py:75-105), at tL = tL0 and tL1.
+// R = 1 AU / c (OrbitR / C_SI, transform.py:25, :99; tSSBfromLframe with t0 = 0,
+// transform.py:75-105), at tL = tL0 and tL1.
 __global__ void k_window_times(double* __restrict__ ts, double* __restrict__ te, const double* __restrict__ lam,
                                const double* __restrict__ beta, int B, double tL0, double tL1)
 {
