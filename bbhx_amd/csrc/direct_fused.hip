@@ -60,6 +60,7 @@ struct bbhx_direct_plan {
     double* X = nullptr;   // (S, M)
     double* Cf = nullptr;  // (S, M, RS): segment records (bounds, value and cubic coefficients)
     double* Yc = nullptr;  // (S, M, NQ): the grid pass's nine values per knot, compact
+    double* Fs = nullptr;  // (S, M, NQ + 1): the spline's forward sweep, d' of each quantity, then c'
     double* grid_lo = nullptr;    // (B, M) log2 of the first refitted grid point
     double* grid_istep = nullptr; // (B, M) 1 / log2 step (0 for a degenerate grid)
     int* band = nullptr;          // (B, 3): st, et, chunk_begin
@@ -285,7 +286,7 @@ constexpr int DSPL_BLOCK = 8;  // rows loaded ahead of the recurrence in each sw
 // Thomas algorithm over the binary's L knots (the system of spline.hip);
 // consecutive lanes read consecutive quantities of the same knot.
 __global__ __launch_bounds__(64) void k_dir_spline(const double* __restrict__ X, const double* __restrict__ Yc,
-                                                   double* __restrict__ Cf,
+                                                   double* __restrict__ Cf, double* __restrict__ Fs,
                                                    SegRecF* __restrict__ CfF, const long* __restrict__ off,
                                                    const int* __restrict__ len, int M)
 {
@@ -300,8 +301,9 @@ __global__ __launch_bounds__(64) void k_dir_spline(const double* __restrict__ X,
     const int q = s - m * NQ;
     auto at = [&](int i) { return ((base + i) * M + m) * RS + RQ + q * 4; };
     auto yi = [&](int i) { return Yc[((base + i) * M + m) * NQ + q]; };
-    const double* dp = Cf + 1;  // d' (the forward sweep's, read back)
-    const double* cp = Cf + 3;  // c'
+    // the forward sweep's d' of this quantity and the mode's c' (the same for
+    // its nine quantities: it depends on the knots alone), NQ + 1 per (knot, mode)
+    auto fs = [&](int i) { return ((base + i) * M + m) * (NQ + 1); };
     // the FP32-bulk record of point i (CfF != nullptr): this lane's quantity
     auto put_f = [&](int i, double x_lo, double x_hi, double c0, double c1, double c2, double c3) {
         SegRecF& rf = CfF[(base + i) * M + m];
@@ -328,15 +330,12 @@ __global__ __launch_bounds__(64) void k_dir_spline(const double* __restrict__ X,
         }
     };
 
-    // The forward sweep keeps its values (d' in slot c1, c' in slot c3) in the
-    // records and writes each quantity's whole 32-B group (with the knot value
-    // and a zero c2) and the bounds: every line of the records is then fully
-    // written, so no partially written line leaves the cache to be merged in
-    // memory; the backward sweep overwrites all of it with the final values.
-    auto put_fwd = [&](int i, double x_lo, double x_hi, double y, double d, double c) {
-        *reinterpret_cast<Cubic*>(Cf + at(i)) = Cubic{y, d, 0.0, c};
-        if (q == 0)
-            *reinterpret_cast<Cubic*>(Cf + ((base + i) * M + m) * RS) = Cubic{x_lo, x_hi, 0.0, 0.0};
+    // The forward sweep writes only d' and c' (80 B per (knot, mode), the
+    // wavefront's lanes consecutive) to the scratch Fs; the backward sweep
+    // writes every record once, in whole 32-B groups.
+    auto put_fwd = [&](int i, double d, double c) {
+        Fs[fs(i) + q] = d;
+        if (q == 0) Fs[fs(i) + NQ] = c;
     };
     double x0 = xi(0), x1 = xi(1), x2 = xi(2);
     double y0 = yi(0), y1 = yi(1), y2 = yi(2);
@@ -347,7 +346,7 @@ __global__ __launch_bounds__(64) void k_dir_spline(const double* __restrict__ X,
         double bb = ((dx1 + 2 * dd) * dx2 * s1 + dx1 * dx1 * s2) / dd;
         cprev = dd / dx2;
         dprev = bb / dx2;
-        put_fwd(0, x0, x1, y0, dprev, cprev);
+        put_fwd(0, dprev, cprev);
     }
     // each sweep loads DSPL_BLOCK rows ahead of its recurrence (one memory
     // latency per block) and divides once per row
@@ -372,7 +371,7 @@ __global__ __launch_bounds__(64) void k_dir_spline(const double* __restrict__ X,
             const double im = 1.0 / (diag - lo * cprev);
             cprev = up * im;
             dprev = (rhs - lo * dprev) * im;
-            put_fwd(i, xb, xc, yb, dprev, cprev);
+            put_fwd(i, dprev, cprev);
             xa = xb;
             xb = xc;
             ya = yb;
@@ -398,8 +397,8 @@ __global__ __launch_bounds__(64) void k_dir_spline(const double* __restrict__ X,
 #pragma unroll
       for (int u = 0; u < DSPL_BLOCK; u++) {
           const int i = i0 - u;
-          dv[u] = i >= 0 ? dp[at(i)] : 0.0;
-          cv[u] = i >= 0 ? cp[at(i)] : 0.0;
+          dv[u] = i >= 0 ? Fs[fs(i) + q] : 0.0;
+          cv[u] = i >= 0 ? Fs[fs(i) + NQ] : 0.0;
           xv[u] = i >= 0 ? xi(i) : 0.0;
           yv[u] = i >= 0 ? yi(i) : 0.0;
       }
@@ -1341,7 +1340,7 @@ T* dalloc(size_t n, bool& ok)
 
 void plan_free(bbhx_direct_plan* p)
 {
-    void* bufs[] = {p->off, p->len, p->recs, p->X, p->Cf, p->Yc, p->grid_lo, p->grid_istep, p->band,
+    void* bufs[] = {p->off, p->len, p->recs, p->X, p->Cf, p->Yc, p->Fs, p->grid_lo, p->grid_istep, p->band,
                     p->total, p->partial, p->perm, p->cnt, p->next, p->Yadj, p->grecs, p->gpart, p->CfF,
                     p->loud, p->quiet, p->loud_cnt, p->route, p->split_cnt};
     for (void* b : bufs)
@@ -1388,6 +1387,7 @@ extern "C" int bbhx_hip_direct_plan_create(bbhx_direct_plan** out, int numBinAll
     p->X = dalloc<double>(SM, ok);
     p->Cf = dalloc<double>(SM * RS, ok);
     p->Yc = dalloc<double>(SM * NQ, ok);
+    p->Fs = dalloc<double>(SM * (NQ + 1), ok);
     p->grid_lo = dalloc<double>((size_t)numBinAll * numModes, ok);
     p->grid_istep = dalloc<double>((size_t)numBinAll * numModes, ok);
     p->band = dalloc<int>((size_t)numBinAll * 3, ok);
@@ -1425,7 +1425,7 @@ extern "C" long bbhx_hip_direct_plan_bytes(const bbhx_direct_plan* p)
 {
     if (!p) return -1;
     const long SM = p->S * p->M;
-    long bytes = SM * 8 * (1 + RS + NQ) + (long)p->B * p->M * (sizeof(FusedRec) + 16) + p->max_items * 36 + p->B * 24 +
+    long bytes = SM * 8 * (1 + RS + NQ + NQ + 1) + (long)p->B * p->M * (sizeof(FusedRec) + 16) + p->max_items * 36 + p->B * 24 +
                  4L * (p->max_chunks + 1);
     if (p->Yadj) bytes += SM * 8 * NA + (long)p->B * p->M * (sizeof(bbhx::gradk::GradRec) + 8 * bbhx::gradk::NGRAD);
     if (p->CfF) bytes += SM * (long)sizeof(SegRecF) + p->max_items * 8 + 16 + p->B * 4L + 4L * (p->max_items / 256 + 1);
@@ -1484,7 +1484,7 @@ int direct_launch(bbhx_direct_plan* p, double* d_h, double* h_h, double* grad, i
                        orb, TDItag, rescaled, tdi2, initial_t_val, mf_min, mf_max);
     // the FP32-bulk form (not for the gradient, whose adjoint pass reads the FP64 records)
     const bool f32 = !grad && p->precision == 1;
-    hipLaunchKernelGGL(k_dir_spline, dim3(B), dim3(64), 0, st, p->X, p->Yc, p->Cf,
+    hipLaunchKernelGGL(k_dir_spline, dim3(B), dim3(64), 0, st, p->X, p->Yc, p->Cf, p->Fs,
                        f32 ? static_cast<SegRecF*>(p->CfF) : nullptr, p->off, p->len, M);
     hipLaunchKernelGGL(k_dir_band, dim3(1), dim3(1024), 0, st, p->band, p->total, p->X, p->off, p->len, dataFreqs,
                        p->N, B, M, chunk);
