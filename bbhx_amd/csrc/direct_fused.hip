@@ -18,7 +18,8 @@
 //                 Response.cu), written compact: nine values per (knot, mode)
 //   k_dir_spline  one wavefront per binary, one lane per (mode, quantity):
 //                 not-a-knot spline coefficients (Interpolate.cu:15-359) from
-//                 the compact values, each record's (y, c1, c2, c3) written together
+//                 the compact values (forward sweep into the scratch Fs, then
+//                 each record's (y, c1, c2, c3) written together, once)
 //   k_dir_band    dense band [st, et) of each binary (waveformbuild.py:139-156)
 //                 and the exclusive prefix of its chunks of dense bins (chunks
 //                 aligned to absolute multiples of the chunk size)
